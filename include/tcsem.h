@@ -1,0 +1,187 @@
+/*
+ * tcsem.h -- C-ABI of the MI355X-native TCSEM_FDTD time-stepping library (libtcsem.so).
+ *
+ * The library advances the DuFort-Frankel scheme of xltu/TCSEM_FDTD on one AMD
+ * Instinct MI355X (gfx950): E update, source injection, B update with the
+ * div(B)=0 Bz integral, FFT upward continuation of Bz at the sea surface, and
+ * receiver sampling.  It reads and writes the reference's own driver files.
+ *
+ * Every entry point returns 0 on success or a negative TCSEM_E* code; none of
+ * them exits the process.  tcsem_last_error() returns a message for the last
+ * failure on the calling thread.  Arrays are plain host pointers owned by the
+ * caller; a context owns all device memory.  One context is driven by one host
+ * thread at a time.
+ *
+ * Reference interfaces each group replaces (file:line in xltu/TCSEM_FDTD):
+ *   file surface     FDTD_setup.in  ModelingSetup.cpp:51-59
+ *                    Cond.bin       ModelingSetup.cpp:201-209
+ *                    Receiver.in    LoadRxTxPos.cpp:51-84
+ *                    Transmitter.in LoadRxTxPos.cpp:134-200
+ *                    Tx<id>.bout    main.cu:280-291
+ *   host setup       ModelingSetup.cpp:100-192, ModelGridMapping.cpp:13-395,
+ *                    SourceWave.cpp:8-58, LoadRxTxPos.cpp:93-460, main.cu:149-165
+ *   device setup     prepare_free_mem.cu:45-239, initialize_UpCont_gpu_resources.cu:146-341,
+ *                    initialize_fft.cu:43-156
+ *   step API         UpContinuation_gpu  FTDT.h:317-318 (UpContinuation_gpu.cu:166-274)
+ *                    ElectricalField     FTDT.h:296     (ElectricalFiled.cu:478-655)
+ *                    Bderivate           FTDT.h:291     (Bderivate.cu:269-316)
+ *                    EFInterp2Rx         FTDT.h:342-344 (EFInterp2Rx.cu:472-585)
+ *   driver loop      main.cu:195-298
+ */
+#ifndef TCSEM_H
+#define TCSEM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------------------------ */
+#define TCSEM_OK 0
+#define TCSEM_EINVAL (-1)     /* bad argument or inconsistent sizes                           */
+#define TCSEM_EDOMAIN (-2)    /* receiver / transmitter outside the model                     */
+#define TCSEM_EIO (-3)        /* file open / parse / write failure                            */
+#define TCSEM_ENOMEM (-4)     /* host or device allocation failed                             */
+#define TCSEM_EDEVICE (-5)    /* HIP runtime error or no usable gfx950 device                 */
+#define TCSEM_ESIZE (-6)      /* size beyond a supported limit                                */
+#define TCSEM_ESTATE (-7)     /* call out of order (e.g. run before transmitters were set)    */
+
+/* ---- precision of the field state --------------------------------------------------------- */
+#define TCSEM_FP64 64         /* fields, coefficients and sums in FP64 (reference precision)  */
+#define TCSEM_FP32 32         /* FP32 fields; FP64 coefficients, schedule, air-plane FFT;
+                                 compensated FP32 Bz integral                                  */
+
+/* ---- field component ids (tcsem_get_field / tcsem_set_field) ------------------------------ */
+#define TCSEM_EX 0            /* (N+1, M+1, L)   x fastest */
+#define TCSEM_EY 1            /* (N+1, M,   L+1) */
+#define TCSEM_EZ 2            /* (N,   M+1, L+1) */
+#define TCSEM_BX 3            /* (N,   M,   L+1) */
+#define TCSEM_BY 4            /* (N,   M+1, L)   */
+#define TCSEM_BZ 5            /* (N+1, M,   L)   derived: bottom-up div(B)=0 integral of Bx, By */
+#define TCSEM_BXAIR 6         /* (M,   L+1)      Bx at z=-dz0/2 from the last upward continuation */
+#define TCSEM_BYAIR 7         /* (M+1, L)        By at z=-dz0/2 */
+
+/* FDTD_setup.in, in file order (ModelingSetup.cpp:51-59). */
+typedef struct tcsem_setup {
+  double sfa1, sfa2;                 /* line 1: time-step speed-up factor, lambda stabiliser   */
+  int n_src;                         /* line 2: Gaussian discretisation count (N_Src_Steps = n_src/2) */
+  double sigma_t;                    /* line 2: Gaussian width [s]                             */
+  double x0, y0, z0;                 /* line 3: grid origin                                    */
+  double dxmin, dymin, dzmin;        /* line 4: core cell sizes [m]                            */
+  double arg1, arg2;                 /* line 5: geometric growth factors                       */
+  int ALx2, ALx1, Nx, ARx1, ARx2;    /* line 6: x zone cell counts, outer-left first           */
+  int Ay2, Ay1, Ny;                  /* line 7: y zones (symmetric)                            */
+  int Az2, Az1, Nz;                  /* line 8: z zones (file order Az2 Az1 Nz)                */
+} tcsem_setup;
+
+/* Receiver.in lines 1-3 (LoadRxTxPos.cpp:51-55). */
+typedef struct tcsem_rxtime {
+  double T0, Tn;                     /* recording window [s]                                   */
+  int Nt;                            /* samples per component                                  */
+  int is_log;                        /* 1: log-spaced sample times, 0: linear                  */
+  int is_comp[6];                    /* record Ex Ey Ez Bx By Bz                               */
+} tcsem_rxtime;
+
+/* One Transmitter.in row (LoadRxTxPos.cpp:151-178).  ort 1-3: x/y/z bipole; ort 4: one bipole
+ * with azimuth/dip; ort 5: two bipoles.  Angles are in degrees as in the file.  rx_ids holds nr
+ * 1-based receiver ids that record this transmitter. */
+typedef struct tcsem_tx {
+  int id, ort;
+  double bipolen, x, y, z, azi_deg, dip_deg;
+  double bipolen1, x1, y1, z1, azi1_deg, dip1_deg;
+  int nr;
+  const int *rx_ids;
+} tcsem_tx;
+
+/* Sizes derived from a setup (ModelingSetup.cpp:79-84,190-192; main.cu:102-103). */
+typedef struct tcsem_grid_info {
+  int L, M, N;                       /* cells in x, y, z                                       */
+  int AnLx_New, AnRx_New, Any_New;   /* regular-grid padding widths of the air plane           */
+  int L1, M1;                        /* regular air-plane size (x, y)                          */
+} tcsem_grid_info;
+
+typedef struct tcsem_ctx tcsem_ctx;
+
+/* ---- library ------------------------------------------------------------------------------- */
+const char *tcsem_version(void);
+const char *tcsem_last_error(void);
+
+/* ---- file surface (host only, no device use) ---------------------------------------------- */
+/* ModelingSetup.cpp:43-61 */
+int tcsem_read_setup(const char *path, tcsem_setup *out);
+/* ModelingSetup.cpp:201-209: L*M*N FP64, x fastest. */
+int tcsem_read_cond(const char *path, size_t ncell, double *out);
+/* LoadRxTxPos.cpp:43-86.  Pass xyz=NULL to query *nrx; otherwise xyz holds 3*cap doubles
+ * (x,y,z per receiver, indexed by id-1). */
+int tcsem_read_receivers(const char *path, tcsem_rxtime *rt, int *nrx, double *xyz, int cap);
+/* LoadRxTxPos.cpp:126-200.  Pass out=NULL to query *ntx and *nids (total receiver ids); then
+ * out[cap] rows are filled and their rx_ids point into ids[ids_cap]. */
+int tcsem_read_transmitters(const char *path, tcsem_tx *out, int cap, int *ntx, int *ids,
+                            int ids_cap, int *nids);
+/* main.cu:280-291: int32 {nr, ncomp, nt} then FP64 [nr][ncomp][nt]. */
+int tcsem_write_bout(const char *path, int nr, int ncomp, int nt, const double *data);
+int tcsem_read_bout(const char *path, int *nr, int *ncomp, int *nt, double *data, size_t cap);
+
+/* ---- host setup restatements (no device use) ---------------------------------------------- */
+int tcsem_grid_info_of(const tcsem_setup *s, tcsem_grid_info *out);
+/* ModelingSetup.cpp:100-187, ModelGridMapping.cpp:58-75.  Any pointer may be NULL. */
+int tcsem_grid_arrays(const tcsem_setup *s, double *dx, double *dy, double *dz, double *X,
+                      double *Y, double *Z, double *Xnew, double *Ynew);
+/* ModelGridMapping.cpp:77-366.  Any pointer may be NULL. */
+int tcsem_grid_maps(const tcsem_setup *s, int *xN2o, int *yN2o, int *xo2N, int *yo2N,
+                    int *xBx2N, int *yBy2N);
+/* SourceWave.cpp:8-58 (t, wave: 2*ns+1 values; dt: 2*ns+2 values). */
+int tcsem_waveform(int ns, double sigma_t, double *t, double *dt, double *wave);
+
+/* ---- device context ------------------------------------------------------------------------ */
+/* ModelingSetup + ModelGridMapping + prepare_device_arrays + initialize_UpCont_gpu_resources.
+ * cond: L*M*N FP64 conductivity, x fastest.  precision: TCSEM_FP64 or TCSEM_FP32. */
+int tcsem_create(const tcsem_setup *s, const double *cond, int precision, tcsem_ctx **out);
+void tcsem_destroy(tcsem_ctx *ctx);
+int tcsem_info(const tcsem_ctx *ctx, tcsem_grid_info *out);
+
+/* LoadRxTxPos.cpp:51-121,431-460.  rx_xyz: 3*nrx (x,y,z per receiver id-1). */
+int tcsem_set_receivers(tcsem_ctx *ctx, const tcsem_rxtime *rt, int nrx, const double *rx_xyz);
+/* LoadRxTxPos.cpp:143-428 (index search) and ElectricalFiled.cu:532-641 (bipole currents). */
+int tcsem_set_transmitters(tcsem_ctx *ctx, const tcsem_tx *tx, int ntx);
+
+/* Number of field sets (transmitters advanced together) the context holds; grows on demand. */
+int tcsem_reserve_batch(tcsem_ctx *ctx, int nbatch);
+/* set_zeros_EM_arrays (prepare_free_mem.cu:164-186) for field sets [first, first+n). */
+int tcsem_reset_fields(tcsem_ctx *ctx, int first, int n);
+
+/* One DuFort-Frankel step for field sets [0, nbatch), each driven by transmitter
+ * tx_first+b with the schedule of the step counter.  Equivalent to main.cu:223-260:
+ * UpContinuation_gpu, ElectricalField (with source), Bderivate, then time advance. */
+int tcsem_begin(tcsem_ctx *ctx, int tx_first, int nbatch);
+int tcsem_step(tcsem_ctx *ctx, int nsteps);
+/* Steps still needed until every receiver sample is taken (main.cu:219). */
+int tcsem_steps_remaining(const tcsem_ctx *ctx);
+
+/* Full per-transmitter run (main.cu:195-298) for transmitters [tx_first, tx_first+ntx), all
+ * advanced in one batch.  traces: ntx blocks of [nr][NComp][Nt] FP64 (nr of each Tx).
+ * max_steps > 0 caps the loop.  *steps_done (nullable) receives the steps taken. */
+int tcsem_run(tcsem_ctx *ctx, int tx_first, int ntx, int max_steps, double *traces,
+              int *steps_done);
+/* Traces of batch slot b after tcsem_begin/tcsem_step (nr x NComp x Nt FP64). */
+int tcsem_get_traces(tcsem_ctx *ctx, int slot, double *traces);
+
+/* Field access for batch slot `slot`; FP32 contexts convert.  Sizes as the component ids. */
+int tcsem_get_field(tcsem_ctx *ctx, int slot, int comp, double *out);
+int tcsem_set_field(tcsem_ctx *ctx, int slot, int comp, const double *in);
+
+/* UpContinuation_gpu on a caller plane: bz0 (M*L) -> bxair (M*(L+1)), byair ((M+1)*L). */
+int tcsem_upcont_plane(tcsem_ctx *ctx, const double *bz0, double *bxair, double *byair);
+
+/* Average device time per launch [ms] of the kernels of the last tcsem_step/tcsem_run, measured
+ * with HIP events on the context's stream: t[0] volume march, t[1] surface level,
+ * t[2] upward continuation chain, t[3] receiver gather; n[i] launches. */
+int tcsem_timing(const tcsem_ctx *ctx, double t[4], long n[4]);
+int tcsem_enable_timing(tcsem_ctx *ctx, int on);
+int tcsem_sync(tcsem_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TCSEM_H */

@@ -1,0 +1,1760 @@
+// tcsem_host.cpp -- host runtime of libtcsem.so: the C-ABI of include/tcsem.h.
+//
+// Owns the reference's file surface (FDTD_setup.in, Cond.bin, Receiver.in, Transmitter.in,
+// Tx<id>.bout), the host setup it restates (grid, regular/irregular index maps, Gaussian source
+// schedule, receiver/transmitter cell search), the precomputed operators of the air boundary
+// (spline window weights, FFT plans, wavenumber gains), device memory, and the step driver of
+// main.cu:195-298 with the EFInterp2Rx sampling state machine.  All numerics of a step run in the
+// gfx950 kernels of tcsem_kernels.hip and tcsem_upcont.hip; there is no CPU fallback.
+#include "../../include/tcsem.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "tcsem_internal.h"
+
+using namespace tcsem;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(x)                                                                        \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) return fail(TCSEM_EDEVICE, "%s: %s", #x, hipGetErrorString(e_)); \
+  } while (0)
+
+// ---- host setup restatements --------------------------------------------------------------------
+struct HostGrid {
+  tcsem_grid_info info{};
+  int AL = 0, AR = 0, AY = 0;
+  double mindx = 0;
+  std::vector<double> dx, dy, dz, X, Y, Z, Xn, Yn;
+  std::vector<int> xN2o, yN2o, xo2N, yo2N, xBx2N, yBy2N;
+};
+
+void coords(const tcsem_setup &s, HostGrid &g) {  // ModelingSetup.cpp:154-187
+  const int L = g.info.L, M = g.info.M, N = g.info.N;
+  int nc = (s.Nx % 2 == 0) ? s.Nx / 2 + g.AL : (s.Nx - 1) / 2 + g.AL;
+  g.X[nc] = (s.Nx % 2 == 0) ? 0.5 * g.dx[nc] + s.x0 : s.x0;
+  for (int i = nc + 1; i < L; ++i) g.X[i] = g.X[i - 1] + 0.5 * g.dx[i - 1] + 0.5 * g.dx[i];
+  for (int i = nc - 1; i >= 0; --i) g.X[i] = g.X[i + 1] - 0.5 * g.dx[i + 1] - 0.5 * g.dx[i];
+  nc = (s.Ny % 2 == 0) ? s.Ny / 2 + g.AY : (s.Ny - 1) / 2 + g.AY;
+  g.Y[nc] = (s.Ny % 2 == 0) ? 0.5 * g.dy[nc] + s.y0 : s.y0;
+  for (int j = nc + 1; j < M; ++j) g.Y[j] = g.Y[j - 1] + 0.5 * g.dy[j - 1] + 0.5 * g.dy[j];
+  for (int j = nc - 1; j >= 0; --j) g.Y[j] = g.Y[j + 1] - 0.5 * g.dy[j + 1] - 0.5 * g.dy[j];
+  g.Z[0] = 0.5 * g.dz[0] + s.z0;
+  for (int k = 1; k < N; ++k) g.Z[k] = g.Z[k - 1] + 0.5 * g.dz[k - 1] + 0.5 * g.dz[k];
+  // regular air-plane widths (ModelingSetup.cpp:190-192)
+  g.info.AnLx_New = (int)std::ceil((g.X[g.AL - 1] - g.X[0] + g.dx[g.AL - 1] / 2 + g.dx[0] / 2) / s.dxmin);
+  g.info.AnRx_New = (int)std::ceil((g.X[L - 1] - g.X[L - g.AR] + g.dx[L - 1] / 2 + g.dx[L - g.AR] / 2) / s.dxmin);
+  g.info.Any_New = (int)std::ceil((g.Y[g.AY - 1] - g.Y[0] + g.dy[g.AY - 1] / 2 + g.dy[0] / 2) / s.dymin);
+  g.info.L1 = g.info.AnLx_New + g.info.AnRx_New + s.Nx;  // main.cu:102-103
+  g.info.M1 = 2 * g.info.Any_New + s.Ny;
+}
+
+int build_grid(const tcsem_setup &s, HostGrid &g) {  // ModelingSetup.cpp:79-152
+  if (!(s.dxmin > 0 && s.dymin > 0 && s.dzmin > 0 && s.arg1 > 0 && s.arg2 > 0))
+    return fail(TCSEM_EINVAL, "cell sizes and growth factors must be positive");
+  if (s.ALx1 < 0 || s.ALx2 < 0 || s.ARx1 < 0 || s.ARx2 < 0 || s.Ay1 < 0 || s.Ay2 < 0 || s.Az1 < 0 ||
+      s.Az2 < 0 || s.Nz < 1)
+    return fail(TCSEM_EINVAL, "negative zone count");
+  g.AL = s.ALx1 + s.ALx2;
+  g.AR = s.ARx1 + s.ARx2;
+  g.AY = s.Ay1 + s.Ay2;
+  // Zero padding makes the reference index X_Bzold[-1] (ModelingSetup.cpp:190-192).
+  if (g.AL < 1 || g.AR < 1 || g.AY < 1)
+    return fail(TCSEM_EINVAL, "each side needs at least one padding cell");
+  if (s.Nx < kNPad || s.Ny < kNPad) return fail(TCSEM_EINVAL, "core needs at least %d cells per axis", kNPad);
+  const int L = g.AL + g.AR + s.Nx, M = 2 * g.AY + s.Ny, N = s.Nz + s.Az1 + s.Az2;
+  if (N < 3) return fail(TCSEM_EINVAL, "need at least 3 levels");
+  if (g.AL + kNPad > L || g.AR + kNPad > L || g.AY + kNPad > M) return fail(TCSEM_EINVAL, "spline window exceeds grid");
+  g.info.L = L;
+  g.info.M = M;
+  g.info.N = N;
+  g.dx.assign(L, 0);
+  g.dy.assign(M, 0);
+  g.dz.assign(N, 0);
+  g.X.assign(L, 0);
+  g.Y.assign(M, 0);
+  g.Z.assign(N, 0);
+  for (int i = g.AL; i < s.Nx + g.AL; ++i) g.dx[i] = s.dxmin;
+  for (int i = g.AL - 1; i >= s.ALx2; --i) g.dx[i] = s.dxmin * std::pow(s.arg1, double(g.AL - i));
+  for (int i = s.ALx2 - 1; i >= 0; --i)
+    g.dx[i] = s.dxmin * std::pow(s.arg1, double(s.ALx1)) * std::pow(s.arg2, double(s.ALx2 - i));
+  for (int i = s.Nx + g.AL; i < s.Nx + g.AL + s.ARx1; ++i)
+    g.dx[i] = s.dxmin * std::pow(s.arg1, double(i - g.AL - s.Nx + 1));
+  for (int i = s.Nx + g.AL + s.ARx1; i < L; ++i)
+    g.dx[i] = s.dxmin * std::pow(s.arg1, double(s.ARx1)) * std::pow(s.arg2, double(i - s.Nx - g.AL - s.ARx1 + 1));
+  for (int j = g.AY; j < s.Ny + g.AY; ++j) g.dy[j] = s.dymin;
+  for (int j = g.AY - 1; j >= s.Ay2; --j) g.dy[j] = g.dy[j + 1] * s.arg1;
+  for (int j = s.Ay2 - 1; j >= 0; --j) g.dy[j] = g.dy[j + 1] * s.arg2;
+  for (int j = s.Ny + g.AY; j < s.Ny + s.Ay2 + 2 * s.Ay1; ++j) g.dy[j] = g.dy[j - 1] * s.arg1;
+  for (int j = s.Ny + s.Ay2 + 2 * s.Ay1; j < M; ++j) g.dy[j] = g.dy[j - 1] * s.arg2;
+  for (int k = 0; k < s.Nz; ++k) g.dz[k] = s.dzmin;
+  for (int k = s.Nz; k < s.Nz + s.Az1; ++k) g.dz[k] = g.dz[k - 1] * s.arg1;
+  for (int k = s.Nz + s.Az1; k < N; ++k) g.dz[k] = g.dz[k - 1] * s.arg2;
+  coords(s, g);
+  g.mindx = std::min(s.dzmin, std::min(s.dxmin, s.dymin));  // main.cu:107
+  return TCSEM_OK;
+}
+
+// Walks of ModelGridMapping.cpp:77-366: from a seed index, step towards the target until the
+// bracket condition holds.
+void build_maps(const tcsem_setup &s, HostGrid &g) {
+  const int L0 = g.info.L, M0 = g.info.M, L1 = g.info.L1, M1 = g.info.M1;
+  const int aLx = g.info.AnLx_New, aY = g.info.Any_New, Nx = s.Nx, Ny = s.Ny;
+  g.Xn.assign(L1, 0);
+  g.Yn.assign(M1, 0);
+  for (int i = aLx; i < aLx + Nx; ++i) g.Xn[i] = g.X[i - aLx + g.AL];
+  for (int i = aLx - 1; i >= 0; --i) g.Xn[i] = g.Xn[i + 1] - s.dxmin;
+  for (int i = aLx + Nx; i < L1; ++i) g.Xn[i] = g.Xn[i - 1] + s.dxmin;
+  for (int j = aY; j < aY + Ny; ++j) g.Yn[j] = g.Y[j - aY + g.AY];
+  for (int j = aY - 1; j >= 0; --j) g.Yn[j] = g.Yn[j + 1] - s.dymin;
+  for (int j = aY + Ny; j < M1; ++j) g.Yn[j] = g.Yn[j - 1] + s.dymin;
+
+  // new -> old: index of the first old node >= the new point (core points coincide)
+  auto new2old = [](const std::vector<double> &O, const std::vector<double> &Nw, int core0, int ncore,
+                    int ocore0, std::vector<int> &out) {
+    const int n0 = (int)O.size(), n1 = (int)Nw.size();
+    out.assign(n1, 0);
+    for (int i = core0; i < core0 + ncore; ++i) out[i] = i - core0 + ocore0;
+    for (int i = core0 - 1; i >= 0; --i) {
+      int j = out[i + 1];
+      while (j > 0 && O[j - 1] >= Nw[i]) --j;
+      out[i] = j;
+    }
+    for (int i = core0 + ncore; i < n1; ++i) {
+      int j = out[i - 1] - 1;
+      while (j < n0 - 1 && O[j + 1] < Nw[i]) ++j;
+      out[i] = j + 1;
+    }
+  };
+  new2old(g.X, g.Xn, aLx, Nx, g.AL, g.xN2o);
+  new2old(g.Y, g.Yn, aY, Ny, g.AY, g.yN2o);
+  // old (possibly shifted by -d/2, plus one end point) -> new
+  auto old2new = [](const std::vector<double> &P, const std::vector<double> &Nw, int ocore0, int ncore,
+                    int core0, std::vector<int> &out) {
+    const int np = (int)P.size(), n1 = (int)Nw.size();
+    out.assign(np, 0);
+    for (int i = ocore0; i < ocore0 + ncore; ++i) out[i] = i - ocore0 + core0;
+    for (int i = ocore0 - 1; i >= 0; --i) {
+      int j = out[i + 1];
+      while (j > 0 && Nw[j - 1] >= P[i]) --j;
+      out[i] = j;
+    }
+    for (int i = ocore0 + ncore; i < np; ++i) {
+      int j = out[i - 1];
+      while (j < n1 - 1 && Nw[j + 1] < P[i]) ++j;
+      out[i] = j + 1;
+    }
+  };
+  old2new(g.X, g.Xn, g.AL, Nx, aLx, g.xo2N);
+  old2new(g.Y, g.Yn, g.AY, Ny, aY, g.yo2N);
+  std::vector<double> px(L0 + 1), py(M0 + 1);
+  for (int i = 0; i < L0; ++i) px[i] = g.X[i] - 0.5 * g.dx[i];
+  px[L0] = g.X[L0 - 1] + 0.5 * g.dx[L0 - 1];
+  for (int j = 0; j < M0; ++j) py[j] = g.Y[j] - 0.5 * g.dy[j];
+  py[M0] = g.Y[M0 - 1] + 0.5 * g.dy[M0 - 1];
+  old2new(px, g.Xn, g.AL, Nx, aLx, g.xBx2N);
+  old2new(py, g.Yn, g.AY, Ny, aY, g.yBy2N);
+}
+
+// SourceWave.cpp:8-58; dt[2ns] is consumed before the reference writes it (its malloc'd value
+// was 0 in the run that produced testwaveform.dat), so it starts at 0 here.
+int waveform(int ns, double st, double *t, double *dt, double *wave) {
+  if (ns < 1 || !(st > 0)) return fail(TCSEM_EINVAL, "bad source discretisation");
+  for (int i = 0; i < 2 * ns + 2; ++i) dt[i] = 0.0;
+  t[0] = 0;
+  for (int i = 0; i < ns; ++i) {
+    const double t0 = t[i] - 3 * st;
+    dt[i] = -st * st * (1 - std::exp(-4.5)) / ns / t0 / std::exp(-t0 * t0 / (2 * st * st));
+    t[i + 1] = t[i] + dt[i];
+  }
+  if (3 * st < t[ns]) t[ns] = 3 * st;
+  for (int i = ns + 1; i < 2 * ns + 1; ++i) {
+    t[i] = 6 * st - t[2 * ns - i];
+    dt[i - 1] = t[i] - t[i - 1];
+  }
+  for (int i = 0; i < 2 * ns + 1; ++i)
+    wave[i] = 1 / (std::sqrt(2 * kPi) * st) * std::exp(-(t[i] - 3 * st) * (t[i] - 3 * st) / 2 / st / st);
+  dt[2 * ns + 1] = 0.5 * (dt[2 * ns] * dt[2 * ns] / dt[2 * ns - 1] + dt[2 * ns]);
+  double t0 = dt[0];
+  dt[0] = 0.5 * (t0 * t0 / dt[1] + t0);
+  for (int i = 1; i < 2 * ns + 1; ++i) {
+    const double t1 = dt[i];
+    dt[i] = 0.5 * (t0 + dt[i]);
+    t0 = t1;
+  }
+  return TCSEM_OK;
+}
+
+int search_insert(const double *A, int n, double target) {  // logspace.cpp:30-64
+  if (n < 1) return 0;
+  int low = 0, high = n - 1, pos = 0;
+  while (low <= high) {
+    const int mid = low + (high - low) / 2;
+    if (A[mid] == target) return mid;
+    if (A[mid] > target) {
+      high = mid - 1;
+      pos = mid;
+    } else {
+      low = mid + 1;
+      pos = mid + 1;
+    }
+  }
+  return pos;
+}
+
+void sample_times(const tcsem_rxtime &rt, std::vector<double> &tm) {  // LoadRxTxPos.cpp:438-447
+  tm.resize(rt.Nt);
+  if (rt.is_log) {
+    const double step = (std::log10(rt.Tn) - std::log10(rt.T0)) / (rt.Nt - 1);
+    for (int i = 0; i < rt.Nt; ++i) tm[i] = std::pow(10.0, std::log10(rt.T0) + i * step);
+  } else {
+    const double step = (rt.Tn - rt.T0) / (rt.Nt - 1);
+    for (int i = 0; i < rt.Nt; ++i) tm[i] = rt.T0 + i * step;
+  }
+}
+
+// Natural cubic spline of SplineGridConvIR2R.cu (node spacing 0.5 d[i] + 0.5 d[i+1], natural
+// ends, cubic inside the window, linear beyond it) evaluated at np points.  It is linear in ys,
+// so the window operator is tabulated once by evaluating it on unit vectors.
+void spline_eval(int n, const double *d, const double *xs, const double *ys, int np, const double *xnew,
+                 const int *idraw, double *out) {
+  std::vector<double> dl(n), dd(n), du(n), rhs(n), cp(n), rp(n), b(n);
+  for (int i = 0; i < n; ++i) {
+    if (i == 0 || i == n - 1) {
+      dd[i] = 2.0;
+      dl[i] = du[i] = rhs[i] = 0.0;
+    } else {
+      dd[i] = 2.0 / 3.0 * (d[i] + 0.5 * d[i - 1] + 0.5 * d[i + 1]);
+      dl[i] = 1.0 / 3.0 * (0.5 * d[i - 1] + 0.5 * d[i]);
+      du[i] = 1.0 / 3.0 * (0.5 * d[i] + 0.5 * d[i + 1]);
+      rhs[i] = (ys[i + 1] - ys[i]) / (0.5 * d[i] + 0.5 * d[i + 1]) - (ys[i] - ys[i - 1]) / (0.5 * d[i - 1] + 0.5 * d[i]);
+    }
+  }
+  cp[0] = du[0] / dd[0];
+  rp[0] = rhs[0] / dd[0];
+  for (int i = 1; i < n; ++i) {
+    const double den = dd[i] - dl[i] * cp[i - 1];
+    cp[i] = du[i] / den;
+    rp[i] = (rhs[i] - dl[i] * rp[i - 1]) / den;
+  }
+  b[n - 1] = rp[n - 1];
+  for (int i = n - 2; i >= 0; --i) b[i] = rp[i] - cp[i] * b[i + 1];
+  for (int p = 0; p < np; ++p) {
+    const int id = std::max(idraw[p] - 1, 0);
+    double ma, mc;
+    if (id < n - 1) {
+      const double h = 0.5 * d[id] + 0.5 * d[id + 1];
+      ma = 1.0 / 3.0 * (b[id + 1] - b[id]) / h;
+      mc = (ys[id + 1] - ys[id]) / h - 1.0 / 3.0 * (2.0 * b[id] + b[id + 1]) * h;
+    } else {
+      const double h = 0.5 * d[id - 1] + 0.5 * d[id];
+      ma = 1.0 / 3.0 * (b[id] - b[id - 1]) / h;
+      mc = (ys[id] - ys[id - 1]) / h - 1.0 / 3.0 * (2.0 * b[id - 1] + b[id]) * h;
+      mc += 3.0 * ma * h * h + 2.0 * b[id - 1] * h;
+      ma = 0.0;
+    }
+    const double hh = xnew[p] - xs[id];
+    const int ir = idraw[p] - 1;
+    if (ir >= 0 && ir < n - 1) out[p] = ma * hh * hh * hh + b[ir] * hh * hh + mc * hh + ys[ir];
+    else out[p] = (b[std::max(ir, 0)] * hh + mc) * hh + ys[std::max(ir, 0)];
+  }
+}
+
+// weights W[p][q]: the window operator on unit vectors
+void spline_weights(int n, const double *d, const double *xs, int np, const double *xnew, const int *idraw,
+                    std::vector<double> &W) {
+  W.assign((size_t)np * n, 0.0);
+  std::vector<double> e(n, 0.0), col(np);
+  for (int q = 0; q < n; ++q) {
+    e[q] = 1.0;
+    spline_eval(n, d, xs, e.data(), np, xnew, idraw, col.data());
+    for (int p = 0; p < np; ++p) W[(size_t)p * n + q] = col[p];
+    e[q] = 0.0;
+  }
+}
+
+// host radix-2 FFT (plan building only)
+void fft_pow2_host(int n, std::vector<double> &re, std::vector<double> &im, int sign) {
+  for (int i = 1, j = 0; i < n; ++i) {
+    int bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) {
+      std::swap(re[i], re[j]);
+      std::swap(im[i], im[j]);
+    }
+  }
+  for (int len = 2; len <= n; len <<= 1) {
+    const int h = len >> 1;
+    for (int k = 0; k < h; ++k) {
+      const double ang = sign * 2.0 * kPi * k / len, wr = std::cos(ang), wi = std::sin(ang);
+      for (int i = k; i < n; i += len) {
+        const double xr = re[i + h] * wr - im[i + h] * wi, xi = re[i + h] * wi + im[i + h] * wr;
+        re[i + h] = re[i] - xr;
+        im[i + h] = im[i] - xi;
+        re[i] += xr;
+        im[i] += xi;
+      }
+    }
+  }
+}
+
+struct HostPlan {
+  int n = 0, m = 0, log2m = 0;
+  std::vector<double> tw, chirp, bhat;  // interleaved complex
+};
+
+void make_plan(int n, HostPlan &P) {
+  P.n = n;
+  P.m = 1;
+  P.log2m = 0;
+  const bool pow2 = (n & (n - 1)) == 0;
+  const int need = pow2 ? n : 2 * n - 1;
+  while (P.m < need) {
+    P.m <<= 1;
+    P.log2m++;
+  }
+  P.tw.resize(P.m);  // m/2 complex
+  for (int k = 0; k < P.m / 2; ++k) {
+    const double ang = -2.0 * kPi * k / P.m;
+    P.tw[2 * k] = std::cos(ang);
+    P.tw[2 * k + 1] = std::sin(ang);
+  }
+  if (pow2) return;
+  P.chirp.resize(2 * n);
+  for (int k = 0; k < n; ++k) {
+    const long long q = ((long long)k * k) % (2LL * n);
+    const double ang = -kPi * (double)q / n;
+    P.chirp[2 * k] = std::cos(ang);
+    P.chirp[2 * k + 1] = std::sin(ang);
+  }
+  std::vector<double> br(P.m, 0.0), bi(P.m, 0.0);
+  br[0] = P.chirp[0];
+  bi[0] = -P.chirp[1];
+  for (int k = 1; k < n; ++k) {
+    br[k] = br[P.m - k] = P.chirp[2 * k];
+    bi[k] = bi[P.m - k] = -P.chirp[2 * k + 1];
+  }
+  fft_pow2_host(P.m, br, bi, -1);
+  P.bhat.resize(2 * P.m);
+  for (int k = 0; k < P.m; ++k) {
+    P.bhat[2 * k] = br[k] / P.m;
+    P.bhat[2 * k + 1] = bi[k] / P.m;
+  }
+}
+
+// located transmitter (LoadRxTxPos.cpp:143-428)
+struct TxLoc {
+  int ort = 0;
+  double BipoLen = 0, x = 0, y = 0, z = 0, azi = 0, dip = 0;
+  int ixa = 0, iya = 0, iza = 0, ixb = 0, iyb = 0, izb = 0, ictx = 0, icty = 0, ictz = 0;
+  double BipoLen1 = 0, x1 = 0, y1 = 0, z1 = 0, azi1 = 0, dip1 = 0;
+  int ixa1 = 0, iya1 = 0, iza1 = 0, ixb1 = 0, iyb1 = 0, izb1 = 0, ictx1 = 0, icty1 = 0, ictz1 = 0;
+  std::vector<int> rx;  // 0-based receiver indices
+  std::vector<SrcCell> cells;
+};
+
+void order_pair(int &a, int &b) {  // LoadRxTxPos.cpp:297-302
+  if (b > a) --b;
+  else if (b < a) --a;
+}
+
+}  // namespace
+
+// ---- context -------------------------------------------------------------------------------------
+struct tcsem_ctx {
+  int prec = 64;
+  size_t esz = 8;
+  tcsem_setup setup{};
+  HostGrid g;
+  std::vector<double> cond;
+  int PX = 0, PY = 0;
+  long long PS = 0, fstride = 0, pstride = 0, wstride = 0;
+  int ntx = 0, nty = 0, nper = 0;
+  int nslots = 0;
+  hipStream_t stream = nullptr;
+  // device buffers
+  char *d_fields = nullptr;  // 2 states x 5 fields x nslots x fstride elements
+  int cur = 0;               // state holding the current fields
+  char *d_sig = nullptr;
+  char *d_planes = nullptr;  // bz0, bxair, byair: 3 x nslots x pstride elements
+  char *d_rd = nullptr;      // rdx, rdx2, rdy, rdy2, dzt, rdz, rdz2 (field type)
+  double *d_grid = nullptr;  // X Y Z dx dy dz Xn Yn
+  int *d_maps = nullptr;     // xBx2N yo2N xo2N yBy2N
+  double *d_up = nullptr;    // wl wr wf wb gx gy, plan tables
+  double *d_work = nullptr;  // per slot: cplx, cplx2 (complex), bxr, byr
+  double *d_scratch = nullptr;
+  size_t scratch_bytes = 0;
+  UpcontArgs<double> up64{};
+  UpcontArgs<float> up32{};
+  HostPlan plx, ply;
+  std::vector<double> wl, wr, wf, wb, gx, gy;
+  int n1 = 0, n2 = 0, n3 = 0;
+  // receivers
+  bool have_rx = false;
+  tcsem_rxtime rt{};
+  int nrx = 0, ncomp = 0, nefcmp = 0, comps[6] = {0};
+  std::vector<double> rxx, rxy, rxz, times;
+  std::vector<int> rix, riy, riz;
+  double *d_rx = nullptr;
+  int *d_rxi = nullptr;
+  // transmitters
+  std::vector<TxLoc> tx;
+  // run state
+  int tx_first = 0, nbatch = 0, nrmax = 0;
+  bool begun = false;
+  int n = 0, steps = 0, ne = 0, nb = 0;
+  double toT = 0, dt1 = 0, dt2 = 0, Tpof[2][3] = {{0}};
+  std::vector<double> wt, wdt, wwave;
+  double *d_ef0 = nullptr, *d_efs = nullptr;
+  SrcCell *d_src = nullptr;
+  int *d_srcidx = nullptr;  // first[nbatch], count[nbatch]
+  int *d_rxl = nullptr;     // rx_local, rx_first[nbatch], rx_count[nbatch]
+  int *d_ops = nullptr;     // it of every op of the run (E ops then B ops)
+  double *d_opl1 = nullptr;
+  int src_kmin = 1, src_kmax = 0;
+  std::vector<int> op_it;
+  std::vector<double> op_l1;
+  // timing
+  bool timing = false;
+  hipEvent_t ev[8] = {};
+  double tacc[4] = {0, 0, 0, 0};
+  long tn[4] = {0, 0, 0, 0};
+};
+
+namespace {
+
+template <typename T>
+MarchArgs<T> march_args(tcsem_ctx *c) {
+  MarchArgs<T> a{};
+  const long long org = c->PX + 1;
+  const long long fs = c->fstride * c->nslots;
+  T *fi = (T *)c->d_fields + (long long)c->cur * 5 * fs;
+  T *fo = (T *)c->d_fields + (long long)(1 - c->cur) * 5 * fs;
+  a.exi = fi + org;
+  a.eyi = fi + fs + org;
+  a.ezi = fi + 2 * fs + org;
+  a.bxi = fi + 3 * fs + org;
+  a.byi = fi + 4 * fs + org;
+  a.ex = fo + org;
+  a.ey = fo + fs + org;
+  a.ez = fo + 2 * fs + org;
+  a.bx = fo + 3 * fs + org;
+  a.by = fo + 4 * fs + org;
+  a.sig = (const T *)c->d_sig + org;
+  T *pl = (T *)c->d_planes;
+  const long long ps = c->pstride * c->nslots;
+  a.bz0 = pl + org;
+  a.bxair = pl + ps + org;
+  a.byair = pl + 2 * ps + org;
+  a.fstride = c->fstride;
+  a.pstride = c->pstride;
+  a.PS = c->PS;
+  a.L = c->g.info.L;
+  a.M = c->g.info.M;
+  a.N = c->g.info.N;
+  a.PX = c->PX;
+  a.ntx = c->ntx;
+  a.nty = c->nty;
+  a.nper = c->nper;
+  const T *rd = (const T *)c->d_rd;
+  const int L = a.L, M = a.M, N = a.N;
+  a.rdx = rd;
+  a.rdx2 = rd + (L + 3);
+  a.rdy = rd + 2 * (L + 3);
+  a.rdy2 = rd + 2 * (L + 3) + (M + 3);
+  a.dzt = rd + 2 * (L + 3) + 2 * (M + 3);
+  a.rdz = a.dzt + N;
+  a.rdz2 = a.dzt + 2 * N;
+  a.src = c->d_src;
+  a.src_first = c->d_srcidx;
+  a.src_count = c->d_srcidx ? c->d_srcidx + std::max(c->nbatch, 1) : nullptr;
+  a.src_kmin = c->src_kmin;
+  a.src_kmax = c->src_kmax;
+  return a;
+}
+
+template <typename T>
+UpcontArgs<T> upcont_args(tcsem_ctx *c) {
+  UpcontArgs<T> u{};
+  const long long org = c->PX + 1;
+  T *pl = (T *)c->d_planes;
+  const long long ps = c->pstride * c->nslots;
+  u.bz0 = pl + org;
+  u.bxair = pl + ps + org;
+  u.byair = pl + 2 * ps + org;
+  u.pstride = c->pstride;
+  const tcsem_grid_info &I = c->g.info;
+  u.L = I.L;
+  u.M = I.M;
+  u.PX = c->PX;
+  u.L1 = I.L1;
+  u.M1 = I.M1;
+  u.AnLx = I.AnLx_New;
+  u.AnRx = I.AnRx_New;
+  u.Any = I.Any_New;
+  u.AL = c->g.AL;
+  u.AY = c->g.AY;
+  u.Nx = c->setup.Nx;
+  u.Ny = c->setup.Ny;
+  u.n1 = c->n1;
+  u.n2 = c->n2;
+  u.n3 = c->n3;
+  double *p = c->d_up;
+  u.wl = p;
+  p += c->wl.size();
+  u.wr = p;
+  p += c->wr.size();
+  u.wf = p;
+  p += c->wf.size();
+  u.wb = p;
+  p += c->wb.size();
+  u.gx = p;
+  p += c->gx.size();
+  u.gy = p;
+  p += c->gy.size();
+  auto plan = [&](const HostPlan &H, FftPlan &P) {
+    P.n = H.n;
+    P.m = H.m;
+    P.log2m = H.log2m;
+    P.tw = p;
+    p += H.tw.size();
+    P.chirp = H.chirp.empty() ? nullptr : p;
+    p += H.chirp.size();
+    P.bhat = H.bhat.empty() ? nullptr : p;
+    p += H.bhat.size();
+  };
+  plan(c->plx, u.px);
+  plan(c->ply, u.py);
+  u.H = I.L1 / 2 + 1;
+  u.wstride = c->wstride;
+  double *w = c->d_work;
+  const long long ws = c->wstride * c->nslots;
+  u.cplx = (double2 *)w;
+  u.cplx2 = (double2 *)(w + 2 * ws);
+  u.bxr = w + 4 * ws;
+  u.byr = w + 5 * ws;
+  const double *gd = c->d_grid;
+  u.X = gd;
+  u.Y = gd + I.L;
+  u.dx = gd + I.L + I.M + I.N;
+  u.dy = u.dx + I.L;
+  u.Xn = u.dy + I.M + I.N;
+  u.Yn = u.Xn + I.L1;
+  const int *mp = c->d_maps;
+  u.xBx2N = mp;
+  u.yo2N = mp + (I.L + 1);
+  u.xo2N = u.yo2N + I.M;
+  u.yBy2N = u.xo2N + I.L;
+  return u;
+}
+
+void free_run(tcsem_ctx *c) {
+  (void)hipFree(c->d_ef0);
+  (void)hipFree(c->d_efs);
+  (void)hipFree(c->d_src);
+  (void)hipFree(c->d_srcidx);
+  (void)hipFree(c->d_rxl);
+  (void)hipFree(c->d_ops);
+  (void)hipFree(c->d_opl1);
+  c->d_ef0 = c->d_efs = nullptr;
+  c->d_src = nullptr;
+  c->d_srcidx = c->d_rxl = c->d_ops = nullptr;
+  c->d_opl1 = nullptr;
+  c->begun = false;
+}
+
+int alloc_slots(tcsem_ctx *c, int nslots) {
+  if (nslots <= c->nslots) return TCSEM_OK;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  (void)hipFree(c->d_fields);
+  (void)hipFree(c->d_planes);
+  (void)hipFree(c->d_work);
+  c->d_fields = c->d_planes = nullptr;
+  c->d_work = nullptr;
+  c->nslots = 0;
+  const size_t fb = (size_t)2 * 5 * nslots * c->fstride * c->esz;
+  const size_t pb = (size_t)3 * nslots * c->pstride * c->esz;
+  const size_t wb = (size_t)6 * nslots * c->wstride * sizeof(double);
+  if (hipMalloc(&c->d_fields, fb) != hipSuccess) return fail(TCSEM_ENOMEM, "field allocation of %zu bytes failed", fb);
+  if (hipMalloc(&c->d_planes, pb) != hipSuccess) return fail(TCSEM_ENOMEM, "plane allocation failed");
+  if (hipMalloc((void **)&c->d_work, wb) != hipSuccess) return fail(TCSEM_ENOMEM, "work allocation failed");
+  HIPCHK(launch_fill_zero(c->d_fields, fb, c->stream));
+  HIPCHK(launch_fill_zero(c->d_planes, pb, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->nslots = nslots;
+  c->cur = 0;
+  return TCSEM_OK;
+}
+
+template <typename T>
+void put(std::vector<char> &buf, size_t idx, double v) {
+  ((T *)buf.data())[idx] = (T)v;
+}
+
+int upload_double_as(tcsem_ctx *c, char *dst, const std::vector<double> &v) {
+  if (c->prec == 64) {
+    HIPCHK(hipMemcpy(dst, v.data(), v.size() * 8, hipMemcpyHostToDevice));
+  } else {
+    std::vector<float> f(v.begin(), v.end());
+    HIPCHK(hipMemcpy(dst, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+  }
+  return TCSEM_OK;
+}
+
+// source cells of a located transmitter: AddTx_x/y/z (ElectricalFiled.cu:347-475) with the
+// currents of the host switch (ElectricalFiled.cu:532-641).  Per step the cell receives
+// E -= Wave[n] * w.  The second bipole of ort 5 is injected on the first bipole's cells, as the
+// reference's kernels read only the first bipole's indices.
+void source_cells(const tcsem_ctx *c, TxLoc &t) {
+  const HostGrid &g = c->g;
+  const int L = g.info.L, M = g.info.M;
+  const double *cd = c->cond.data();
+  auto C = [&](int k, int j, int i) { return cd[((size_t)k * M + j) * L + i]; };
+  t.cells.clear();
+  auto addx = [&](double cur, int nthreads) {
+    const int ixa = std::min(t.ixa, t.ixb), iya = t.icty, iza = t.ictz;
+    for (int ix = 0; ix < nthreads; ++ix) {
+      if (ix > std::abs(t.ixb - t.ixa)) continue;
+      double s = (iza == 0) ? C(iza, iya, ix + ixa) + C(iza, iya - 1, ix + ixa)
+                            : C(iza, iya, ix + ixa) + C(iza, iya - 1, ix + ixa) + C(iza - 1, iya, ix + ixa) +
+                                  C(iza - 1, iya - 1, ix + ixa);
+      t.cells.push_back({0, iza, iya, ix + ixa, 2.0 * cur / (0.25 * s * g.dy[iya] * g.dz[iza])});
+    }
+  };
+  auto addy = [&](double cur, int nthreads) {
+    const int ixa = t.ictx, iya = std::min(t.iya, t.iyb), iza = t.ictz;
+    for (int iy = 0; iy < nthreads; ++iy) {
+      if (iy > std::abs(t.iyb - t.iya)) continue;
+      double s = (iza == 0) ? C(iza, iy + iya, ixa - 1) + C(iza, iy + iya, ixa)
+                            : C(iza, iy + iya, ixa - 1) + C(iza, iy + iya, ixa) + C(iza - 1, iy + iya, ixa - 1) +
+                                  C(iza - 1, iy + iya, ixa);
+      t.cells.push_back({1, iza, iy + iya, ixa, 2.0 * cur / (0.25 * s * g.dx[ixa] * g.dz[iza])});
+    }
+  };
+  auto addz = [&](double cur, int nthreads) {
+    const int ixa = t.ictx, iya = t.icty, iza = std::min(t.iza, t.izb);
+    for (int iz = 0; iz < nthreads; ++iz) {
+      if (iz > std::abs(t.izb - t.iza)) continue;
+      double s = C(iza + iz, iya, ixa - 1) + C(iza + iz, iya, ixa) + C(iza + iz, iya - 1, ixa - 1) +
+                 C(iza + iz, iya - 1, ixa);
+      t.cells.push_back({2, iza + iz, iya, ixa, 2.0 * cur / (0.25 * s * g.dx[ixa] * g.dy[iya])});
+    }
+  };
+  const double *X = g.X.data(), *Y = g.Y.data(), *Z = g.Z.data(), *dx = g.dx.data(), *dy = g.dy.data(),
+               *dz = g.dz.data();
+  switch (t.ort) {
+    case 1:
+      addx(1.0 / (X[t.ixb] - X[t.ixa] + dx[t.ixb] / 2 + dx[t.ixa] / 2), std::abs(t.ixb - t.ixa) + 1);
+      break;
+    case 2:
+      addy(1.0 / (Y[t.iyb] - Y[t.iya] + dy[t.iyb] / 2 + dy[t.iya] / 2), std::abs(t.iyb - t.iya) + 1);
+      break;
+    case 3:
+      addz(1.0 / (Z[t.izb] - Z[t.iza] + dz[t.izb] / 2 + dz[t.iza] / 2), std::abs(t.izb - t.iza) + 1);
+      break;
+    case 4:
+    case 5:
+      for (int b = 0; b < (t.ort == 5 ? 2 : 1); ++b) {
+        const double dip = b ? t.dip1 : t.dip, azi = b ? t.azi1 : t.azi;
+        const int ixa = b ? t.ixa1 : t.ixa, ixb = b ? t.ixb1 : t.ixb, iya = b ? t.iya1 : t.iya,
+                  iyb = b ? t.iyb1 : t.iyb, iza = b ? t.iza1 : t.iza, izb = b ? t.izb1 : t.izb;
+        const double f = t.ort == 5 ? (b ? t.BipoLen1 : t.BipoLen) : 1.0;
+        addx(std::cos(dip) * std::cos(azi) * f / (std::fabs(X[ixb] - X[ixa]) + dx[ixb] / 2.0 + dx[ixa] / 2.0),
+             std::abs(ixb - ixa) + 1);
+        addy(std::cos(dip) * std::sin(azi) * f / (std::fabs(Y[iyb] - Y[iya]) + dy[iyb] / 2.0 + dy[iya] / 2.0),
+             std::abs(iyb - iya) + 1);
+        addz(std::sin(dip) * f / (std::fabs(Z[izb] - Z[iza]) + dz[izb] / 2.0 + dz[iza] / 2.0),
+             std::abs(izb - iza) + 1);
+      }
+      break;
+  }
+  // cells with zero weight (a zero current component) are dropped
+  t.cells.erase(std::remove_if(t.cells.begin(), t.cells.end(), [](const SrcCell &s) { return s.w == 0.0; }),
+                t.cells.end());
+}
+
+int locate_tx(const tcsem_ctx *c, const tcsem_tx &in, TxLoc &t) {  // LoadRxTxPos.cpp:143-428
+  const HostGrid &g = c->g;
+  const int L = g.info.L, M = g.info.M, K = g.info.N;
+  const double *X = g.X.data(), *Y = g.Y.data(), *Z = g.Z.data();
+  t = TxLoc();
+  t.ort = in.ort;
+  t.BipoLen = in.bipolen;
+  t.x = in.x;
+  t.y = in.y;
+  t.z = in.z;
+  t.BipoLen1 = in.bipolen1;
+  t.x1 = in.x1;
+  t.y1 = in.y1;
+  t.z1 = in.z1;
+  if (in.ort == 4 || in.ort == 5) {
+    t.azi = in.azi_deg * kPi / 180.0;
+    t.dip = in.dip_deg * kPi / 180.0;
+  }
+  if (in.ort == 5) {
+    t.azi1 = in.azi1_deg * kPi / 180.0;
+    t.dip1 = in.dip1_deg * kPi / 180.0;
+  }
+  switch (in.ort) {
+    case 1:
+      t.ixa = search_insert(X, L, t.x - t.BipoLen / 2);
+      t.ixb = search_insert(X, L, t.x + t.BipoLen / 2) - 1;
+      if (t.ixa == 0 || t.ixb == L) return fail(TCSEM_EDOMAIN, "transmitter %d outside the model in x", in.id);
+      t.icty = search_insert(Y, M, t.y);
+      if (t.icty == 0 || t.icty == M) return fail(TCSEM_EDOMAIN, "transmitter %d outside the model in y", in.id);
+      t.ictz = search_insert(Z, K, t.z);
+      break;
+    case 2:
+      t.ictx = search_insert(X, L, t.x);
+      if (t.ictx == 0 || t.ictx == L) return fail(TCSEM_EDOMAIN, "transmitter %d outside the model in x", in.id);
+      t.iya = search_insert(Y, M, t.y - t.BipoLen / 2);
+      t.iyb = search_insert(Y, M, t.y + t.BipoLen / 2) - 1;
+      if (t.iya == 0 || t.iyb == M) return fail(TCSEM_EDOMAIN, "transmitter %d outside the model in y", in.id);
+      t.ictz = search_insert(Z, K, t.z);
+      break;
+    case 3:
+      t.ictx = search_insert(X, L, t.x);
+      if (t.ictx == 0 || t.ictx == L) return fail(TCSEM_EDOMAIN, "transmitter %d outside the model in x", in.id);
+      t.icty = search_insert(Y, M, t.y);
+      if (t.icty == 0 || t.icty == M) return fail(TCSEM_EDOMAIN, "transmitter %d outside the model in y", in.id);
+      t.iza = search_insert(Z, K, t.z - t.BipoLen / 2);
+      t.izb = search_insert(Z, K, t.z + t.BipoLen / 2) - 1;
+      break;
+    case 4:
+    case 5: {
+      const double cx = t.BipoLen * std::cos(t.dip) * std::cos(t.azi) / 2.0;
+      const double cy = t.BipoLen * std::cos(t.dip) * std::sin(t.azi) / 2.0;
+      const double cz = t.BipoLen * std::sin(t.dip) / 2.0;
+      t.ixa = search_insert(X, L, t.x - cx);
+      t.ixb = search_insert(X, L, t.x + cx);
+      order_pair(t.ixa, t.ixb);
+      t.iya = search_insert(Y, M, t.y - cy);
+      t.iyb = search_insert(Y, M, t.y + cy);
+      order_pair(t.iya, t.iyb);
+      t.iza = search_insert(Z, K, t.z - cz);
+      t.izb = search_insert(Z, K, t.z + cz);
+      order_pair(t.iza, t.izb);
+      t.ictx = search_insert(X, L, t.x);
+      t.icty = search_insert(Y, M, t.y);
+      t.ictz = search_insert(Z, K, t.z);
+      if (in.ort == 5) {
+        const double ax = t.BipoLen1 * std::cos(t.dip1) * std::cos(t.azi1) / 2.0;
+        const double ay = t.BipoLen1 * std::cos(t.dip1) * std::sin(t.azi1) / 2.0;
+        const double az = t.BipoLen1 * std::sin(t.dip1) / 2.0;
+        t.ixa1 = search_insert(X, L, t.x1 - ax);
+        t.ixb1 = search_insert(X, L, t.x1 + ax);
+        order_pair(t.ixa1, t.ixb1);
+        t.iya1 = search_insert(Y, M, t.y1 - ay);
+        t.iyb1 = search_insert(Y, M, t.y1 + ay);
+        order_pair(t.iya1, t.iyb1);
+        t.iza1 = search_insert(Z, K, t.z1 - az);
+        t.izb1 = search_insert(Z, K, t.z1 + az);
+        order_pair(t.iza1, t.izb1);
+        t.ictx1 = search_insert(X, L, t.x1);
+        t.icty1 = search_insert(Y, M, t.y1);
+        t.ictz1 = search_insert(Z, K, t.z1);
+        if (t.ixa1 == 0 || t.ixb1 == L || t.iya1 == 0 || t.iyb1 == M)
+          return fail(TCSEM_EDOMAIN, "transmitter %d (second bipole) outside the model", in.id);
+      }
+      if (t.ixa == 0 || t.ixb == L) return fail(TCSEM_EDOMAIN, "transmitter %d outside the model in x", in.id);
+      if (t.iya == 0 || t.iyb == M) return fail(TCSEM_EDOMAIN, "transmitter %d outside the model in y", in.id);
+      break;
+    }
+    default:
+      return fail(TCSEM_EINVAL, "transmitter %d: illegal orientation %d", in.id, in.ort);
+  }
+  // the injected cells must be interior cells of the model (the reference reads and writes
+  // out of bounds otherwise)
+  auto okx = [&](int i) { return i >= 1 && i < L; };
+  auto oky = [&](int j) { return j >= 1 && j < M; };
+  auto okz = [&](int k) { return k >= 0 && k < K; };
+  bool ok = true;
+  if (in.ort == 1) ok = okx(std::min(t.ixa, t.ixb)) && std::max(t.ixa, t.ixb) < L && oky(t.icty) && okz(t.ictz);
+  if (in.ort == 2) ok = okx(t.ictx) && oky(std::min(t.iya, t.iyb)) && std::max(t.iya, t.iyb) < M && okz(t.ictz);
+  if (in.ort == 3) ok = okx(t.ictx) && oky(t.icty) && okz(std::min(t.iza, t.izb)) && std::max(t.iza, t.izb) < K;
+  if (in.ort >= 4)
+    ok = okx(std::min(t.ixa, t.ixb)) && std::max(t.ixa, t.ixb) < L && oky(std::min(t.iya, t.iyb)) &&
+         std::max(t.iya, t.iyb) < M && okz(std::min(t.iza, t.izb)) && std::max(t.iza, t.izb) < K &&
+         okx(t.ictx) && oky(t.icty) && okz(t.ictz);
+  if (!ok) return fail(TCSEM_EDOMAIN, "transmitter %d cells leave the model interior", in.id);
+  t.rx.clear();
+  for (int r = 0; r < in.nr; ++r) {
+    const int id = in.rx_ids[r];
+    if (id < 1 || id > c->nrx) return fail(TCSEM_EINVAL, "transmitter %d references receiver %d", in.id, id);
+    t.rx.push_back(id - 1);
+  }
+  return TCSEM_OK;
+}
+
+// time-interpolation bookkeeping of EFInterp2Rx.cu:472-585 for one field group
+struct Event {
+  int slot_store;  // 0 or 1
+  int op_first, op_count;
+};
+
+bool sample_group(tcsem_ctx *c, int fld, double t2, std::vector<int> &it, std::vector<double> &l1,
+                  Event &ev) {
+  int &cnt = fld == 0 ? c->ne : c->nb;
+  double *T = c->Tpof[fld];
+  const int Nt = c->rt.Nt;
+  const std::vector<double> &tm = c->times;
+  bool fire = false;
+  T[2] = t2;
+  while (cnt < Nt && tm[cnt] <= T[0]) ++cnt;
+  if (cnt < Nt && tm[cnt] <= T[2] && tm[cnt] > T[1]) {
+    ev.slot_store = 0;
+    ev.op_first = (int)it.size();
+    ev.op_count = 0;
+    fire = true;
+  } else if (cnt < Nt && tm[cnt] <= T[1] && tm[cnt] > T[0]) {
+    ev.slot_store = 1;
+    ev.op_first = (int)it.size();
+    ev.op_count = 0;
+    while (cnt < Nt && tm[cnt] <= T[1]) {
+      it.push_back(cnt);
+      l1.push_back((tm[cnt] - T[0]) / (T[1] - T[0]));
+      ++ev.op_count;
+      ++cnt;
+    }
+    fire = true;
+  }
+  T[0] = T[1];
+  T[1] = T[2];
+  return fire;
+}
+
+template <typename T>
+UpcontArgs<T> &up_of(tcsem_ctx *c);
+template <>
+UpcontArgs<double> &up_of<double>(tcsem_ctx *c) {
+  return c->up64;
+}
+template <>
+UpcontArgs<float> &up_of<float>(tcsem_ctx *c) {
+  return c->up32;
+}
+
+template <typename T>
+int do_step(tcsem_ctx *c) {
+  const tcsem_setup &s = c->setup;
+  const int ns = s.n_src / 2;
+  const double miu = kMu0, minsita = 0.1, mindx = c->g.mindx;
+  MarchArgs<T> a = march_args<T>(c);
+  a.lam = 7.5 * s.sfa2 * c->dt2 * c->dt1 / miu / mindx / mindx;  // main.cu:223
+  a.dt = c->dt1;
+  a.dtb = 0.5 * (c->dt1 + c->dt2);
+  a.amp = c->n < 2 * ns ? c->wwave[c->n] : 0.0;
+  const UpcontArgs<T> &u = up_of<T>(c);
+  const int nb = c->nbatch;
+  hipStream_t st = c->stream;
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[0], st));
+  HIPCHK(launch_march<T>(a, nb, st));
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[1], st));
+  HIPCHK(launch_upcont<T>(u, nb, st));
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[2], st));
+  HIPCHK(launch_surface<T>(a, nb, st));
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[3], st));
+  c->cur ^= 1;  // the new state is current
+  // time advance (main.cu:253-260)
+  c->toT += c->dt1;
+  c->dt1 = c->dt2;
+  c->n++;
+  if (c->n < 2 * ns) c->dt2 = c->wdt[c->n + 1];
+  else c->dt2 = std::min(0.15 * s.sfa1 * std::sqrt(miu * c->toT * minsita / 8.) * mindx, 1.2 * c->dt1);
+  c->steps++;
+  // receiver sampling (EFInterp2Rx(t0 = toT + dt1, dt = dt2))
+  int ngather = 0;
+  if (c->have_rx) {
+    const double t0 = c->toT + c->dt1;
+    GatherArgs gA{};
+    const long long org = c->PX + 1;
+    const long long fs = c->fstride * c->nslots;
+    const T *f = (const T *)c->d_fields + (long long)c->cur * 5 * fs;
+    for (int q = 0; q < 5; ++q) gA.f[q] = f + q * fs + org;
+    gA.f[5] = nullptr;
+    gA.fstride = c->fstride;
+    gA.PS = c->PS;
+    gA.L = c->g.info.L;
+    gA.M = c->g.info.M;
+    gA.N = c->g.info.N;
+    gA.PX = c->PX;
+    const tcsem_grid_info &I = c->g.info;
+    gA.X = c->d_grid;
+    gA.Y = c->d_grid + I.L;
+    gA.Z = c->d_grid + I.L + I.M;
+    gA.dx = c->d_grid + I.L + I.M + I.N;
+    gA.dy = gA.dx + I.L;
+    gA.dz = gA.dy + I.M;
+    gA.rx_first = c->d_rxl + 0;
+    gA.rx_count = c->d_rxl + nb;
+    gA.rx_local = c->d_rxl + 2 * nb;
+    gA.rxx = c->d_rx;
+    gA.rxy = c->d_rx + c->nrx;
+    gA.rxz = c->d_rx + 2 * c->nrx;
+    gA.rix = c->d_rxi;
+    gA.riy = c->d_rxi + c->nrx;
+    gA.riz = c->d_rxi + 2 * c->nrx;
+    gA.ncomp = c->ncomp;
+    gA.nt = c->rt.Nt;
+    gA.ef0 = c->d_ef0;
+    gA.efs = c->d_efs;
+    gA.nrmax = c->nrmax;
+    const bool has_b = c->rt.is_comp[3] || c->rt.is_comp[4] || c->rt.is_comp[5];
+    for (int fld = 0; fld < 2; ++fld) {
+      if (fld == 0 && c->ne >= c->rt.Nt) continue;
+      if (fld == 1 && (c->nb >= c->rt.Nt || !has_b)) continue;
+      Event ev{};
+      const size_t before = c->op_it.size();
+      if (!sample_group(c, fld, fld == 0 ? t0 : t0 + 0.5 * c->dt2, c->op_it, c->op_l1, ev)) continue;
+      // upload the new ops (few per event)
+      if (ev.op_count > 0) {
+        HIPCHK(hipMemcpyAsync(c->d_ops + before, c->op_it.data() + before, ev.op_count * sizeof(int),
+                              hipMemcpyHostToDevice, st));
+        HIPCHK(hipMemcpyAsync(c->d_opl1 + before, c->op_l1.data() + before, ev.op_count * sizeof(double),
+                              hipMemcpyHostToDevice, st));
+      }
+      const int c0 = fld == 0 ? 0 : c->nefcmp, c1 = fld == 0 ? c->nefcmp : c->ncomp;
+      if (c1 <= c0) continue;
+      gA.ncomp_group = c1 - c0;
+      gA.comp_base = c0;
+      for (int q = 0; q < gA.ncomp_group; ++q) gA.comps[q] = c->comps[c0 + q];
+      gA.slot_store = ev.slot_store;
+      gA.op_it = c->d_ops + ev.op_first;
+      gA.op_l1 = c->d_opl1 + ev.op_first;
+      gA.nops = ev.op_count;
+      if (c->timing && ngather == 0) HIPCHK(hipEventRecord(c->ev[4], st));
+      HIPCHK(launch_gather<T>(gA, fld, nb, march_args<T>(c), st));
+      ++ngather;
+    }
+    if (c->timing && ngather) HIPCHK(hipEventRecord(c->ev[5], st));
+  }
+  if (c->timing) {
+    HIPCHK(hipStreamSynchronize(st));
+    float ms;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    c->tacc[0] += ms;
+    c->tn[0]++;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+    c->tacc[1] += ms;
+    c->tn[1]++;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[1], c->ev[2]));
+    c->tacc[2] += ms;
+    c->tn[2]++;
+    if (ngather) {
+      HIPCHK(hipEventElapsedTime(&ms, c->ev[4], c->ev[5]));
+      c->tacc[3] += ms;
+      c->tn[3] += ngather;
+    }
+  }
+  return TCSEM_OK;
+}
+
+bool sampling_done(const tcsem_ctx *c) {  // main.cu:219
+  if (!c->have_rx) return false;
+  return !(c->ne < c->rt.Nt || (c->nb < c->rt.Nt && c->nefcmp < c->ncomp));
+}
+
+int check_ctx(const tcsem_ctx *c) {
+  if (!c) return fail(TCSEM_EINVAL, "null context");
+  return TCSEM_OK;
+}
+
+}  // namespace
+
+// ==================================================================================================
+// C ABI
+// ==================================================================================================
+extern "C" {
+
+const char *tcsem_version(void) { return "tcsem-mi355x 0.1 (gfx950)"; }
+const char *tcsem_last_error(void) { return g_err.c_str(); }
+
+int tcsem_read_setup(const char *path, tcsem_setup *o) {  // ModelingSetup.cpp:43-61
+  if (!path || !o) return fail(TCSEM_EINVAL, "null argument");
+  FILE *f = fopen(path, "r");
+  if (!f) return fail(TCSEM_EIO, "cannot open %s", path);
+  int n1 = 0, ok = 1;
+  ok &= fscanf(f, "%lf %lf\n", &o->sfa1, &o->sfa2) == 2;
+  ok &= fscanf(f, "%d %lf\n", &n1, &o->sigma_t) == 2;
+  ok &= fscanf(f, "%lf %lf %lf\n", &o->x0, &o->y0, &o->z0) == 3;
+  ok &= fscanf(f, "%lf %lf %lf\n", &o->dxmin, &o->dymin, &o->dzmin) == 3;
+  ok &= fscanf(f, "%lf %lf\n", &o->arg1, &o->arg2) == 2;
+  ok &= fscanf(f, "%d %d %d %d %d\n", &o->ALx2, &o->ALx1, &o->Nx, &o->ARx1, &o->ARx2) == 5;
+  ok &= fscanf(f, "%d %d %d\n", &o->Ay2, &o->Ay1, &o->Ny) == 3;
+  ok &= fscanf(f, "%d %d %d\n", &o->Az2, &o->Az1, &o->Nz) == 3;
+  fclose(f);
+  o->n_src = n1;
+  if (!ok) return fail(TCSEM_EIO, "%s: malformed setup file", path);
+  return TCSEM_OK;
+}
+
+int tcsem_read_cond(const char *path, size_t ncell, double *out) {
+  if (!path || !out) return fail(TCSEM_EINVAL, "null argument");
+  FILE *f = fopen(path, "rb");
+  if (!f) return fail(TCSEM_EIO, "cannot open %s", path);
+  const size_t got = fread(out, sizeof(double), ncell, f);
+  fclose(f);
+  if (got != ncell) return fail(TCSEM_EIO, "%s: %zu of %zu values", path, got, ncell);
+  return TCSEM_OK;
+}
+
+int tcsem_read_receivers(const char *path, tcsem_rxtime *rt, int *nrx, double *xyz, int cap) {
+  if (!path || !rt || !nrx) return fail(TCSEM_EINVAL, "null argument");
+  FILE *f = fopen(path, "r");
+  if (!f) return fail(TCSEM_EIO, "cannot open %s", path);
+  int ok = 1;
+  ok &= fscanf(f, "%lf %lf\n", &rt->T0, &rt->Tn) == 2;
+  ok &= fscanf(f, "%d %d\n", &rt->Nt, &rt->is_log) == 2;
+  ok &= fscanf(f, "%d %d %d %d %d %d\n", &rt->is_comp[0], &rt->is_comp[1], &rt->is_comp[2], &rt->is_comp[3],
+               &rt->is_comp[4], &rt->is_comp[5]) == 6;
+  ok &= fscanf(f, "%d\n", nrx) == 1;
+  if (!ok || *nrx < 0) {
+    fclose(f);
+    return fail(TCSEM_EIO, "%s: malformed header", path);
+  }
+  if (xyz) {
+    if (cap < *nrx) {
+      fclose(f);
+      return fail(TCSEM_ESIZE, "receiver buffer holds %d of %d", cap, *nrx);
+    }
+    for (int r = 0; r < *nrx; ++r) {
+      int id;
+      double x, y, z;
+      if (fscanf(f, "%d %lf %lf %lf\n", &id, &x, &y, &z) != 4) {
+        fclose(f);
+        return fail(TCSEM_EIO, "%s: only %d of %d receivers", path, r, *nrx);
+      }
+      if (id < 1 || id > *nrx) {
+        fclose(f);
+        return fail(TCSEM_EIO, "%s: receiver id %d out of range", path, id);
+      }
+      xyz[3 * (id - 1)] = x;
+      xyz[3 * (id - 1) + 1] = y;
+      xyz[3 * (id - 1) + 2] = z;
+    }
+  }
+  fclose(f);
+  return TCSEM_OK;
+}
+
+int tcsem_read_transmitters(const char *path, tcsem_tx *out, int cap, int *ntx, int *ids, int ids_cap,
+                            int *nids) {
+  if (!path || !ntx) return fail(TCSEM_EINVAL, "null argument");
+  FILE *f = fopen(path, "r");
+  if (!f) return fail(TCSEM_EIO, "cannot open %s", path);
+  if (fscanf(f, "%d\n", ntx) != 1 || *ntx < 0) {
+    fclose(f);
+    return fail(TCSEM_EIO, "%s: malformed header", path);
+  }
+  int used = 0, rc = TCSEM_OK;
+  for (int r = 0; r < *ntx && rc == TCSEM_OK; ++r) {
+    tcsem_tx t{};
+    int itx = 0, ort = 0;
+    if (fscanf(f, "%d %d", &itx, &ort) != 2) {
+      rc = fail(TCSEM_EIO, "%s: row %d", path, r + 1);
+      break;
+    }
+    t.id = itx;
+    t.ort = ort;
+    int ok;
+    if (ort < 4) ok = fscanf(f, " %lf %lf %lf %lf %d", &t.bipolen, &t.x, &t.y, &t.z, &t.nr) == 5;
+    else if (ort == 4)
+      ok = fscanf(f, " %lf %lf %lf %lf %lf %lf %d", &t.bipolen, &t.x, &t.y, &t.z, &t.azi_deg, &t.dip_deg, &t.nr) == 7;
+    else if (ort == 5)
+      ok = fscanf(f, " %lf %lf %lf %lf %lf %lf", &t.bipolen, &t.x, &t.y, &t.z, &t.azi_deg, &t.dip_deg) == 6 &&
+           fscanf(f, " %lf %lf %lf %lf %lf %lf %d", &t.bipolen1, &t.x1, &t.y1, &t.z1, &t.azi1_deg, &t.dip1_deg,
+                  &t.nr) == 7;
+    else {
+      rc = fail(TCSEM_EINVAL, "%s: illegal transmitter type %d", path, ort);
+      break;
+    }
+    if (!ok || t.nr < 0) {
+      rc = fail(TCSEM_EIO, "%s: row %d", path, r + 1);
+      break;
+    }
+    int *dst = (ids && used + t.nr <= ids_cap) ? ids + used : nullptr;
+    for (int q = 0; q < t.nr; ++q) {
+      int v;
+      if (fscanf(f, " %d", &v) != 1) {
+        rc = fail(TCSEM_EIO, "%s: row %d receiver list", path, r + 1);
+        break;
+      }
+      if (dst) dst[q] = v;
+    }
+    if (out) {
+      if (itx < 1 || itx > cap) {
+        rc = fail(TCSEM_EIO, "%s: transmitter id %d out of range", path, itx);
+        break;
+      }
+      if (!dst) {
+        rc = fail(TCSEM_ESIZE, "receiver id buffer too small");
+        break;
+      }
+      t.rx_ids = dst;
+      out[itx - 1] = t;
+    }
+    used += t.nr;
+  }
+  fclose(f);
+  if (nids) *nids = used;
+  return rc;
+}
+
+int tcsem_write_bout(const char *path, int nr, int ncomp, int nt, const double *data) {
+  if (!path || !data || nr < 0 || ncomp < 0 || nt < 0) return fail(TCSEM_EINVAL, "bad argument");
+  FILE *f = fopen(path, "wb");
+  if (!f) return fail(TCSEM_EIO, "cannot open %s", path);
+  const size_t n = (size_t)nr * ncomp * nt;
+  bool ok = fwrite(&nr, sizeof(int), 1, f) == 1 && fwrite(&ncomp, sizeof(int), 1, f) == 1 &&
+            fwrite(&nt, sizeof(int), 1, f) == 1 && fwrite(data, sizeof(double), n, f) == n;
+  ok = (fclose(f) == 0) && ok;
+  return ok ? TCSEM_OK : fail(TCSEM_EIO, "write to %s failed", path);
+}
+
+int tcsem_read_bout(const char *path, int *nr, int *ncomp, int *nt, double *data, size_t cap) {
+  if (!path || !nr || !ncomp || !nt) return fail(TCSEM_EINVAL, "null argument");
+  FILE *f = fopen(path, "rb");
+  if (!f) return fail(TCSEM_EIO, "cannot open %s", path);
+  bool ok = fread(nr, sizeof(int), 1, f) == 1 && fread(ncomp, sizeof(int), 1, f) == 1 &&
+            fread(nt, sizeof(int), 1, f) == 1;
+  if (ok && data) {
+    const size_t n = (size_t)(*nr) * (*ncomp) * (*nt);
+    if (n > cap) {
+      fclose(f);
+      return fail(TCSEM_ESIZE, "buffer holds %zu of %zu values", cap, n);
+    }
+    ok = fread(data, sizeof(double), n, f) == n;
+  }
+  fclose(f);
+  return ok ? TCSEM_OK : fail(TCSEM_EIO, "%s: truncated", path);
+}
+
+int tcsem_grid_info_of(const tcsem_setup *s, tcsem_grid_info *out) {
+  if (!s || !out) return fail(TCSEM_EINVAL, "null argument");
+  HostGrid g;
+  const int rc = build_grid(*s, g);
+  if (rc) return rc;
+  *out = g.info;
+  return TCSEM_OK;
+}
+
+int tcsem_grid_arrays(const tcsem_setup *s, double *dx, double *dy, double *dz, double *X, double *Y, double *Z,
+                      double *Xnew, double *Ynew) {
+  if (!s) return fail(TCSEM_EINVAL, "null argument");
+  HostGrid g;
+  const int rc = build_grid(*s, g);
+  if (rc) return rc;
+  build_maps(*s, g);
+  auto cp = [](double *d, const std::vector<double> &v) {
+    if (d) std::memcpy(d, v.data(), v.size() * sizeof(double));
+  };
+  cp(dx, g.dx);
+  cp(dy, g.dy);
+  cp(dz, g.dz);
+  cp(X, g.X);
+  cp(Y, g.Y);
+  cp(Z, g.Z);
+  cp(Xnew, g.Xn);
+  cp(Ynew, g.Yn);
+  return TCSEM_OK;
+}
+
+int tcsem_grid_maps(const tcsem_setup *s, int *a, int *b, int *c, int *d, int *e, int *f) {
+  if (!s) return fail(TCSEM_EINVAL, "null argument");
+  HostGrid g;
+  const int rc = build_grid(*s, g);
+  if (rc) return rc;
+  build_maps(*s, g);
+  auto cp = [](int *dd, const std::vector<int> &v) {
+    if (dd) std::memcpy(dd, v.data(), v.size() * sizeof(int));
+  };
+  cp(a, g.xN2o);
+  cp(b, g.yN2o);
+  cp(c, g.xo2N);
+  cp(d, g.yo2N);
+  cp(e, g.xBx2N);
+  cp(f, g.yBy2N);
+  return TCSEM_OK;
+}
+
+int tcsem_waveform(int ns, double sigma_t, double *t, double *dt, double *wave) {
+  if (!t || !dt || !wave) return fail(TCSEM_EINVAL, "null argument");
+  return waveform(ns, sigma_t, t, dt, wave);
+}
+
+int tcsem_create(const tcsem_setup *s, const double *cond, int precision, tcsem_ctx **out) {
+  if (!s || !cond || !out) return fail(TCSEM_EINVAL, "null argument");
+  if (precision != TCSEM_FP64 && precision != TCSEM_FP32) return fail(TCSEM_EINVAL, "precision must be 32 or 64");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return fail(TCSEM_EDEVICE, "no HIP device");
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HIPCHK(hipGetDeviceProperties(&prop, dev));
+  if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+    return fail(TCSEM_EDEVICE, "device %s is not gfx950 (MI355X); this build targets gfx950 only", prop.gcnArchName);
+  tcsem_ctx *c = new tcsem_ctx();
+  c->prec = precision;
+  c->esz = precision == TCSEM_FP64 ? 8 : 4;
+  c->setup = *s;
+  int rc = build_grid(*s, c->g);
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  build_maps(*s, c->g);
+  const tcsem_grid_info &I = c->g.info;
+  const int L = I.L, M = I.M, N = I.N;
+  if (s->n_src / 2 < 1 || !(s->sigma_t > 0)) {
+    delete c;
+    return fail(TCSEM_EINVAL, "source discretisation must be >= 2 and sigma_t > 0");
+  }
+  const size_t ncell = (size_t)L * M * N;
+  c->cond.assign(cond, cond + ncell);
+  for (size_t q = 0; q < ncell; ++q)
+    if (!(c->cond[q] >= 0.0) || !std::isfinite(c->cond[q])) {
+      delete c;
+      return fail(TCSEM_EINVAL, "conductivity must be finite and >= 0 (cell %zu)", q);
+    }
+  c->PX = ((L + 3) + 63) / 64 * 64;
+  c->PY = M + 3;
+  c->PS = (long long)c->PX * c->PY;
+  c->fstride = (long long)(N + 1) * c->PS;
+  c->pstride = c->PS;
+  c->wstride = (long long)I.L1 * I.M1;
+  c->ntx = (L + 1 + kMarchOwnX - 1) / kMarchOwnX;
+  c->nty = (M + 1 + kMarchOwnY - 1) / kMarchOwnY;
+  c->nper = (c->ntx * c->nty + 7) / 8;
+  // air-plane operators
+  c->n1 = c->g.AL + kNPad;
+  c->n2 = c->g.AR + kNPad;
+  c->n3 = c->g.AY + kNPad;
+  {
+    const HostGrid &g = c->g;
+    const int aL = I.AnLx_New, aR = I.AnRx_New, aY = I.Any_New, Nx = s->Nx, Ny = s->Ny;
+    const int ox = Nx + g.AL - kNPad, oy = Ny + g.AY - kNPad;
+    std::vector<int> idr(std::max(std::max(aL, aR), aY));
+    for (int p = 0; p < aL; ++p) idr[p] = g.xN2o[p];
+    spline_weights(c->n1, g.dx.data(), g.X.data(), aL, g.Xn.data(), idr.data(), c->wl);
+    for (int p = 0; p < aR; ++p) idr[p] = g.xN2o[aL + Nx + p] - ox;
+    spline_weights(c->n2, g.dx.data() + ox, g.X.data() + ox, aR, g.Xn.data() + aL + Nx, idr.data(), c->wr);
+    for (int p = 0; p < aY; ++p) idr[p] = g.yN2o[p];
+    spline_weights(c->n3, g.dy.data(), g.Y.data(), aY, g.Yn.data(), idr.data(), c->wf);
+    for (int p = 0; p < aY; ++p) idr[p] = g.yN2o[aY + Ny + p] - oy;
+    spline_weights(c->n3, g.dy.data() + oy, g.Y.data() + oy, aY, g.Yn.data() + aY + Ny, idr.data(), c->wb);
+    // wavenumber gains on the half spectrum (UpContinuation_gpu.cu:113-164)
+    const int L1 = I.L1, M1 = I.M1, H = L1 / 2 + 1;
+    c->gx.assign((size_t)M1 * H, 0.0);
+    c->gy.assign((size_t)M1 * H, 0.0);
+    const double dzdx = s->dzmin / s->dxmin, dzdy = s->dzmin / s->dymin;
+    const double rxy = s->dxmin / s->dymin, ryx = s->dymin / s->dxmin;
+    for (int iy = 0; iy < M1; ++iy)
+      for (int ix = 0; ix < H; ++ix) {
+        const double v1 = (iy < M1 / 2 + 1) ? (1.0 * kPi * iy) / M1 * dzdy : -(1.0 * kPi * (M1 - iy)) / M1 * dzdy;
+        const double u1 = (1.0 * kPi * ix) / L1 * dzdx;
+        double gxv = 1.0, gyv = 1.0;
+        if (!(ix == 0 && iy == 0)) {
+          const double rr = u1 * u1 + v1 * v1;
+          const double upcc = std::exp(-std::sqrt(rr)) * (1.0 / std::sqrt(rr));
+          gxv = rxy * u1 * upcc;
+          gyv = ryx * v1 * upcc;
+        }
+        c->gx[(size_t)iy * H + ix] = gxv;
+        c->gy[(size_t)iy * H + ix] = gyv;
+      }
+    make_plan(L1, c->plx);
+    make_plan(M1, c->ply);
+  }
+  const size_t lds_max = 160 * 1024 - 1024;
+  if ((size_t)c->ply.m * 16 + (size_t)M * 8 > lds_max || (size_t)c->plx.m * 16 > lds_max) {
+    delete c;
+    return fail(TCSEM_ESIZE, "air plane %d x %d needs FFT lengths %d/%d beyond the LDS budget", I.L1, I.M1,
+                c->plx.m, c->ply.m);
+  }
+  if (hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return fail(TCSEM_EDEVICE, "stream creation failed");
+  }
+  for (auto &e : c->ev)
+    if (hipEventCreate(&e) != hipSuccess) {
+      tcsem_destroy(c);
+      return fail(TCSEM_EDEVICE, "event creation failed");
+    }
+  // device constants
+  auto bail = [&](int code) {
+    tcsem_destroy(c);
+    return code;
+  };
+  {
+    std::vector<double> pad((size_t)N * c->PS, 0.0);
+    for (int k = 0; k < N; ++k)
+      for (int j = 0; j < M; ++j)
+        for (int i = 0; i < L; ++i)
+          pad[(size_t)k * c->PS + (size_t)(j + 1) * c->PX + (i + 1)] = c->cond[((size_t)k * M + j) * L + i];
+    if (hipMalloc(&c->d_sig, pad.size() * c->esz) != hipSuccess) return bail(fail(TCSEM_ENOMEM, "sigma allocation"));
+    if ((rc = upload_double_as(c, c->d_sig, pad))) return bail(rc);
+  }
+  {
+    const HostGrid &g = c->g;
+    std::vector<double> rd(2 * (L + 3) + 2 * (M + 3) + 3 * N, 0.0);
+    double *rdx = rd.data(), *rdx2 = rdx + (L + 3), *rdy = rdx2 + (L + 3), *rdy2 = rdy + (M + 3),
+           *dzt = rdy2 + (M + 3), *rdz = dzt + N, *rdz2 = rdz + N;
+    for (int i = 0; i < L; ++i) rdx[i + 1] = 1.0 / g.dx[i];
+    for (int i = 1; i < L; ++i) rdx2[i + 1] = 1.0 / (g.dx[i - 1] + g.dx[i]);
+    for (int j = 0; j < M; ++j) rdy[j + 1] = 1.0 / g.dy[j];
+    for (int j = 1; j < M; ++j) rdy2[j + 1] = 1.0 / (g.dy[j - 1] + g.dy[j]);
+    for (int k = 0; k < N; ++k) {
+      dzt[k] = g.dz[k];
+      rdz[k] = 1.0 / g.dz[k];
+      rdz2[k] = k ? 1.0 / (g.dz[k] + g.dz[k - 1]) : 1.0 / (2.0 * g.dz[0]);
+    }
+    if (hipMalloc(&c->d_rd, rd.size() * c->esz) != hipSuccess) return bail(fail(TCSEM_ENOMEM, "grid allocation"));
+    if ((rc = upload_double_as(c, c->d_rd, rd))) return bail(rc);
+    std::vector<double> gd;
+    for (auto *v : {&g.X, &g.Y, &g.Z, &g.dx, &g.dy, &g.dz, &g.Xn, &g.Yn}) gd.insert(gd.end(), v->begin(), v->end());
+    if (hipMalloc((void **)&c->d_grid, gd.size() * 8) != hipSuccess) return bail(fail(TCSEM_ENOMEM, "grid allocation"));
+    if (hipMemcpy(c->d_grid, gd.data(), gd.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(TCSEM_EDEVICE, "grid upload"));
+    std::vector<int> mp;
+    for (auto *v : {&g.xBx2N, &g.yo2N, &g.xo2N, &g.yBy2N}) mp.insert(mp.end(), v->begin(), v->end());
+    if (hipMalloc((void **)&c->d_maps, mp.size() * 4) != hipSuccess) return bail(fail(TCSEM_ENOMEM, "map allocation"));
+    if (hipMemcpy(c->d_maps, mp.data(), mp.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(TCSEM_EDEVICE, "map upload"));
+    std::vector<double> up;
+    for (auto *v : {&c->wl, &c->wr, &c->wf, &c->wb, &c->gx, &c->gy}) up.insert(up.end(), v->begin(), v->end());
+    for (auto *P : {&c->plx, &c->ply})
+      for (auto *v : {&P->tw, &P->chirp, &P->bhat}) up.insert(up.end(), v->begin(), v->end());
+    if (hipMalloc((void **)&c->d_up, up.size() * 8) != hipSuccess) return bail(fail(TCSEM_ENOMEM, "operator allocation"));
+    if (hipMemcpy(c->d_up, up.data(), up.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(TCSEM_EDEVICE, "operator upload"));
+  }
+  if ((rc = alloc_slots(c, 1))) return bail(rc);
+  c->nbatch = 1;
+  c->up64 = upcont_args<double>(c);
+  c->up32 = upcont_args<float>(c);
+  // empty per-slot source/receiver tables so single-slot kernels can run before a batch starts
+  {
+    std::vector<int> z(2, 0);
+    if (hipMalloc((void **)&c->d_srcidx, 2 * sizeof(int)) != hipSuccess) return bail(fail(TCSEM_ENOMEM, "alloc"));
+    if (hipMemcpy(c->d_srcidx, z.data(), 2 * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+      return bail(fail(TCSEM_EDEVICE, "upload"));
+  }
+  *out = c;
+  return TCSEM_OK;
+}
+
+void tcsem_destroy(tcsem_ctx *c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_run(c);
+  (void)hipFree(c->d_fields);
+  (void)hipFree(c->d_sig);
+  (void)hipFree(c->d_planes);
+  (void)hipFree(c->d_rd);
+  (void)hipFree(c->d_grid);
+  (void)hipFree(c->d_maps);
+  (void)hipFree(c->d_up);
+  (void)hipFree(c->d_work);
+  (void)hipFree(c->d_scratch);
+  (void)hipFree(c->d_rx);
+  (void)hipFree(c->d_rxi);
+  for (auto &e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int tcsem_info(const tcsem_ctx *c, tcsem_grid_info *out) {
+  if (!c || !out) return fail(TCSEM_EINVAL, "null argument");
+  *out = c->g.info;
+  return TCSEM_OK;
+}
+
+int tcsem_set_receivers(tcsem_ctx *c, const tcsem_rxtime *rt, int nrx, const double *xyz) {
+  if (check_ctx(c) || !rt || (nrx > 0 && !xyz) || nrx < 0) return fail(TCSEM_EINVAL, "bad receiver arguments");
+  if (rt->Nt < 2) return fail(TCSEM_EINVAL, "need at least 2 sample times");
+  if (rt->is_log && !(rt->T0 > 0)) return fail(TCSEM_EINVAL, "log-spaced sampling needs T0 > 0");
+  const tcsem_grid_info &I = c->g.info;
+  const int L = I.L, M = I.M, K = I.N;
+  std::vector<double> x(nrx), y(nrx), z(nrx);
+  std::vector<int> ix(nrx), iy(nrx), iz(nrx);
+  for (int r = 0; r < nrx; ++r) {  // LoadRxTxPos.cpp:93-121
+    x[r] = xyz[3 * r];
+    y[r] = xyz[3 * r + 1];
+    z[r] = xyz[3 * r + 2];
+    ix[r] = search_insert(c->g.X.data(), L, x[r]);
+    iy[r] = search_insert(c->g.Y.data(), M, y[r]);
+    iz[r] = search_insert(c->g.Z.data(), K, z[r]);
+    // the 4-point stencils need two nodes on each side (the reference reads outside the arrays)
+    if (ix[r] < 2 || ix[r] > L - 3) return fail(TCSEM_EDOMAIN, "receiver %d too close to the x boundary", r + 1);
+    if (iy[r] < 2 || iy[r] > M - 3) return fail(TCSEM_EDOMAIN, "receiver %d too close to the y boundary", r + 1);
+    if (iz[r] > K - 3) return fail(TCSEM_EDOMAIN, "receiver %d too close to the bottom", r + 1);
+  }
+  free_run(c);
+  c->rt = *rt;
+  c->nrx = nrx;
+  c->rxx = x;
+  c->rxy = y;
+  c->rxz = z;
+  c->rix = ix;
+  c->riy = iy;
+  c->riz = iz;
+  c->ncomp = 0;
+  c->nefcmp = 0;
+  for (int q = 0; q < 6; ++q)
+    if (rt->is_comp[q]) c->comps[c->ncomp++] = q;
+  for (int q = 0; q < 3; ++q) c->nefcmp += rt->is_comp[q] != 0;
+  sample_times(*rt, c->times);
+  (void)hipFree(c->d_rx);
+  (void)hipFree(c->d_rxi);
+  c->d_rx = nullptr;
+  c->d_rxi = nullptr;
+  const int nn = std::max(nrx, 1);
+  std::vector<double> hx(3 * nn, 0.0);
+  std::vector<int> hi(3 * nn, 0);
+  for (int r = 0; r < nrx; ++r) {
+    hx[r] = x[r];
+    hx[nn + r] = y[r];
+    hx[2 * nn + r] = z[r];
+    hi[r] = ix[r];
+    hi[nn + r] = iy[r];
+    hi[2 * nn + r] = iz[r];
+  }
+  if (hipMalloc((void **)&c->d_rx, hx.size() * 8) != hipSuccess || hipMalloc((void **)&c->d_rxi, hi.size() * 4) != hipSuccess)
+    return fail(TCSEM_ENOMEM, "receiver allocation");
+  HIPCHK(hipMemcpy(c->d_rx, hx.data(), hx.size() * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_rxi, hi.data(), hi.size() * 4, hipMemcpyHostToDevice));
+  // receiver tables are addressed with stride nn; keep nrx for id checks
+  c->nrx = nrx;
+  c->have_rx = true;
+  // re-locate transmitters against the new receiver list
+  c->tx.clear();
+  return TCSEM_OK;
+}
+
+int tcsem_set_transmitters(tcsem_ctx *c, const tcsem_tx *tx, int ntx) {
+  if (check_ctx(c) || (ntx > 0 && !tx) || ntx < 0) return fail(TCSEM_EINVAL, "bad transmitter arguments");
+  if (!c->have_rx) return fail(TCSEM_ESTATE, "set receivers before transmitters");
+  std::vector<TxLoc> loc(ntx);
+  for (int q = 0; q < ntx; ++q) {
+    if (tx[q].nr > 0 && !tx[q].rx_ids) return fail(TCSEM_EINVAL, "transmitter %d has no receiver list", tx[q].id);
+    const int rc = locate_tx(c, tx[q], loc[q]);
+    if (rc) return rc;
+    source_cells(c, loc[q]);
+  }
+  free_run(c);
+  c->tx = std::move(loc);
+  return TCSEM_OK;
+}
+
+int tcsem_reserve_batch(tcsem_ctx *c, int nbatch) {
+  if (check_ctx(c) || nbatch < 1) return fail(TCSEM_EINVAL, "bad batch size");
+  const int rc = alloc_slots(c, nbatch);
+  if (rc) return rc;
+  c->up64 = upcont_args<double>(c);
+  c->up32 = upcont_args<float>(c);
+  return TCSEM_OK;
+}
+
+int tcsem_reset_fields(tcsem_ctx *c, int first, int n) {
+  if (check_ctx(c) || first < 0 || n < 0 || first + n > c->nslots) return fail(TCSEM_EINVAL, "bad slot range");
+  const size_t fb = (size_t)c->fstride * c->esz, pb = (size_t)c->pstride * c->esz;
+  for (int f = 0; f < 10; ++f)
+    HIPCHK(launch_fill_zero(c->d_fields + ((size_t)f * c->nslots + first) * fb, fb * n, c->stream));
+  for (int f = 0; f < 3; ++f)
+    HIPCHK(launch_fill_zero(c->d_planes + ((size_t)f * c->nslots + first) * pb, pb * n, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return TCSEM_OK;
+}
+
+int tcsem_begin(tcsem_ctx *c, int tx_first, int nbatch) {
+  if (check_ctx(c)) return TCSEM_EINVAL;
+  if (nbatch < 1) return fail(TCSEM_EINVAL, "batch must hold at least one transmitter");
+  const bool have_tx = !c->tx.empty();
+  if (have_tx && (tx_first < 0 || tx_first + nbatch > (int)c->tx.size()))
+    return fail(TCSEM_EINVAL, "transmitters [%d, %d) outside the %zu set", tx_first, tx_first + nbatch, c->tx.size());
+  int rc = tcsem_reserve_batch(c, nbatch);
+  if (rc) return rc;
+  free_run(c);
+  c->tx_first = tx_first;
+  c->nbatch = nbatch;
+  if ((rc = tcsem_reset_fields(c, 0, nbatch))) return rc;
+  // sources (per slot)
+  std::vector<SrcCell> cells;
+  std::vector<int> idx(2 * nbatch, 0);
+  c->src_kmin = 1 << 30;
+  c->src_kmax = -1;
+  for (int b = 0; b < nbatch; ++b) {
+    idx[b] = (int)cells.size();
+    if (have_tx) {
+      const TxLoc &t = c->tx[tx_first + b];
+      for (const SrcCell &s : t.cells) {
+        cells.push_back(s);
+        c->src_kmin = std::min(c->src_kmin, s.k);
+        c->src_kmax = std::max(c->src_kmax, s.k);
+      }
+    }
+    idx[nbatch + b] = (int)cells.size() - idx[b];
+  }
+  if (cells.empty()) cells.push_back({-1, -1, -1, -1, 0.0});
+  if (hipMalloc((void **)&c->d_src, cells.size() * sizeof(SrcCell)) != hipSuccess ||
+      hipMalloc((void **)&c->d_srcidx, idx.size() * sizeof(int)) != hipSuccess)
+    return fail(TCSEM_ENOMEM, "source table allocation");
+  HIPCHK(hipMemcpy(c->d_src, cells.data(), cells.size() * sizeof(SrcCell), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_srcidx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
+  // receivers (per slot)
+  c->nrmax = 1;
+  if (c->have_rx) {
+    std::vector<int> rl(2 * nbatch, 0);
+    std::vector<int> loc;
+    for (int b = 0; b < nbatch; ++b) {
+      rl[b] = (int)loc.size();
+      if (have_tx) {
+        const TxLoc &t = c->tx[tx_first + b];
+        loc.insert(loc.end(), t.rx.begin(), t.rx.end());
+        rl[nbatch + b] = (int)t.rx.size();
+        c->nrmax = std::max(c->nrmax, (int)t.rx.size());
+      }
+    }
+    rl.insert(rl.end(), loc.begin(), loc.end());
+    if (hipMalloc((void **)&c->d_rxl, std::max<size_t>(rl.size(), 1) * sizeof(int)) != hipSuccess)
+      return fail(TCSEM_ENOMEM, "receiver list allocation");
+    HIPCHK(hipMemcpy(c->d_rxl, rl.data(), rl.size() * sizeof(int), hipMemcpyHostToDevice));
+    const size_t ne0 = (size_t)nbatch * 2 * c->ncomp * c->nrmax, nes = (size_t)nbatch * c->nrmax * c->ncomp * c->rt.Nt;
+    if (hipMalloc((void **)&c->d_ef0, std::max<size_t>(ne0, 1) * 8) != hipSuccess ||
+        hipMalloc((void **)&c->d_efs, std::max<size_t>(nes, 1) * 8) != hipSuccess ||
+        hipMalloc((void **)&c->d_ops, (size_t)2 * c->rt.Nt * sizeof(int)) != hipSuccess ||
+        hipMalloc((void **)&c->d_opl1, (size_t)2 * c->rt.Nt * sizeof(double)) != hipSuccess)
+      return fail(TCSEM_ENOMEM, "trace allocation");
+    // ef0 is zeroed (the reference leaves it uninitialised, main.cu:208)
+    HIPCHK(launch_fill_zero(c->d_ef0, std::max<size_t>(ne0, 1) * 8, c->stream));
+    HIPCHK(launch_fill_zero(c->d_efs, std::max<size_t>(nes, 1) * 8, c->stream));
+  }
+  // schedule state (main.cu:212-217), reset per batch
+  const int ns = c->setup.n_src / 2;
+  c->wt.assign(2 * ns + 1, 0);
+  c->wdt.assign(2 * ns + 2, 0);
+  c->wwave.assign(2 * ns + 1, 0);
+  if ((rc = waveform(ns, c->setup.sigma_t, c->wt.data(), c->wdt.data(), c->wwave.data()))) return rc;
+  c->n = 0;
+  c->steps = 0;
+  c->ne = c->nb = 0;
+  std::memset(c->Tpof, 0, sizeof c->Tpof);
+  c->toT = 0;
+  c->dt1 = c->wdt[0];
+  c->dt2 = c->wdt[1];
+  c->Tpof[0][1] = c->toT + c->dt1;
+  c->Tpof[1][1] = c->toT + c->dt1 + 0.5 * c->dt2;
+  c->op_it.clear();
+  c->op_l1.clear();
+  c->up64 = upcont_args<double>(c);
+  c->up32 = upcont_args<float>(c);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->begun = true;
+  return TCSEM_OK;
+}
+
+int tcsem_step(tcsem_ctx *c, int nsteps) {
+  if (check_ctx(c)) return TCSEM_EINVAL;
+  if (!c->begun) return fail(TCSEM_ESTATE, "tcsem_begin first");
+  for (int q = 0; q < nsteps; ++q) {
+    const int rc = c->prec == 64 ? do_step<double>(c) : do_step<float>(c);
+    if (rc) return rc;
+  }
+  return TCSEM_OK;
+}
+
+int tcsem_steps_remaining(const tcsem_ctx *c) {
+  if (!c || !c->begun || !c->have_rx) return -1;
+  return sampling_done(c) ? 0 : 1;
+}
+
+int tcsem_get_traces(tcsem_ctx *c, int slot, double *traces) {
+  if (check_ctx(c) || !traces) return fail(TCSEM_EINVAL, "null argument");
+  if (!c->begun || !c->have_rx) return fail(TCSEM_ESTATE, "no run in progress");
+  if (slot < 0 || slot >= c->nbatch) return fail(TCSEM_EINVAL, "slot out of range");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  const int nr = c->tx.empty() ? 0 : (int)c->tx[c->tx_first + slot].rx.size();
+  const size_t per = (size_t)c->nrmax * c->ncomp * c->rt.Nt;
+  std::vector<double> h(per);
+  HIPCHK(hipMemcpy(h.data(), c->d_efs + per * slot, per * 8, hipMemcpyDeviceToHost));
+  std::memcpy(traces, h.data(), (size_t)nr * c->ncomp * c->rt.Nt * 8);
+  return TCSEM_OK;
+}
+
+int tcsem_run(tcsem_ctx *c, int tx_first, int ntx, int max_steps, double *traces, int *steps_done) {
+  if (check_ctx(c)) return TCSEM_EINVAL;
+  if (!c->have_rx || c->tx.empty()) return fail(TCSEM_ESTATE, "set receivers and transmitters first");
+  int rc = tcsem_begin(c, tx_first, ntx);
+  if (rc) return rc;
+  while (!sampling_done(c) && (max_steps <= 0 || c->steps < max_steps)) {
+    if ((rc = tcsem_step(c, 1))) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (steps_done) *steps_done = c->steps;
+  if (traces) {
+    size_t off = 0;
+    for (int b = 0; b < ntx; ++b) {
+      if ((rc = tcsem_get_traces(c, b, traces + off))) return rc;
+      off += (size_t)c->tx[tx_first + b].rx.size() * c->ncomp * c->rt.Nt;
+    }
+  }
+  return TCSEM_OK;
+}
+
+int tcsem_get_field(tcsem_ctx *c, int slot, int comp, double *out) {
+  if (check_ctx(c) || !out) return fail(TCSEM_EINVAL, "null argument");
+  if (slot < 0 || slot >= c->nslots) return fail(TCSEM_EINVAL, "slot out of range");
+  const int L = c->g.info.L, M = c->g.info.M, N = c->g.info.N;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (comp == TCSEM_BZ) {
+    void *tmp = nullptr;
+    const size_t n = (size_t)(N + 1) * M * L;
+    if (hipMalloc(&tmp, n * c->esz) != hipSuccess) return fail(TCSEM_ENOMEM, "Bz read-back allocation");
+    hipError_t e = c->prec == 64 ? launch_bz_volume<double>(march_args<double>(c), (double *)tmp, slot, c->stream)
+                                 : launch_bz_volume<float>(march_args<float>(c), (float *)tmp, slot, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    std::vector<char> h(n * c->esz);
+    if (e == hipSuccess) e = hipMemcpy(h.data(), tmp, n * c->esz, hipMemcpyDeviceToHost);
+    (void)hipFree(tmp);
+    if (e != hipSuccess) return fail(TCSEM_EDEVICE, "Bz read-back: %s", hipGetErrorString(e));
+    for (size_t q = 0; q < n; ++q) out[q] = c->prec == 64 ? ((double *)h.data())[q] : ((float *)h.data())[q];
+    return TCSEM_OK;
+  }
+  int nk, nj, ni;
+  const char *base;
+  size_t bytes;
+  if (comp >= TCSEM_EX && comp <= TCSEM_BY) {
+    static const int dk[5] = {1, 1, 0, 0, 0}, dj[5] = {1, 0, 1, 0, 1}, di[5] = {0, 1, 1, 1, 0};
+    nk = N + dk[comp];
+    nj = M + dj[comp];
+    ni = L + di[comp];
+    base = c->d_fields + ((size_t)(c->cur * 5 + comp) * c->nslots + slot) * c->fstride * c->esz;
+    bytes = (size_t)c->fstride * c->esz;
+  } else if (comp == TCSEM_BXAIR || comp == TCSEM_BYAIR) {
+    nk = 1;
+    nj = comp == TCSEM_BXAIR ? M : M + 1;
+    ni = comp == TCSEM_BXAIR ? L + 1 : L;
+    base = c->d_planes + ((size_t)(comp == TCSEM_BXAIR ? 1 : 2) * c->nslots + slot) * c->pstride * c->esz;
+    bytes = (size_t)c->pstride * c->esz;
+  } else {
+    return fail(TCSEM_EINVAL, "unknown component %d", comp);
+  }
+  std::vector<char> h(bytes);
+  HIPCHK(hipMemcpy(h.data(), base, bytes, hipMemcpyDeviceToHost));
+  for (int k = 0; k < nk; ++k)
+    for (int j = 0; j < nj; ++j)
+      for (int i = 0; i < ni; ++i) {
+        const size_t src = (size_t)k * c->PS + (size_t)(j + 1) * c->PX + (i + 1);
+        out[((size_t)k * nj + j) * ni + i] = c->prec == 64 ? ((double *)h.data())[src] : ((float *)h.data())[src];
+      }
+  return TCSEM_OK;
+}
+
+int tcsem_set_field(tcsem_ctx *c, int slot, int comp, const double *in) {
+  if (check_ctx(c) || !in) return fail(TCSEM_EINVAL, "null argument");
+  if (slot < 0 || slot >= c->nslots) return fail(TCSEM_EINVAL, "slot out of range");
+  const int L = c->g.info.L, M = c->g.info.M, N = c->g.info.N;
+  int nk, nj, ni;
+  char *base;
+  size_t bytes;
+  if (comp >= TCSEM_EX && comp <= TCSEM_BY) {
+    static const int dk[5] = {1, 1, 0, 0, 0}, dj[5] = {1, 0, 1, 0, 1}, di[5] = {0, 1, 1, 1, 0};
+    nk = N + dk[comp];
+    nj = M + dj[comp];
+    ni = L + di[comp];
+    base = c->d_fields + ((size_t)(c->cur * 5 + comp) * c->nslots + slot) * c->fstride * c->esz;
+    bytes = (size_t)c->fstride * c->esz;
+  } else if (comp == TCSEM_BXAIR || comp == TCSEM_BYAIR) {
+    nk = 1;
+    nj = comp == TCSEM_BXAIR ? M : M + 1;
+    ni = comp == TCSEM_BXAIR ? L + 1 : L;
+    base = c->d_planes + ((size_t)(comp == TCSEM_BXAIR ? 1 : 2) * c->nslots + slot) * c->pstride * c->esz;
+    bytes = (size_t)c->pstride * c->esz;
+  } else {
+    return fail(TCSEM_EINVAL, "component %d is not settable (Bz is derived from Bx, By)", comp);
+  }
+  std::vector<char> h(bytes, 0);
+  for (int k = 0; k < nk; ++k)
+    for (int j = 0; j < nj; ++j)
+      for (int i = 0; i < ni; ++i) {
+        const size_t dst = (size_t)k * c->PS + (size_t)(j + 1) * c->PX + (i + 1);
+        const double v = in[((size_t)k * nj + j) * ni + i];
+        if (c->prec == 64) ((double *)h.data())[dst] = v;
+        else ((float *)h.data())[dst] = (float)v;
+      }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(base, h.data(), bytes, hipMemcpyHostToDevice));
+  return TCSEM_OK;
+}
+
+int tcsem_upcont_plane(tcsem_ctx *c, const double *bz0, double *bxair, double *byair) {
+  if (check_ctx(c) || !bz0 || !bxair || !byair) return fail(TCSEM_EINVAL, "null argument");
+  const int L = c->g.info.L, M = c->g.info.M;
+  std::vector<char> h((size_t)c->pstride * c->esz, 0);
+  for (int j = 0; j < M; ++j)
+    for (int i = 0; i < L; ++i) {
+      const size_t dst = (size_t)(j + 1) * c->PX + (i + 1);
+      if (c->prec == 64) ((double *)h.data())[dst] = bz0[(size_t)j * L + i];
+      else ((float *)h.data())[dst] = (float)bz0[(size_t)j * L + i];
+    }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(c->d_planes, h.data(), h.size(), hipMemcpyHostToDevice));
+  hipError_t e = c->prec == 64 ? launch_upcont<double>(c->up64, 1, c->stream) : launch_upcont<float>(c->up32, 1, c->stream);
+  if (e != hipSuccess) return fail(TCSEM_EDEVICE, "upward continuation: %s", hipGetErrorString(e));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int rc = tcsem_get_field(c, 0, TCSEM_BXAIR, bxair);
+  if (rc) return rc;
+  return tcsem_get_field(c, 0, TCSEM_BYAIR, byair);
+}
+
+int tcsem_enable_timing(tcsem_ctx *c, int on) {
+  if (check_ctx(c)) return TCSEM_EINVAL;
+  c->timing = on != 0;
+  for (int q = 0; q < 4; ++q) {
+    c->tacc[q] = 0;
+    c->tn[q] = 0;
+  }
+  return TCSEM_OK;
+}
+
+int tcsem_timing(const tcsem_ctx *c, double t[4], long n[4]) {
+  if (!c || !t || !n) return fail(TCSEM_EINVAL, "null argument");
+  for (int q = 0; q < 4; ++q) {
+    n[q] = c->tn[q];
+    t[q] = c->tn[q] ? c->tacc[q] / c->tn[q] : 0.0;
+  }
+  return TCSEM_OK;
+}
+
+int tcsem_sync(tcsem_ctx *c) {
+  if (check_ctx(c)) return TCSEM_EINVAL;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return TCSEM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,123 @@
+// tcsem_internal.h -- types shared by the host runtime (tcsem_host.cpp) and the gfx950 kernels.
+//
+// Field layout in HBM (per field, per batch slot): a zero margin ring around a pitched
+// (k, j, i) volume, x fastest.  Element (k, j, i) lives at origin[k*PS + j*PX + i] with
+// origin = allocation + PX + 1, so the halo indices i = -1 and j = -1 are addressable and
+// read as zero.  PX >= L + 3 (rounded to 64 elements), PY = M + 3 rows per level,
+// PS = PX * PY; every field has N + 1 levels.  The conductivity uses the same layout (N levels,
+// shared by all slots).  The surface planes (Bz0, BxAir, ByAir) use one level of the same
+// pitch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace tcsem {
+
+constexpr double kPi = 3.141592653589793238;  // FTDT.h:39
+constexpr double kMu0 = 4 * kPi * 1.e-7;      // main.cu:55
+constexpr int kNPad = 4;                      // FTDT.h:40
+
+// Volume march tile: 64 lanes in x (62 owned columns + one halo column each side) by
+// kMarchRows rows in y (kMarchRows - 2 owned rows + one halo row each side).
+constexpr int kMarchLanes = 64;
+constexpr int kMarchOwnX = kMarchLanes - 2;
+constexpr int kMarchRows = 8;
+constexpr int kMarchOwnY = kMarchRows - 2;
+
+// One injected E cell: E[comp](k, j, i) -= amp * w each source step (ElectricalFiled.cu:347-475).
+struct SrcCell {
+  int comp;  // 0 Ex, 1 Ey, 2 Ez
+  int k, j, i;
+  double w;
+};
+
+// The field state is double-buffered: a step reads the old state (the *i pointers) and writes
+// the new one (the *o pointers), so workgroups never see a neighbour's half-updated halo.
+template <typename T>
+struct MarchArgs {
+  const T *exi, *eyi, *ezi, *bxi, *byi;  // old state, origin of slot 0
+  T *ex, *ey, *ez, *bx, *by;             // new state, origin of slot 0
+  const T *sig;               // origin, shared by all slots
+  T *bz0;                     // Bz at level 0 (plane origin, slot 0)
+  const T *bxair, *byair;     // air planes (origin, slot 0)
+  long long fstride;          // elements between slots of one field
+  long long pstride;          // elements between slots of one plane
+  long long PS;               // plane stride of a volume
+  int L, M, N, PX;
+  int ntx, nty, nper;         // tiles in x, y; tiles per XCD group
+  const T *rdx, *rdx2, *rdy, *rdy2;  // 1/dx[i], 1/(dx[i-1]+dx[i]), same for y (index+1 origin)
+  const T *dzt, *rdz, *rdz2;         // dz[k], 1/dz[k], 1/(dz[k-1]+dz[k]) (rdz2[0] = 1/(2 dz[0]))
+  double lam, dt, dtb, amp;          // step scalars (FP64, main.cu:223-260)
+  const SrcCell *src;                // all cells of all slots
+  const int *src_first, *src_count;  // per slot
+  int src_kmin, src_kmax;            // level range holding any source cell
+};
+
+// Receiver gather descriptor (EFInterp2Rx.cu:60-470) for one field group (E or B) of one event.
+struct GatherArgs {
+  const void *f[6];          // Ex Ey Ez Bx By (origin, slot 0); Bz entry unused
+  long long fstride, PS;
+  int L, M, N, PX;
+  const double *X, *Y, *Z, *dx, *dy, *dz;  // host-equal FP64 grid
+  const int *rx_local;       // per slot: nr local receiver indices into the receiver tables
+  const int *rx_first;       // per slot offset into rx_local
+  const int *rx_count;       // per slot count
+  const double *rxx, *rxy, *rxz;
+  const int *rix, *riy, *riz;
+  int comps[6];              // component ids of this group (field order), ncomp_group entries
+  int ncomp_group, comp_base;  // index of the first group component among NComp
+  int ncomp, nt;             // NComp, Nt
+  int slot_store;            // 0 or 1: ef0 slot written
+  double *ef0;               // [slot][2][NComp][nrmax]
+  double *efs;               // [slot][nrmax][NComp][Nt]
+  int nrmax;
+  const int *op_it;          // time-interpolation ops of this event
+  const double *op_l1;
+  int nops;
+};
+
+struct FftPlan {
+  int n;            // transform length
+  int m;            // power-of-two working length (n, or Bluestein length >= 2n-1)
+  int log2m;
+  const double *tw;      // m/2 complex twiddles exp(-2 pi i k / m), interleaved re,im
+  const double *chirp;   // n complex w_k = exp(-i pi k^2 / n) (Bluestein only)
+  const double *bhat;    // m complex FFT of the conjugate chirp, scaled by 1/m (Bluestein only)
+};
+
+template <typename T>
+struct UpcontArgs {
+  const T *bz0;            // plane origin (slot 0), pitch PX
+  T *bxair, *byair;        // plane origins (slot 0)
+  long long pstride;
+  int L, M, PX, L1, M1, AnLx, AnRx, Any, AL, AY, Nx, Ny;
+  int n1, n2, n3;          // spline window lengths
+  const double *wl, *wr;   // x-window weights: AnLx x n1, AnRx x n2
+  const double *wf, *wb;   // y-window weights: Any x n3 (front, back)
+  double2 *cplx;           // per slot M1 x L1 complex work (column spectra / row results)
+  double2 *cplx2;          // per slot M1 x L1 second complex work
+  double *bxr, *byr;       // per slot M1 x L1 real results
+  long long wstride;       // elements between slots of the work planes
+  FftPlan px, py;          // x (length L1) and y (length M1) plans
+  const double *gx, *gy;   // filter gains on the half spectrum: M1 x (L1/2+1)
+  int H;                   // L1/2+1
+  // bilinear (UpContinuation_gpu.cu:42-110)
+  const double *X, *Y, *Xn, *Yn, *dx, *dy;
+  const int *xBx2N, *yo2N, *xo2N, *yBy2N;
+};
+
+// ---- launchers (tcsem_kernels.hip, tcsem_upcont.hip) ------------------------------------------
+template <typename T>
+hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s);
+template <typename T>
+hipError_t launch_surface(const MarchArgs<T> &a, int nslot, hipStream_t s);
+template <typename T>
+hipError_t launch_bz_volume(const MarchArgs<T> &a, T *bz, int slot, hipStream_t s);
+template <typename T>
+hipError_t launch_gather(const GatherArgs &g, int is_b, int nslot, const MarchArgs<T> &a,
+                         hipStream_t s);
+template <typename T>
+hipError_t launch_upcont(const UpcontArgs<T> &u, int nslot, hipStream_t s);
+hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);
+
+}  // namespace tcsem

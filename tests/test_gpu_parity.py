@@ -1,0 +1,154 @@
+"""Parity of the gfx950 HIP path (libtcsem.so, through the C-ABI) with the FP64 oracle.
+
+FP64 contexts are compared at round-off level (rel L2 <= 1e-10 per field after a few steps, the
+kernels reorder a few FP64 operations); FP32 contexts at the north-star tolerance of 1e-4
+relative trace error per receiver (components below 1e-6 of the receiver's largest norm -- the
+symmetry nulls -- excluded).
+"""
+import numpy as np
+import pytest
+
+import tcsem_fdtd_amd as T
+from oracle import oracle as O
+from tests import decks
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-10
+FP32_TRACE_TOL = 1e-4
+
+
+def _pair(setup_kw=None, cond_seed=7, blob=True):
+    kw = setup_kw or {}
+    so = decks.small_setup(O, **kw)
+    st = decks.small_setup(T, **kw)
+    m = O.Model(so)
+    g = m.grid()
+    cond = decks.layered_cond((m.N, m.M, m.L), g["Z"][:, None, None] * np.ones((1, m.M, m.L)),
+                              seed=cond_seed, blob=blob)
+    m = O.Model(so, cond)
+    return so, st, m, cond
+
+
+def _zero_walls(f, comp, L, M, N):
+    """Entries the reference never updates (PEC-like walls) are kept at zero."""
+    f = f.copy()
+    if comp == O.EX:
+        f[N:] = 0; f[:, 0] = 0; f[:, M] = 0
+    elif comp == O.EY:
+        f[N:] = 0; f[:, :, 0] = 0; f[:, :, L] = 0
+    elif comp == O.EZ:
+        f[:, 0] = 0; f[:, M] = 0; f[:, :, 0] = 0; f[:, :, L] = 0
+    elif comp == O.BX:
+        f[:, :, 0] = 0; f[:, :, L] = 0
+    elif comp == O.BY:
+        f[:, 0] = 0; f[:, M] = 0
+    return f
+
+
+def test_upcont_plane_fp64():
+    so, st, m, cond = _pair()
+    rng = np.random.default_rng(1)
+    g = m.grid()
+    X, Y = np.meshgrid(g["X"], g["Y"])
+    bz0 = np.exp(-(X ** 2 + Y ** 2) / 4e5) * (1 + 0.1 * rng.standard_normal(X.shape))
+    bxo, byo = m.upcont_plane(bz0)
+    with T.Context(st, cond, T.FP64) as c:
+        bx, by = c.upcont_plane(bz0)
+    assert decks.rel_l2(bx, bxo) < 1e-12
+    assert decks.rel_l2(by, byo) < 1e-12
+
+
+def test_upcont_plane_bluestein_fp64():
+    """A plane whose regular-grid sizes are not powers of two (Bluestein FFT lengths)."""
+    so, st, m, cond = _pair({"nx": 23, "ny": 17})
+    assert (m.L1 & (m.L1 - 1)) and (m.M1 & (m.M1 - 1))
+    rng = np.random.default_rng(2)
+    bz0 = rng.standard_normal((m.M, m.L))
+    bxo, byo = m.upcont_plane(bz0)
+    with T.Context(st, cond, T.FP64) as c:
+        bx, by = c.upcont_plane(bz0)
+    assert decks.rel_l2(bx, bxo) < 1e-11
+    assert decks.rel_l2(by, byo) < 1e-11
+
+
+def _random_state(m, seed):
+    rng = np.random.default_rng(seed)
+    st = {}
+    for comp in (O.EX, O.EY, O.EZ, O.BX, O.BY):
+        st[comp] = _zero_walls(rng.standard_normal(m.shape(comp)), comp, m.L, m.M, m.N)
+    return st
+
+
+@pytest.mark.parametrize("nsteps", [1, 4])
+def test_step_fp64_random_state(nsteps):
+    so, st, m, cond = _pair()
+    state = _random_state(m, 11)
+    for comp, f in state.items():
+        m.set(comp, f)
+    m.update_bz()
+    lam, dte, dtb, amp = O.schedule(so, m.mindx(), O.make_rxtime(1e-3, 1.0, 10, 1, [1] * 6), 100)
+    for n in range(nsteps):
+        m.step(lam[n], dte[n], amp[n], None, dtb[n])
+    with T.Context(st, cond, T.FP64) as c:
+        c.begin(0, 1)
+        for comp, f in state.items():
+            c.set(comp, f)
+        c.step(nsteps)
+        for comp in (O.EX, O.EY, O.EZ, O.BX, O.BY, O.BZ, O.BXAIR, O.BYAIR):
+            err = decks.rel_l2(c.get(comp), m.get(comp))
+            assert err < FP64_TOL, (comp, err)
+
+
+def _deck_run(precision, txs_kw, nsteps=0, setup_kw=None, comps=(1, 1, 1, 1, 1, 1), ntx=None):
+    so, st, m, cond = _pair(setup_kw)
+    g = m.grid()
+    rt_o = O.make_rxtime(0.02, 0.25, 16, 1, comps)
+    rt_t = T.make_rxtime(0.02, 0.25, 16, 1, comps)
+    rx = np.array([[150.0, 0.0, 400.0], [300.0, 60.0, 400.0], [-200.0, -50.0, 350.0],
+                   [400.0, 0.0, 300.0]])
+    ref = []
+    for kw in txs_kw:
+        efs, steps = m.run(rt_o, rx, O.make_tx(**kw), max_steps=nsteps)
+        ref.append(efs)
+    with T.Context(st, cond, precision) as c:
+        c.set_receivers(rt_t, rx)
+        c.set_transmitters([T.make_tx(**kw) for kw in txs_kw])
+        got = c.run(0, len(txs_kw) if ntx is None else ntx, max_steps=nsteps)
+    return ref, got
+
+
+TX_X = dict(ort=1, bipolen=100.0, x=0.0, y=0.0, z=300.0, rx_ids=[1, 2, 3, 4])
+
+
+def test_traces_fp64():
+    ref, got = _deck_run(T.FP64, [TX_X])
+    err = decks.trace_error(got[0], ref[0])
+    assert np.all(err < 1e-9), err
+
+
+def test_traces_fp32():
+    ref, got = _deck_run(T.FP32, [TX_X])
+    err = decks.trace_error(got[0], ref[0])
+    assert np.all(err < FP32_TRACE_TOL), err
+
+
+@pytest.mark.parametrize("tx", [
+    dict(ort=2, bipolen=100.0, x=0.0, y=0.0, z=300.0, rx_ids=[1, 2, 3, 4]),
+    dict(ort=3, bipolen=100.0, x=0.0, y=0.0, z=300.0, rx_ids=[4, 3, 2, 1]),
+    dict(ort=4, bipolen=150.0, x=25.0, y=-10.0, z=280.0, azi=30.0, dip=20.0, rx_ids=[1, 2, 3, 4]),
+    dict(ort=5, bipolen=150.0, x=25.0, y=-10.0, z=280.0, azi=30.0, dip=20.0, bipolen1=100.0,
+         x1=-40.0, y1=20.0, z1=320.0, azi1=-45.0, dip1=10.0, rx_ids=[2, 3]),
+])
+def test_traces_orientations_fp64(tx):
+    ref, got = _deck_run(T.FP64, [tx], nsteps=150)
+    err = decks.trace_error(got[0], ref[0])
+    assert np.all(err < 1e-9), err
+
+
+def test_batch_equals_independent_runs_fp64():
+    txs = [dict(TX_X, x=x, txid=i + 1) for i, x in enumerate((-100.0, 0.0, 120.0))]
+    ref, got = _deck_run(T.FP64, txs, nsteps=200)
+    for b in range(3):
+        err = decks.trace_error(got[b], ref[b])
+        assert np.all(err < 1e-9), (b, err)
