@@ -14,6 +14,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -392,7 +393,7 @@ struct tcsem_ctx {
   std::vector<double> cond;
   int PX = 0, PY = 0;
   long long PS = 0, fstride = 0, pstride = 0, wstride = 0;
-  int ntx = 0, nty = 0, nper = 0;
+  int by = 8, ntx = 0, nty = 0, nper = 0;
   int nslots = 0;
   hipStream_t stream = nullptr;
   // device buffers
@@ -446,6 +447,37 @@ struct tcsem_ctx {
 
 namespace {
 
+// Workgroup height of the volume march: the tallest tile (least halo work) that still gives every
+// CU of the device at least kMinTilesPerCu workgroups over the batch.  TCSEM_MARCH_ROWS overrides
+// the choice (4, 6, 8, 12 or 16).
+void choose_tiling(tcsem_ctx *c, int nbatch) {
+  const int L = c->g.info.L, M = c->g.info.M;
+  static const int cands[] = {16, 12, 8, 6, 4};
+  int cu = 256;
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
+    cu = prop.multiProcessorCount;
+  constexpr double kMinTilesPerCu = 1.5;
+  int by = 4;
+  for (int b : cands) {
+    const long long tiles = (long long)((L + 1 + kMarchOwnX - 1) / kMarchOwnX) * ((M + 1 + (b - 2) - 1) / (b - 2));
+    if (tiles * nbatch >= kMinTilesPerCu * cu) {
+      by = b;
+      break;
+    }
+  }
+  if (const char *e = std::getenv("TCSEM_MARCH_ROWS")) {
+    const int v = std::atoi(e);
+    for (int b : cands)
+      if (b == v) by = v;
+  }
+  c->by = by;
+  c->ntx = (L + 1 + kMarchOwnX - 1) / kMarchOwnX;
+  c->nty = (M + 1 + (by - 2) - 1) / (by - 2);
+  c->nper = (c->ntx * c->nty + 7) / 8;
+}
+
 template <typename T>
 MarchArgs<T> march_args(tcsem_ctx *c) {
   MarchArgs<T> a{};
@@ -476,6 +508,7 @@ MarchArgs<T> march_args(tcsem_ctx *c) {
   a.M = c->g.info.M;
   a.N = c->g.info.N;
   a.PX = c->PX;
+  a.rows = c->by;
   a.ntx = c->ntx;
   a.nty = c->nty;
   a.nper = c->nper;
@@ -1246,9 +1279,7 @@ int tcsem_create(const tcsem_setup *s, const double *cond, int precision, tcsem_
   c->fstride = (long long)(N + 1) * c->PS;
   c->pstride = c->PS;
   c->wstride = (long long)I.L1 * I.M1;
-  c->ntx = (L + 1 + kMarchOwnX - 1) / kMarchOwnX;
-  c->nty = (M + 1 + kMarchOwnY - 1) / kMarchOwnY;
-  c->nper = (c->ntx * c->nty + 7) / 8;
+  choose_tiling(c, 1);
   // air-plane operators
   c->n1 = c->g.AL + kNPad;
   c->n2 = c->g.AR + kNPad;
@@ -1502,6 +1533,7 @@ int tcsem_begin(tcsem_ctx *c, int tx_first, int nbatch) {
   free_run(c);
   c->tx_first = tx_first;
   c->nbatch = nbatch;
+  choose_tiling(c, nbatch);
   if ((rc = tcsem_reset_fields(c, 0, nbatch))) return rc;
   // sources (per slot)
   std::vector<SrcCell> cells;

@@ -17,12 +17,10 @@ constexpr double kPi = 3.141592653589793238;  // FTDT.h:39
 constexpr double kMu0 = 4 * kPi * 1.e-7;      // main.cu:55
 constexpr int kNPad = 4;                      // FTDT.h:40
 
-// Volume march tile: 64 lanes in x (62 owned columns + one halo column each side) by
-// kMarchRows rows in y (kMarchRows - 2 owned rows + one halo row each side).
+// Volume march tile: 64 lanes in x (62 owned columns + one halo column each side) by BY rows in
+// y (BY - 2 owned rows + one halo row each side); BY is chosen per grid and batch size.
 constexpr int kMarchLanes = 64;
 constexpr int kMarchOwnX = kMarchLanes - 2;
-constexpr int kMarchRows = 8;
-constexpr int kMarchOwnY = kMarchRows - 2;
 
 // One injected E cell: E[comp](k, j, i) -= amp * w each source step (ElectricalFiled.cu:347-475).
 struct SrcCell {
@@ -44,6 +42,7 @@ struct MarchArgs {
   long long pstride;          // elements between slots of one plane
   long long PS;               // plane stride of a volume
   int L, M, N, PX;
+  int rows;                   // workgroup rows (tile height + 2): 4, 6, 8, 12 or 16
   int ntx, nty, nper;         // tiles in x, y; tiles per XCD group
   const T *rdx, *rdx2, *rdy, *rdy2;  // 1/dx[i], 1/(dx[i-1]+dx[i]), same for y (index+1 origin)
   const T *dzt, *rdz, *rdz2;         // dz[k], 1/dz[k], 1/(dz[k-1]+dz[k]) (rdz2[0] = 1/(2 dz[0]))

@@ -44,19 +44,43 @@ struct Acc<float> {
 };
 
 // DuFort-Frankel update E <- (2l - s dt)/(2l + s dt) E + 4 dt/(2l + s dt) curl / mu0
-// (ElectricalFiled.cu:138-139).  FP64 fields: the reference expression.  FP32 fields: the two
-// coefficients in FP64, the product in FP32.
-__device__ inline double df_new(double e, double sig, double lam, double dt, double c4, double curl) {
-  (void)c4;
-  const double den = 2.0 * lam + sig * dt;
-  return (2.0 * lam - sig * dt) / den * e + 4.0 * dt / den * curl / kMu0;
+// (ElectricalFiled.cu:138-139), written with x = s dt / (2l): E <- (1-x)/(1+x) E + c4/(1+x) curl,
+// c4 = 4 dt / (2l mu0).  FP64 fields: one FP64 division per update.  FP32 fields: the reciprocal
+// of 1+x from v_rcp_f32 refined by one Newton step (error below one ulp, unbiased in sign).
+struct Coef64 {
+  double q2, c4;  // dt/(2l), 4 dt/(2l mu0)
+};
+__device__ inline double df_new(double e, double sig, const Coef64 &k, double curl) {
+  const double x = sig * k.q2;
+  const double r = 1.0 / (1.0 + x);
+  return (1.0 - x) * r * e + k.c4 * r * curl;
 }
-__device__ inline float df_new(float e, float sig, double lam, double dt, double c4, float curl) {
-  const double sd = (double)sig * dt;
-  const double r = 1.0 / (2.0 * lam + sd);
-  const float ca = (float)((2.0 * lam - sd) * r);
-  const float cb = (float)(c4 * r);
-  return ca * e + cb * curl;
+struct Coef32 {
+  float q2, c4;
+};
+__device__ inline float df_new(float e, float sig, const Coef32 &k, float curl) {
+  const float x = sig * k.q2;
+  const float d = 1.0f + x;
+  float r = __builtin_amdgcn_rcpf(d);
+  r = r * (2.0f - d * r);
+  return (1.0f - x) * r * e + k.c4 * r * curl;
+}
+template <typename T>
+struct CoefOf;
+template <>
+struct CoefOf<double> {
+  using type = Coef64;
+};
+template <>
+struct CoefOf<float> {
+  using type = Coef32;
+};
+template <typename T>
+__device__ inline typename CoefOf<T>::type make_coef(double lam, double dt) {
+  typename CoefOf<T>::type k;
+  k.q2 = (T)(dt / (2.0 * lam));
+  k.c4 = (T)(4.0 * dt / (2.0 * lam * kMu0));
+  return k;
 }
 
 template <typename T>
@@ -73,7 +97,7 @@ __device__ inline T apply_src(T e, int comp, int k, int j, int i, const MarchArg
 
 struct Tile {
   int lx, ly, i, j;
-  bool inside, own, need_bz, need_ez;
+  bool own, need_bz, need_ez;
 };
 
 template <int BY>
@@ -98,16 +122,64 @@ __device__ inline bool tile_of(int ntx, int nty, int nper, Tile &t) {
   return true;
 }
 
+// loads/stores at a byte offset from a uniform base (global_load ... vOff, s[base])
+template <typename T>
+__device__ __forceinline__ T ldo(const T *__restrict__ base, unsigned off) {
+  return *(const T *)((const char *)base + off);
+}
+template <typename T>
+__device__ __forceinline__ void sto(T *__restrict__ base, unsigned off, T v) {
+  *(T *)((char *)base + off) = v;
+}
+
+// Per-lane geometry shared by the march and the surface kernel.  Every lane computes every
+// quantity (no divergent loads); the validity masks select what is kept and stored.
+template <typename T>
+struct Lane {
+  int i, j;
+  unsigned colb;  // byte offset of the column within a level (allocation base, margin included)
+  bool vEx, vEy, vEzAny, vEz, vBx, vBy, own;
+  T rdx, rdx2, rdy, rdy2;
+  int lxm, lxp, lym, lyp;  // clamped LDS neighbour lanes
+};
+
+template <typename T, int BY>
+__device__ inline void lane_setup(const MarchArgs<T> &a, const Tile &t, Lane<T> &ln) {
+  const int L = a.L, M = a.M;
+  const int i = t.i, j = t.j;
+  ln.i = i;
+  ln.j = j;
+  const int ic = min(i, L + 1), jc = min(j, M + 1);
+  ln.colb = (unsigned)((jc + 1) * a.PX + (ic + 1)) * (unsigned)sizeof(T);
+  ln.own = t.own && i <= L && j <= M;
+  ln.vEx = ln.own && i < L && j >= 1 && j < M;
+  ln.vEy = ln.own && i >= 1 && i < L && j < M;
+  ln.vEzAny = t.need_ez && i >= 1 && i < L && j >= 1 && j < M;
+  ln.vEz = ln.own && ln.vEzAny;
+  ln.vBx = ln.own && i >= 1 && i < L && j < M;
+  ln.vBy = ln.own && i < L && j >= 1 && j < M;
+  ln.rdx = a.rdx[ic + 1];
+  ln.rdx2 = a.rdx2[ic + 1];
+  ln.rdy = a.rdy[jc + 1];
+  ln.rdy2 = a.rdy2[jc + 1];
+  ln.lxm = max(t.lx - 1, 0);
+  ln.lxp = min(t.lx + 1, kMarchLanes - 1);
+  ln.lym = max(t.ly - 1, 0);
+  ln.lyp = min(t.ly + 1, BY - 1);
+}
+
 template <typename T, int BY>
 __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> a) {
   __shared__ T sBx[2][BY][kMarchLanes], sBy[2][BY][kMarchLanes], sS[2][BY][kMarchLanes];
   __shared__ T sBz[BY][kMarchLanes], sEz[BY][kMarchLanes];
   Tile t;
   if (!tile_of<BY>(a.ntx, a.nty, a.nper, t)) return;
-  const int lx = t.lx, ly = t.ly, i = t.i, j = t.j;
-  const int L = a.L, M = a.M, N = a.N;
+  Lane<T> ln;
+  lane_setup<T, BY>(a, t, ln);
+  const int lx = t.lx, ly = t.ly, i = ln.i, j = ln.j;
+  const int N = a.N;
   const int slot = blockIdx.y;
-  const long long fo = (long long)slot * a.fstride, PS = a.PS;
+  const long long fo = (long long)slot * a.fstride - (a.PX + 1);  // allocation base of the slot
   const T *__restrict__ exi = a.exi + fo;
   const T *__restrict__ eyi = a.eyi + fo;
   const T *__restrict__ ezi = a.ezi + fo;
@@ -118,40 +190,20 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> a)
   T *__restrict__ ez = a.ez + fo;
   T *__restrict__ bx = a.bx + fo;
   T *__restrict__ by = a.by + fo;
-  const T *__restrict__ sig = a.sig;
-  const bool inside = i >= -1 && i <= L + 1 && j >= -1 && j <= M + 1;
-  const bool own = t.own && i <= L && j <= M;
-  const bool vEx = own && i < L && j >= 1 && j < M;
-  const bool vEy = own && i >= 1 && i < L && j < M;
-  const bool vEzAny = t.need_ez && i >= 1 && i < L && j >= 1 && j < M;
-  const bool vEz = own && vEzAny;
-  const bool vBx = own && i >= 1 && i < L && j < M;
-  const bool vBy = own && i < L && j >= 1 && j < M;
-  const long long col = (long long)j * a.PX + i;
-  T rdx = 0, rdx2 = 0, rdy = 0, rdy2 = 0;
-  if (inside) {
-    rdx = a.rdx[i + 1];
-    rdx2 = a.rdx2[i + 1];
-    rdy = a.rdy[j + 1];
-    rdy2 = a.rdy2[j + 1];
-  }
+  const T *__restrict__ sig = a.sig - (a.PX + 1);
+  const unsigned PSb = (unsigned)a.PS * (unsigned)sizeof(T);
   const int sfirst = a.src_first ? a.src_first[slot] : 0, scount = a.src_count ? a.src_count[slot] : 0;
-  const double lam = a.lam, dt = a.dt, c4 = 4.0 * a.dt / kMu0;
-  const T dtb = (T)a.dtb;
+  const auto cf = make_coef<T>(a.lam, a.dt);
+  const T dtb = (T)a.dtb, zero = 0;
+  const T rdx = ln.rdx, rdx2 = ln.rdx2, rdy = ln.rdy, rdy2 = ln.rdy2;
+  const int lxm = ln.lxm, lxp = ln.lxp, lym = ln.lym, lyp = ln.lyp;
 
-  // level N-1 state
-  T bx_k = 0, by_k = 0, s_k = 0, exo = 0, eyo = 0, ezo = 0;
-  {
-    const long long o = (long long)(N - 1) * PS + col;
-    if (inside) {
-      bx_k = bxi[o];
-      by_k = byi[o];
-      s_k = sig[o];
-    }
-    if (vEx) exo = exi[o];
-    if (vEy) eyo = eyi[o];
-    if (vEzAny) ezo = ezi[o];
-  }
+  // levels N-1 (current) and N-2 (next) in registers
+  unsigned off = (unsigned)(N - 1) * PSb + ln.colb;
+  T bx_k = ldo(bxi, off), by_k = ldo(byi, off), s_k = ldo(sig, off);
+  T exo = ldo(exi, off), eyo = ldo(eyi, off), ezo = ldo(ezi, off);
+  T bx_m = ldo(bxi, off - PSb), by_m = ldo(byi, off - PSb), s_m = ldo(sig, off - PSb);
+  T exo_n = ldo(exi, off - PSb), eyo_n = ldo(eyi, off - PSb), ezo_n = ldo(ezi, off - PSb);
   int p = (N - 1) & 1;
   sBx[p][ly][lx] = bx_k;
   sBy[p][ly][lx] = by_k;
@@ -166,59 +218,51 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> a)
     const int q = p ^ 1;
     const T dzk = a.dzt[k], rdzk = a.rdz[k], rdz2k = a.rdz2[k];
     // (a) Bz_old(k) = Bz_old(k+1) + dz(k) div_h(B_old(k))
-    if (t.need_bz) bz.add(dzk * ((sBx[p][ly][lx + 1] - bx_k) * rdx + (sBy[p][ly + 1][lx] - by_k) * rdy));
+    bz.add(dzk * ((sBx[p][ly][lxp] - bx_k) * rdx + (sBy[p][lyp][lx] - by_k) * rdy));
     const T bzk = bz.val();
     sBz[ly][lx] = bzk;
-    // (b) next level up
-    T bx_m = 0, by_m = 0, s_m = 0, exo_n = 0, eyo_n = 0, ezo_n = 0;
-    {
-      const long long o = (long long)(k - 1) * PS + col;
-      if (inside) {
-        bx_m = bxi[o];
-        by_m = byi[o];
-        s_m = sig[o];
-      }
-      if (vEx) exo_n = exi[o];
-      if (vEy) eyo_n = eyi[o];
-      if (vEzAny) ezo_n = ezi[o];
-    }
+    // (b) level k-1 into LDS; level k-2 into registers (level k-1 again at k = 1)
     sBx[q][ly][lx] = bx_m;
     sBy[q][ly][lx] = by_m;
     sS[q][ly][lx] = s_m;
+    const unsigned o2 = k >= 2 ? off - 2 * PSb : off - PSb;
+    const T bx_n = ldo(bxi, o2), by_n = ldo(byi, o2), s_n = ldo(sig, o2);
+    const T exo_nn = ldo(exi, o2), eyo_nn = ldo(eyi, o2), ezo_nn = ldo(ezi, o2);
     __syncthreads();
-    const long long ok = (long long)k * PS + col;
-    // (c) Ez(k)
-    T ezn = 0;
-    if (vEzAny) {
-      const T s4 = (T)0.25 * (sS[p][ly - 1][lx - 1] + sS[p][ly - 1][lx] + sS[p][ly][lx - 1] + sS[p][ly][lx]);
-      const T curl = (by_k - sBy[p][ly][lx - 1]) * rdx2 - (bx_k - sBx[p][ly - 1][lx]) * rdy2;
-      ezn = df_new(ezo, s4, lam, dt, c4, curl);
-      ezn = apply_src(ezn, 2, k, j, i, a, sfirst, scount);
-      if (vEz) ez[ok] = ezn;
+    // (c) Ez(k) on the tile and its +x/+y halo
+    T ezn;
+    {
+      const T s4 = (T)0.25 * (sS[p][lym][lxm] + sS[p][lym][lx] + sS[p][ly][lxm] + sS[p][ly][lx]);
+      const T curl = (by_k - sBy[p][ly][lxm]) * rdx2 - (bx_k - sBx[p][lym][lx]) * rdy2;
+      ezn = apply_src(df_new(ezo, s4, cf, curl), 2, k, j, i, a, sfirst, scount);
+      ezn = ln.vEzAny ? ezn : zero;
     }
     sEz[ly][lx] = ezn;
+    if (ln.vEz) sto(ez, off, ezn);
     // (d) Ex(k), Ey(k)
-    T exn = 0, eyn = 0;
-    if (vEx) {
-      const T sxk = (T)0.5 * (sS[p][ly - 1][lx] + sS[p][ly][lx]);
-      const T sxm = (T)0.5 * (sS[q][ly - 1][lx] + sS[q][ly][lx]);
+    T exn, eyn;
+    {
+      const T sxk = (T)0.5 * (sS[p][lym][lx] + sS[p][ly][lx]);
+      const T sxm = (T)0.5 * (sS[q][lym][lx] + sS[q][ly][lx]);
       const T ave = (T)0.5 * (sxk + sxm);
-      const T curl = (bzk - sBz[ly - 1][lx]) * rdy2 - (by_k - by_m) * rdz2k;
-      exn = df_new(exo, ave, lam, dt, c4, curl);
-      exn = apply_src(exn, 0, k, j, i, a, sfirst, scount);
-      ex[ok] = exn;
+      const T curl = (bzk - sBz[lym][lx]) * rdy2 - (by_k - by_m) * rdz2k;
+      exn = apply_src(df_new(exo, ave, cf, curl), 0, k, j, i, a, sfirst, scount);
+      exn = ln.vEx ? exn : zero;
     }
-    if (vEy) {
-      const T s4 = (T)0.25 * (sS[p][ly][lx - 1] + sS[p][ly][lx] + sS[q][ly][lx - 1] + sS[q][ly][lx]);
-      const T curl = (bx_k - bx_m) * rdz2k - (bzk - sBz[ly][lx - 1]) * rdx2;
-      eyn = df_new(eyo, s4, lam, dt, c4, curl);
-      eyn = apply_src(eyn, 1, k, j, i, a, sfirst, scount);
-      ey[ok] = eyn;
+    {
+      const T s4 = (T)0.25 * (sS[p][ly][lxm] + sS[p][ly][lx] + sS[q][ly][lxm] + sS[q][ly][lx]);
+      const T curl = (bx_k - bx_m) * rdz2k - (bzk - sBz[ly][lxm]) * rdx2;
+      eyn = apply_src(df_new(eyo, s4, cf, curl), 1, k, j, i, a, sfirst, scount);
+      eyn = ln.vEy ? eyn : zero;
     }
+    if (ln.vEx) sto(ex, off, exn);
+    if (ln.vEy) sto(ey, off, eyn);
     __syncthreads();
     // (e) Bx(k), By(k) from E_new(k), E_new(k+1)
-    if (vBx) bx[ok] = bx_k + dtb * ((ey_n1 - eyn) * rdzk - (sEz[ly + 1][lx] - ezn) * rdy);
-    if (vBy) by[ok] = by_k + dtb * ((sEz[ly][lx + 1] - ezn) * rdx - (ex_n1 - exn) * rdzk);
+    const T bxn = bx_k + dtb * ((ey_n1 - eyn) * rdzk - (sEz[lyp][lx] - ezn) * rdy);
+    const T byn = by_k + dtb * ((sEz[ly][lxp] - ezn) * rdx - (ex_n1 - exn) * rdzk);
+    if (ln.vBx) sto(bx, off, bxn);
+    if (ln.vBy) sto(by, off, byn);
     ex_n1 = exn;
     ey_n1 = eyn;
     bx_k = bx_m;
@@ -227,10 +271,18 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> a)
     exo = exo_n;
     eyo = eyo_n;
     ezo = ezo_n;
+    bx_m = bx_n;
+    by_m = by_n;
+    s_m = s_n;
+    exo_n = exo_nn;
+    eyo_n = eyo_nn;
+    ezo_n = ezo_nn;
+    off -= PSb;
   }
-  // Bz_old(0) for the upward continuation and the surface level
-  if (t.need_bz) bz.add(a.dzt[0] * ((sBx[0][ly][lx + 1] - bx_k) * rdx + (sBy[0][ly + 1][lx] - by_k) * rdy));
-  if (own && i >= 0 && i < L && j >= 0 && j < M) a.bz0[(long long)slot * a.pstride + col] = bz.val();
+  // Bz_old(0) for the upward continuation and the surface level (level 0 sits in buffer 0)
+  bz.add(a.dzt[0] * ((sBx[0][ly][lxp] - bx_k) * rdx + (sBy[0][lyp][lx] - by_k) * rdy));
+  if (ln.own && i >= 0 && i < a.L && j >= 0 && j < a.M)
+    sto(a.bz0 + (long long)slot * a.pstride - (a.PX + 1), ln.colb, bz.val());
 }
 
 // Level 0: the air cells count as sigma = 0 and the level above is BxAir/ByAir at -dz0/2
@@ -241,89 +293,65 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_surface(const MarchArgs<T> 
   __shared__ T sBz[BY][kMarchLanes], sEz[BY][kMarchLanes];
   Tile t;
   if (!tile_of<BY>(a.ntx, a.nty, a.nper, t)) return;
-  const int lx = t.lx, ly = t.ly, i = t.i, j = t.j;
+  Lane<T> ln;
+  lane_setup<T, BY>(a, t, ln);
+  const int lx = t.lx, ly = t.ly, i = ln.i, j = ln.j;
   const int L = a.L, M = a.M;
   const int slot = blockIdx.y;
-  const long long fo = (long long)slot * a.fstride, po = (long long)slot * a.pstride, PS = a.PS;
-  const T *__restrict__ exi = a.exi + fo;
-  const T *__restrict__ eyi = a.eyi + fo;
-  const T *__restrict__ ezi = a.ezi + fo;
-  const T *__restrict__ bxi = a.bxi + fo;
-  const T *__restrict__ byi = a.byi + fo;
+  const long long fo = (long long)slot * a.fstride - (a.PX + 1);
+  const long long po = (long long)slot * a.pstride - (a.PX + 1);
+  const unsigned off = ln.colb, PSb = (unsigned)a.PS * (unsigned)sizeof(T);
   T *__restrict__ ex = a.ex + fo;
   T *__restrict__ ey = a.ey + fo;
   T *__restrict__ ez = a.ez + fo;
   T *__restrict__ bx = a.bx + fo;
   T *__restrict__ by = a.by + fo;
-  const bool inside = i >= -1 && i <= L + 1 && j >= -1 && j <= M + 1;
-  const bool own = t.own && i <= L && j <= M;
-  const bool vEx = own && i < L && j >= 1 && j < M;
-  const bool vEy = own && i >= 1 && i < L && j < M;
-  const bool vEzAny = t.need_ez && i >= 1 && i < L && j >= 1 && j < M;
-  const bool vEz = own && vEzAny;
-  const bool vBx = own && i >= 1 && i < L && j < M;
-  const bool vBy = own && i < L && j >= 1 && j < M;
-  const long long col = (long long)j * a.PX + i;
-  T rdx = 0, rdx2 = 0, rdy = 0, rdy2 = 0, bx0 = 0, by0 = 0, s0 = 0, bzc = 0;
-  if (inside) {
-    rdx = a.rdx[i + 1];
-    rdx2 = a.rdx2[i + 1];
-    rdy = a.rdy[j + 1];
-    rdy2 = a.rdy2[j + 1];
-    bx0 = bxi[col];
-    by0 = byi[col];
-    s0 = a.sig[col];
-  }
-  if (t.need_bz && i >= 0 && i < L && j >= 0 && j < M) bzc = a.bz0[po + col];
-  T exo = 0, eyo = 0, ezo = 0, ex1 = 0, ey1 = 0, bxa = 0, bya = 0;
-  if (vEx) {
-    exo = exi[col];
-    ex1 = ex[PS + col];  // E_new(1), written by the march
-    bya = a.byair[po + col];
-  }
-  if (vEy) {
-    eyo = eyi[col];
-    ey1 = ey[PS + col];
-    bxa = a.bxair[po + col];
-  }
-  if (vEzAny) ezo = ezi[col];
+  const T bx0 = ldo(a.bxi + fo, off), by0 = ldo(a.byi + fo, off), s0 = ldo(a.sig - (a.PX + 1), off);
+  const bool bzv = t.need_bz && i >= 0 && i < L && j >= 0 && j < M;
+  const T bzc = bzv ? ldo(a.bz0 + po, off) : (T)0;
+  const T exo = ldo(a.exi + fo, off), eyo = ldo(a.eyi + fo, off), ezo = ldo(a.ezi + fo, off);
+  const T ex1 = ldo((const T *)ex, off + PSb), ey1 = ldo((const T *)ey, off + PSb);  // E_new(1)
+  const T bya = ldo(a.byair + po, off), bxa = ldo(a.bxair + po, off);
   sBx[ly][lx] = bx0;
   sBy[ly][lx] = by0;
   sS[ly][lx] = s0;
   sBz[ly][lx] = bzc;
   const int sfirst = a.src_first ? a.src_first[slot] : 0, scount = a.src_count ? a.src_count[slot] : 0;
-  const double lam = a.lam, dt = a.dt, c4 = 4.0 * a.dt / kMu0;
-  const T dtb = (T)a.dtb, rdz0 = a.rdz[0], rdz20 = a.rdz2[0];
-  const T zero = 0;
+  const auto cf = make_coef<T>(a.lam, a.dt);
+  const T dtb = (T)a.dtb, rdz0 = a.rdz[0], rdz20 = a.rdz2[0], zero = 0;
+  const T rdx = ln.rdx, rdx2 = ln.rdx2, rdy = ln.rdy, rdy2 = ln.rdy2;
+  const int lxm = ln.lxm, lxp = ln.lxp, lym = ln.lym, lyp = ln.lyp;
   __syncthreads();
-  T ezn = 0;
-  if (vEzAny) {
-    const T s4 = (T)0.25 * (sS[ly - 1][lx - 1] + sS[ly - 1][lx] + sS[ly][lx - 1] + sS[ly][lx]);
-    const T curl = (by0 - sBy[ly][lx - 1]) * rdx2 - (bx0 - sBx[ly - 1][lx]) * rdy2;
-    ezn = df_new(ezo, s4, lam, dt, c4, curl);
-    ezn = apply_src(ezn, 2, 0, j, i, a, sfirst, scount);
-    if (vEz) ez[col] = ezn;
+  T ezn;
+  {
+    const T s4 = (T)0.25 * (sS[lym][lxm] + sS[lym][lx] + sS[ly][lxm] + sS[ly][lx]);
+    const T curl = (by0 - sBy[ly][lxm]) * rdx2 - (bx0 - sBx[lym][lx]) * rdy2;
+    ezn = apply_src(df_new(ezo, s4, cf, curl), 2, 0, j, i, a, sfirst, scount);
+    ezn = ln.vEzAny ? ezn : zero;
   }
   sEz[ly][lx] = ezn;
-  T exn = 0, eyn = 0;
-  if (vEx) {
-    const T sxk = (T)0.5 * (sS[ly - 1][lx] + sS[ly][lx]);
+  if (ln.vEz) sto(ez, off, ezn);
+  T exn, eyn;
+  {
+    const T sxk = (T)0.5 * (sS[lym][lx] + sS[ly][lx]);
     const T ave = (T)0.5 * (sxk + zero);
-    const T curl = (bzc - sBz[ly - 1][lx]) * rdy2 - (by0 - bya) * rdz20;
-    exn = df_new(exo, ave, lam, dt, c4, curl);
-    exn = apply_src(exn, 0, 0, j, i, a, sfirst, scount);
-    ex[col] = exn;
+    const T curl = (bzc - sBz[lym][lx]) * rdy2 - (by0 - bya) * rdz20;
+    exn = apply_src(df_new(exo, ave, cf, curl), 0, 0, j, i, a, sfirst, scount);
+    exn = ln.vEx ? exn : zero;
   }
-  if (vEy) {
-    const T s4 = (T)0.25 * (sS[ly][lx - 1] + sS[ly][lx]);
-    const T curl = (bx0 - bxa) * rdz20 - (bzc - sBz[ly][lx - 1]) * rdx2;
-    eyn = df_new(eyo, s4, lam, dt, c4, curl);
-    eyn = apply_src(eyn, 1, 0, j, i, a, sfirst, scount);
-    ey[col] = eyn;
+  {
+    const T s4 = (T)0.25 * (sS[ly][lxm] + sS[ly][lx]);
+    const T curl = (bx0 - bxa) * rdz20 - (bzc - sBz[ly][lxm]) * rdx2;
+    eyn = apply_src(df_new(eyo, s4, cf, curl), 1, 0, j, i, a, sfirst, scount);
+    eyn = ln.vEy ? eyn : zero;
   }
+  if (ln.vEx) sto(ex, off, exn);
+  if (ln.vEy) sto(ey, off, eyn);
   __syncthreads();
-  if (vBx) bx[col] = bx0 + dtb * ((ey1 - eyn) * rdz0 - (sEz[ly + 1][lx] - ezn) * rdy);
-  if (vBy) by[col] = by0 + dtb * ((sEz[ly][lx + 1] - ezn) * rdx - (ex1 - exn) * rdz0);
+  const T bxn = bx0 + dtb * ((ey1 - eyn) * rdz0 - (sEz[lyp][lx] - ezn) * rdy);
+  const T byn = by0 + dtb * ((sEz[ly][lxp] - ezn) * rdx - (ex1 - exn) * rdz0);
+  if (ln.vBx) sto(bx, off, bxn);
+  if (ln.vBy) sto(by, off, byn);
 }
 
 // Derived Bz volume (N+1, M, L dense) of one slot (Bderivate.cu:202-265), for field read-back.
@@ -445,17 +473,31 @@ __global__ void k_fill_zero(uint32_t *p, size_t n) {
 }
 
 // ---- launchers -----------------------------------------------------------------------------------
-template <typename T>
-hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s) {
-  dim3 blk(kMarchLanes, kMarchRows), grd(8 * a.nper, nslot);
-  hipLaunchKernelGGL((k_march<T, kMarchRows>), grd, blk, 0, s, a);
+template <typename T, int BY>
+hipError_t launch_march_by(const MarchArgs<T> &a, int nslot, hipStream_t s, bool surface) {
+  dim3 blk(kMarchLanes, BY), grd(8 * a.nper, nslot);
+  if (surface) hipLaunchKernelGGL((k_surface<T, BY>), grd, blk, 0, s, a);
+  else hipLaunchKernelGGL((k_march<T, BY>), grd, blk, 0, s, a);
   return hipGetLastError();
 }
 template <typename T>
+hipError_t launch_by(const MarchArgs<T> &a, int nslot, hipStream_t s, bool surface) {
+  switch (a.rows) {
+    case 4: return launch_march_by<T, 4>(a, nslot, s, surface);
+    case 6: return launch_march_by<T, 6>(a, nslot, s, surface);
+    case 8: return launch_march_by<T, 8>(a, nslot, s, surface);
+    case 12: return launch_march_by<T, 12>(a, nslot, s, surface);
+    case 16: return launch_march_by<T, 16>(a, nslot, s, surface);
+  }
+  return hipErrorInvalidValue;
+}
+template <typename T>
+hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s) {
+  return launch_by<T>(a, nslot, s, false);
+}
+template <typename T>
 hipError_t launch_surface(const MarchArgs<T> &a, int nslot, hipStream_t s) {
-  dim3 blk(kMarchLanes, kMarchRows), grd(8 * a.nper, nslot);
-  hipLaunchKernelGGL((k_surface<T, kMarchRows>), grd, blk, 0, s, a);
-  return hipGetLastError();
+  return launch_by<T>(a, nslot, s, true);
 }
 template <typename T>
 hipError_t launch_bz_volume(const MarchArgs<T> &a, T *bz, int slot, hipStream_t s) {
