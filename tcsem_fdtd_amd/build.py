@@ -45,6 +45,8 @@ def build(verbose=True):
     tmp = LIB + ".tmp"
     _run([hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
+    _run([hipcc()] + HIP_FLAGS + ["-x", "hip", os.path.join(CSRC, "hbm_copy.hip"), "-o",
+                                  os.path.join(HERE, "hbm_copy")])
     drv = os.path.join(CSRC, "tcsem_driver.cpp")
     if os.path.exists(drv):
         _run([hipcc()] + HIP_FLAGS + [drv, "-o", DRIVER, "-L" + HERE, "-ltcsem",

@@ -447,30 +447,24 @@ struct tcsem_ctx {
 
 namespace {
 
-// Workgroup height of the volume march: the tallest tile (least halo work) that still gives every
-// CU of the device at least kMinTilesPerCu workgroups over the batch.  TCSEM_MARCH_ROWS overrides
-// the choice (4, 6, 8, 12 or 16).
+// Workgroup height of the volume march.  Tall tiles waste fewer lanes on the +-1 halo rows but
+// give fewer workgroups: 16 rows (14 owned) once the batch offers at least two workgroups per CU,
+// else 8 rows (6 owned), which measured best for one 256 x 256 model on MI355X (4-16 rows swept).
+// TCSEM_MARCH_ROWS overrides the choice (4, 6, 8, 12 or 16).
 void choose_tiling(tcsem_ctx *c, int nbatch) {
   const int L = c->g.info.L, M = c->g.info.M;
-  static const int cands[] = {16, 12, 8, 6, 4};
   int cu = 256;
   int dev = 0;
   hipDeviceProp_t prop;
   if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
     cu = prop.multiProcessorCount;
-  constexpr double kMinTilesPerCu = 1.5;
-  int by = 4;
-  for (int b : cands) {
-    const long long tiles = (long long)((L + 1 + kMarchOwnX - 1) / kMarchOwnX) * ((M + 1 + (b - 2) - 1) / (b - 2));
-    if (tiles * nbatch >= kMinTilesPerCu * cu) {
-      by = b;
-      break;
-    }
-  }
+  auto tiles = [&](int b) {
+    return (long long)((L + 1 + kMarchOwnX - 1) / kMarchOwnX) * ((M + 1 + (b - 2) - 1) / (b - 2));
+  };
+  int by = tiles(16) * nbatch >= 2LL * cu ? 16 : 8;
   if (const char *e = std::getenv("TCSEM_MARCH_ROWS")) {
     const int v = std::atoi(e);
-    for (int b : cands)
-      if (b == v) by = v;
+    if (v == 4 || v == 6 || v == 8 || v == 12 || v == 16) by = v;
   }
   c->by = by;
   c->ntx = (L + 1 + kMarchOwnX - 1) / kMarchOwnX;

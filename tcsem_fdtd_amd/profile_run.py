@@ -1,0 +1,45 @@
+"""Fixed-step run of one workload for rocprofv3 (kernel traces, counters).
+
+    python -m tcsem_fdtd_amd.profile_run --workload marine_3d --precision 32 --batch 1 --steps 20
+
+Builds the deck, advances `warmup` untimed steps, then `steps` steps, and prints one JSON line with
+the wall time per step.  Used by the rocprof summaries under profiles/.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+
+
+def main():
+    import tcsem_fdtd_amd as T
+    from tcsem_fdtd_amd import workloads as W
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="marine_3d", choices=["marine_1d", "marine_3d", "towline"])
+    ap.add_argument("--precision", type=int, default=32, choices=[32, 64])
+    ap.add_argument("--batch", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    a = ap.parse_args()
+    deck = W.towline(T, ntx=a.batch) if a.workload == "towline" else getattr(W, a.workload)(T)
+    with T.Context(T.make_setup(**deck.setup), deck.cond, a.precision) as c:
+        r = deck.rxtime
+        c.set_receivers(T.make_rxtime(r["T0"], r["Tn"], r["Nt"], r["is_log"], r["comps"]), deck.rx)
+        c.set_transmitters([T.make_tx(**dict(t, txid=i + 1)) for i, t in enumerate(deck.txs)])
+        c.begin(0, a.batch)
+        c.step(a.warmup)
+        c.sync()
+        t0 = time.perf_counter()
+        c.step(a.steps)
+        c.sync()
+        dt = time.perf_counter() - t0
+        print(json.dumps({"workload": deck.name, "precision": a.precision, "batch": a.batch,
+                          "steps": a.steps, "ms_per_step": 1000 * dt / a.steps,
+                          "cells": c.L * c.M * c.N}), flush=True)
+
+
+if __name__ == "__main__":
+    main()
