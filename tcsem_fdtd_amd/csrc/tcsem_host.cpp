@@ -337,8 +337,8 @@ void make_plan(int n, HostPlan &P) {
     P.m <<= 1;
     P.log2m++;
   }
-  P.tw.resize(P.m);  // m/2 complex
-  for (int k = 0; k < P.m / 2; ++k) {
+  P.tw.resize(2 * P.m);  // m complex
+  for (int k = 0; k < P.m; ++k) {
     const double ang = -2.0 * kPi * k / P.m;
     P.tw[2 * k] = std::cos(ang);
     P.tw[2 * k + 1] = std::sin(ang);
@@ -393,19 +393,22 @@ struct tcsem_ctx {
   std::vector<double> cond;
   int PX = 0, PY = 0;
   long long PS = 0, fstride = 0, pstride = 0, wstride = 0;
-  int by = 8, ntx = 0, nty = 0, nper = 0;
+  int by = 8, pf = 2, ntx = 0, nty = 0, nper = 0;
   int nslots = 0;
-  hipStream_t stream = nullptr;
+  // stream: the volume march and every host-ordered operation; stream2: upward continuation,
+  // surface level and receiver gathers of a step, overlapping the lower march of the next step
+  hipStream_t stream = nullptr, stream2 = nullptr;
+  hipEvent_t xev[2] = {};  // [0] march of the step done (stream), [1] stream2 work of the step done
   // device buffers
   char *d_fields = nullptr;  // 2 states x 5 fields x nslots x fstride elements
   int cur = 0;               // state holding the current fields
   char *d_sig = nullptr;
-  char *d_planes = nullptr;  // bz0, bxair, byair: 3 x nslots x pstride elements
+  char *d_planes = nullptr;  // bz0, bxair, byair, carry sum, carry compensation: 5 x nslots x pstride
   char *d_rd = nullptr;      // rdx, rdx2, rdy, rdy2, dzt, rdz, rdz2 (field type)
   double *d_grid = nullptr;  // X Y Z dx dy dz Xn Yn
   int *d_maps = nullptr;     // xBx2N yo2N xo2N yBy2N
   double *d_up = nullptr;    // wl wr wf wb gx gy, plan tables
-  double *d_work = nullptr;  // per slot: cplx, cplx2 (complex), bxr, byr
+  double *d_work = nullptr;  // all slots of: cplx (complex), tx, bxr, byr
   double *d_scratch = nullptr;
   size_t scratch_bytes = 0;
   UpcontArgs<double> up64{};
@@ -450,7 +453,9 @@ namespace {
 // Workgroup height of the volume march.  Tall tiles waste fewer lanes on the +-1 halo rows but
 // give fewer workgroups: 16 rows (14 owned) once the batch offers at least two workgroups per CU,
 // else 8 rows (6 owned), which measured best for one 256 x 256 model on MI355X (4-16 rows swept).
-// TCSEM_MARCH_ROWS overrides the choice (4, 6, 8, 12 or 16).
+// TCSEM_MARCH_ROWS overrides the choice (4, 8 or 16).  The prefetch depth (levels of loads in
+// flight per lane) follows the same sweep: 1 for 16-row tiles (enough waves already), else 2 for
+// FP32 and 3 for FP64 fields; TCSEM_MARCH_PF (1 .. 4) overrides it.
 void choose_tiling(tcsem_ctx *c, int nbatch) {
   const int L = c->g.info.L, M = c->g.info.M;
   int cu = 256;
@@ -464,8 +469,14 @@ void choose_tiling(tcsem_ctx *c, int nbatch) {
   int by = tiles(16) * nbatch >= 2LL * cu ? 16 : 8;
   if (const char *e = std::getenv("TCSEM_MARCH_ROWS")) {
     const int v = std::atoi(e);
-    if (v == 4 || v == 6 || v == 8 || v == 12 || v == 16) by = v;
+    if (v == 4 || v == 8 || v == 16) by = v;
   }
+  int pf = by == 16 ? 1 : (c->prec == 64 ? 3 : 2);
+  if (const char *e = std::getenv("TCSEM_MARCH_PF")) {
+    const int v = std::atoi(e);
+    if (v >= 1 && v <= 4) pf = v;
+  }
+  c->pf = pf;
   c->by = by;
   c->ntx = (L + 1 + kMarchOwnX - 1) / kMarchOwnX;
   c->nty = (M + 1 + (by - 2) - 1) / (by - 2);
@@ -495,6 +506,11 @@ MarchArgs<T> march_args(tcsem_ctx *c) {
   a.bz0 = pl + org;
   a.bxair = pl + ps + org;
   a.byair = pl + 2 * ps + org;
+  a.cs = pl + 3 * ps + org;
+  a.cc = pl + 4 * ps + org;
+  a.k_hi = c->g.info.N - 1;
+  a.k_lo = 1;
+  a.carry = 0;
   a.fstride = c->fstride;
   a.pstride = c->pstride;
   a.PS = c->PS;
@@ -503,6 +519,7 @@ MarchArgs<T> march_args(tcsem_ctx *c) {
   a.N = c->g.info.N;
   a.PX = c->PX;
   a.rows = c->by;
+  a.pf = c->pf;
   a.ntx = c->ntx;
   a.nty = c->nty;
   a.nper = c->nper;
@@ -580,9 +597,10 @@ UpcontArgs<T> upcont_args(tcsem_ctx *c) {
   double *w = c->d_work;
   const long long ws = c->wstride * c->nslots;
   u.cplx = (double2 *)w;
-  u.cplx2 = (double2 *)(w + 2 * ws);
-  u.bxr = w + 4 * ws;
-  u.byr = w + 5 * ws;
+  u.tx = w + 2 * ws;
+  u.bxr = w + 3 * ws;
+  u.ry = u.bxr;  // consumed by the forward column FFTs before the results are written
+  u.byr = w + 4 * ws;
   const double *gd = c->d_grid;
   u.X = gd;
   u.Y = gd + I.L;
@@ -623,8 +641,8 @@ int alloc_slots(tcsem_ctx *c, int nslots) {
   c->d_work = nullptr;
   c->nslots = 0;
   const size_t fb = (size_t)2 * 5 * nslots * c->fstride * c->esz;
-  const size_t pb = (size_t)3 * nslots * c->pstride * c->esz;
-  const size_t wb = (size_t)6 * nslots * c->wstride * sizeof(double);
+  const size_t pb = (size_t)5 * nslots * c->pstride * c->esz;
+  const size_t wb = (size_t)5 * nslots * c->wstride * sizeof(double);
   if (hipMalloc(&c->d_fields, fb) != hipSuccess) return fail(TCSEM_ENOMEM, "field allocation of %zu bytes failed", fb);
   if (hipMalloc(&c->d_planes, pb) != hipSuccess) return fail(TCSEM_ENOMEM, "plane allocation failed");
   if (hipMalloc((void **)&c->d_work, wb) != hipSuccess) return fail(TCSEM_ENOMEM, "work allocation failed");
@@ -895,15 +913,39 @@ int do_step(tcsem_ctx *c) {
   a.dtb = 0.5 * (c->dt1 + c->dt2);
   a.amp = c->n < 2 * ns ? c->wwave[c->n] : 0.0;
   const UpcontArgs<T> &u = up_of<T>(c);
-  const int nb = c->nbatch;
-  hipStream_t st = c->stream;
+  const int nb = c->nbatch, N = c->g.info.N;
+  hipStream_t st = c->stream, st2 = c->stream2;
+  // stream: lower march (levels N-1..2: reads the old state at levels >= 1 only, so it runs
+  // beside the previous step's stream2 work), then, once that work is done (it writes level 0 of
+  // the old state and still reads the Bz(0) plane), the top march (level 1 and Bz_old(0)).
+  // stream2: upward continuation, surface level and receiver gathers of this step.
+  const bool split = N >= 6;
   if (c->timing) HIPCHK(hipEventRecord(c->ev[0], st));
-  HIPCHK(launch_march<T>(a, nb, st));
+  if (split) {
+    MarchArgs<T> lo = a;
+    lo.k_lo = 2;
+    lo.carry = 2;
+    HIPCHK(launch_march<T>(lo, nb, st));
+  }
   if (c->timing) HIPCHK(hipEventRecord(c->ev[1], st));
-  HIPCHK(launch_upcont<T>(u, nb, st));
+  HIPCHK(hipStreamWaitEvent(st, c->xev[1], 0));
   if (c->timing) HIPCHK(hipEventRecord(c->ev[2], st));
-  HIPCHK(launch_surface<T>(a, nb, st));
+  {
+    MarchArgs<T> hi = a;
+    if (split) {
+      hi.k_hi = 1;
+      hi.carry = 1;
+    }
+    HIPCHK(launch_march<T>(hi, nb, st));
+  }
   if (c->timing) HIPCHK(hipEventRecord(c->ev[3], st));
+  HIPCHK(hipEventRecord(c->xev[0], st));
+  HIPCHK(hipStreamWaitEvent(st2, c->xev[0], 0));
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[4], st2));
+  HIPCHK(launch_upcont<T>(u, nb, st2));
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[5], st2));
+  HIPCHK(launch_surface<T>(a, nb, st2));
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[6], st2));
   c->cur ^= 1;  // the new state is current
   // time advance (main.cu:253-260)
   c->toT += c->dt1;
@@ -959,9 +1001,9 @@ int do_step(tcsem_ctx *c) {
       // upload the new ops (few per event)
       if (ev.op_count > 0) {
         HIPCHK(hipMemcpyAsync(c->d_ops + before, c->op_it.data() + before, ev.op_count * sizeof(int),
-                              hipMemcpyHostToDevice, st));
+                              hipMemcpyHostToDevice, st2));
         HIPCHK(hipMemcpyAsync(c->d_opl1 + before, c->op_l1.data() + before, ev.op_count * sizeof(double),
-                              hipMemcpyHostToDevice, st));
+                              hipMemcpyHostToDevice, st2));
       }
       const int c0 = fld == 0 ? 0 : c->nefcmp, c1 = fld == 0 ? c->nefcmp : c->ncomp;
       if (c1 <= c0) continue;
@@ -972,30 +1014,41 @@ int do_step(tcsem_ctx *c) {
       gA.op_it = c->d_ops + ev.op_first;
       gA.op_l1 = c->d_opl1 + ev.op_first;
       gA.nops = ev.op_count;
-      if (c->timing && ngather == 0) HIPCHK(hipEventRecord(c->ev[4], st));
-      HIPCHK(launch_gather<T>(gA, fld, nb, march_args<T>(c), st));
+      HIPCHK(launch_gather<T>(gA, fld, nb, march_args<T>(c), st2));
       ++ngather;
     }
-    if (c->timing && ngather) HIPCHK(hipEventRecord(c->ev[5], st));
+    if (c->timing && ngather) HIPCHK(hipEventRecord(c->ev[7], st2));
   }
+  HIPCHK(hipEventRecord(c->xev[1], st2));
   if (c->timing) {
+    // timed steps run one at a time, so every interval is a kernel's own time
+    HIPCHK(hipStreamSynchronize(st2));
     HIPCHK(hipStreamSynchronize(st));
-    float ms;
+    float ms, ms2;
     HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-    c->tacc[0] += ms;
+    HIPCHK(hipEventElapsedTime(&ms2, c->ev[2], c->ev[3]));
+    c->tacc[0] += ms + ms2;
     c->tn[0]++;
-    HIPCHK(hipEventElapsedTime(&ms, c->ev[2], c->ev[3]));
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[5], c->ev[6]));
     c->tacc[1] += ms;
     c->tn[1]++;
-    HIPCHK(hipEventElapsedTime(&ms, c->ev[1], c->ev[2]));
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[4], c->ev[5]));
     c->tacc[2] += ms;
     c->tn[2]++;
     if (ngather) {
-      HIPCHK(hipEventElapsedTime(&ms, c->ev[4], c->ev[5]));
+      HIPCHK(hipEventElapsedTime(&ms, c->ev[6], c->ev[7]));
       c->tacc[3] += ms;
       c->tn[3] += ngather;
     }
   }
+  return TCSEM_OK;
+}
+
+// Orders everything issued on stream2 before what is issued next on stream; every entry point
+// that returns after stepping calls it, so host-ordered operations see a whole step.
+int join_streams(tcsem_ctx *c) {
+  HIPCHK(hipEventRecord(c->xev[1], c->stream2));
+  HIPCHK(hipStreamWaitEvent(c->stream, c->xev[1], 0));
   return TCSEM_OK;
 }
 
@@ -1315,13 +1368,23 @@ int tcsem_create(const tcsem_setup *s, const double *cond, int precision, tcsem_
     make_plan(M1, c->ply);
   }
   const size_t lds_max = 160 * 1024 - 1024;
-  if ((size_t)c->ply.m * 16 + (size_t)M * 8 > lds_max || (size_t)c->plx.m * 16 > lds_max) {
+  // the march keeps 3 N z-coefficients in LDS beside at most 64 KiB of tiles
+  if ((size_t)3 * N * c->esz + 64 * 1024 > lds_max) {
     delete c;
-    return fail(TCSEM_ESIZE, "air plane %d x %d needs FFT lengths %d/%d beyond the LDS budget", I.L1, I.M1,
-                c->plx.m, c->ply.m);
+    return fail(TCSEM_ESIZE, "%d levels exceed the march's LDS budget", N);
+  }
+  if ((size_t)c->ply.m * 16 + ((size_t)M + 1) / 2 * 16 > lds_max || (size_t)c->plx.m * 16 > lds_max) {
+    const int mx = c->plx.m, my = c->ply.m;
+    delete c;
+    return fail(TCSEM_ESIZE, "air plane %d x %d needs FFT lengths %d/%d beyond the LDS budget", I.L1, I.M1, mx,
+                my);
   }
   if (hipStreamCreate(&c->stream) != hipSuccess) {
     delete c;
+    return fail(TCSEM_EDEVICE, "stream creation failed");
+  }
+  if (hipStreamCreate(&c->stream2) != hipSuccess) {
+    tcsem_destroy(c);
     return fail(TCSEM_EDEVICE, "stream creation failed");
   }
   for (auto &e : c->ev)
@@ -1329,6 +1392,16 @@ int tcsem_create(const tcsem_setup *s, const double *cond, int precision, tcsem_
       tcsem_destroy(c);
       return fail(TCSEM_EDEVICE, "event creation failed");
     }
+  for (auto &e : c->xev)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+      tcsem_destroy(c);
+      return fail(TCSEM_EDEVICE, "event creation failed");
+    }
+  // both ordering events start recorded, so the first step's waits are satisfied
+  if (hipEventRecord(c->xev[0], c->stream) != hipSuccess || hipEventRecord(c->xev[1], c->stream2) != hipSuccess) {
+    tcsem_destroy(c);
+    return fail(TCSEM_EDEVICE, "event record failed");
+  }
   // device constants
   auto bail = [&](int code) {
     tcsem_destroy(c);
@@ -1370,7 +1443,18 @@ int tcsem_create(const tcsem_setup *s, const double *cond, int precision, tcsem_
     if (hipMemcpy(c->d_maps, mp.data(), mp.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
       return bail(fail(TCSEM_EDEVICE, "map upload"));
     std::vector<double> up;
-    for (auto *v : {&c->wl, &c->wr, &c->wf, &c->wb, &c->gx, &c->gy}) up.insert(up.end(), v->begin(), v->end());
+    // x-window weights go tap-major (n x columns) so that lanes over columns read them coalesced
+    auto tap_major = [](const std::vector<double> &w, int n) {
+      const size_t rows = n ? w.size() / n : 0;
+      std::vector<double> t(w.size());
+      for (size_t r = 0; r < rows; ++r)
+        for (int q = 0; q < n; ++q) t[(size_t)q * rows + r] = w[r * n + q];
+      return t;
+    };
+    const std::vector<double> wlt = tap_major(c->wl, c->n1), wrt = tap_major(c->wr, c->n2);
+    for (const std::vector<double> *v : {&wlt, &wrt, (const std::vector<double> *)&c->wf, (const std::vector<double> *)&c->wb,
+                                         (const std::vector<double> *)&c->gx, (const std::vector<double> *)&c->gy})
+      up.insert(up.end(), v->begin(), v->end());
     for (auto *P : {&c->plx, &c->ply})
       for (auto *v : {&P->tw, &P->chirp, &P->bhat}) up.insert(up.end(), v->begin(), v->end());
     if (hipMalloc((void **)&c->d_up, up.size() * 8) != hipSuccess) return bail(fail(TCSEM_ENOMEM, "operator allocation"));
@@ -1394,6 +1478,7 @@ int tcsem_create(const tcsem_setup *s, const double *cond, int precision, tcsem_
 
 void tcsem_destroy(tcsem_ctx *c) {
   if (!c) return;
+  if (c->stream2) (void)hipStreamSynchronize(c->stream2);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   free_run(c);
   (void)hipFree(c->d_fields);
@@ -1409,6 +1494,9 @@ void tcsem_destroy(tcsem_ctx *c) {
   (void)hipFree(c->d_rxi);
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto &e : c->xev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1510,7 +1598,7 @@ int tcsem_reset_fields(tcsem_ctx *c, int first, int n) {
   const size_t fb = (size_t)c->fstride * c->esz, pb = (size_t)c->pstride * c->esz;
   for (int f = 0; f < 10; ++f)
     HIPCHK(launch_fill_zero(c->d_fields + ((size_t)f * c->nslots + first) * fb, fb * n, c->stream));
-  for (int f = 0; f < 3; ++f)
+  for (int f = 0; f < 5; ++f)
     HIPCHK(launch_fill_zero(c->d_planes + ((size_t)f * c->nslots + first) * pb, pb * n, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return TCSEM_OK;
@@ -1611,7 +1699,7 @@ int tcsem_step(tcsem_ctx *c, int nsteps) {
     const int rc = c->prec == 64 ? do_step<double>(c) : do_step<float>(c);
     if (rc) return rc;
   }
-  return TCSEM_OK;
+  return join_streams(c);
 }
 
 int tcsem_steps_remaining(const tcsem_ctx *c) {
@@ -1638,8 +1726,9 @@ int tcsem_run(tcsem_ctx *c, int tx_first, int ntx, int max_steps, double *traces
   int rc = tcsem_begin(c, tx_first, ntx);
   if (rc) return rc;
   while (!sampling_done(c) && (max_steps <= 0 || c->steps < max_steps)) {
-    if ((rc = tcsem_step(c, 1))) return rc;
+    if ((rc = c->prec == 64 ? do_step<double>(c) : do_step<float>(c))) return rc;
   }
+  if ((rc = join_streams(c))) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   if (steps_done) *steps_done = c->steps;
   if (traces) {

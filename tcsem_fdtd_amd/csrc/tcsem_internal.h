@@ -42,7 +42,8 @@ struct MarchArgs {
   long long pstride;          // elements between slots of one plane
   long long PS;               // plane stride of a volume
   int L, M, N, PX;
-  int rows;                   // workgroup rows (tile height + 2): 4, 6, 8, 12 or 16
+  int rows;                   // workgroup rows (tile height + 2): 4, 8 or 16
+  int pf;                     // march prefetch depth in levels: 1 .. 4
   int ntx, nty, nper;         // tiles in x, y; tiles per XCD group
   const T *rdx, *rdx2, *rdy, *rdy2;  // 1/dx[i], 1/(dx[i-1]+dx[i]), same for y (index+1 origin)
   const T *dzt, *rdz, *rdz2;         // dz[k], 1/dz[k], 1/(dz[k-1]+dz[k]) (rdz2[0] = 1/(2 dz[0]))
@@ -50,6 +51,11 @@ struct MarchArgs {
   const SrcCell *src;                // all cells of all slots
   const int *src_first, *src_count;  // per slot
   int src_kmin, src_kmax;            // level range holding any source cell
+  // Level range of one march launch (k_hi down to k_lo).  A step may run as a lower launch
+  // (N-1 .. 2) that stores the running Bz sums (carry 2) and a top launch (level 1) that resumes
+  // from them (carry 1) and writes Bz_old(0); carry 0 marches N-1 .. 1 in one launch.
+  int k_hi, k_lo, carry;
+  T *cs, *cc;                        // carry planes (sum, FP32 compensation), plane origin, slot 0
 };
 
 // Receiver gather descriptor (EFInterp2Rx.cu:60-470) for one field group (E or B) of one event.
@@ -79,7 +85,7 @@ struct FftPlan {
   int n;            // transform length
   int m;            // power-of-two working length (n, or Bluestein length >= 2n-1)
   int log2m;
-  const double *tw;      // m/2 complex twiddles exp(-2 pi i k / m), interleaved re,im
+  const double *tw;      // m complex twiddles exp(-2 pi i k / m), interleaved re,im
   const double *chirp;   // n complex w_k = exp(-i pi k^2 / n) (Bluestein only)
   const double *bhat;    // m complex FFT of the conjugate chirp, scaled by 1/m (Bluestein only)
 };
@@ -91,10 +97,11 @@ struct UpcontArgs {
   long long pstride;
   int L, M, PX, L1, M1, AnLx, AnRx, Any, AL, AY, Nx, Ny;
   int n1, n2, n3;          // spline window lengths
-  const double *wl, *wr;   // x-window weights: AnLx x n1, AnRx x n2
+  const double *wl, *wr;   // x-window weights, tap-major: n1 x AnLx, n2 x AnRx
   const double *wf, *wb;   // y-window weights: Any x n3 (front, back)
+  double *tx;              // per slot M x L1: Bz(0) resampled along x (irregular rows)
+  double *ry;              // per slot M1 x L1: the y-padding rows of the regular plane
   double2 *cplx;           // per slot M1 x L1 complex work (column spectra / row results)
-  double2 *cplx2;          // per slot M1 x L1 second complex work
   double *bxr, *byr;       // per slot M1 x L1 real results
   long long wstride;       // elements between slots of the work planes
   FftPlan px, py;          // x (length L1) and y (length M1) plans

@@ -29,11 +29,21 @@ struct Acc<double> {
   __device__ void init() { s = 0.0; }
   __device__ void add(double x) { s += x; }
   __device__ double val() const { return s; }
+  __device__ void save(double *ps, double *, unsigned off) const { *(double *)((char *)ps + off) = s; }
+  __device__ void load(const double *ps, const double *, unsigned off) { s = *(const double *)((const char *)ps + off); }
 };
 template <>
 struct Acc<float> {
   float s, c;
   __device__ void init() { s = 0.f; c = 0.f; }
+  __device__ void save(float *ps, float *pc, unsigned off) const {
+    *(float *)((char *)ps + off) = s;
+    *(float *)((char *)pc + off) = c;
+  }
+  __device__ void load(const float *ps, const float *pc, unsigned off) {
+    s = *(const float *)((const char *)ps + off);
+    c = *(const float *)((const char *)pc + off);
+  }
   __device__ void add(float x) {
     const float t = s + x;
     const float bp = t - s;
@@ -168,8 +178,15 @@ __device__ inline void lane_setup(const MarchArgs<T> &a, const Tile &t, Lane<T> 
   ln.lyp = min(t.ly + 1, BY - 1);
 }
 
-template <typename T, int BY>
+// One level of one column: old B, sigma and old E.
+template <typename T>
+struct Lv {
+  T bx, by, s, ex, ey, ez;
+};
+
+template <typename T, int BY, int PF>
 __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> a) {
+  extern __shared__ __align__(16) unsigned char zdyn[];  // 3 N entries of T
   __shared__ T sBx[2][BY][kMarchLanes], sBy[2][BY][kMarchLanes], sS[2][BY][kMarchLanes];
   __shared__ T sBz[BY][kMarchLanes], sEz[BY][kMarchLanes];
   Tile t;
@@ -177,7 +194,6 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> a)
   Lane<T> ln;
   lane_setup<T, BY>(a, t, ln);
   const int lx = t.lx, ly = t.ly, i = ln.i, j = ln.j;
-  const int N = a.N;
   const int slot = blockIdx.y;
   const long long fo = (long long)slot * a.fstride - (a.PX + 1);  // allocation base of the slot
   const T *__restrict__ exi = a.exi + fo;
@@ -198,91 +214,122 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> a)
   const T rdx = ln.rdx, rdx2 = ln.rdx2, rdy = ln.rdy, rdy2 = ln.rdy2;
   const int lxm = ln.lxm, lxp = ln.lxp, lym = ln.lym, lyp = ln.lyp;
 
-  // levels N-1 (current) and N-2 (next) in registers
-  unsigned off = (unsigned)(N - 1) * PSb + ln.colb;
-  T bx_k = ldo(bxi, off), by_k = ldo(byi, off), s_k = ldo(sig, off);
-  T exo = ldo(exi, off), eyo = ldo(eyi, off), ezo = ldo(ezi, off);
-  T bx_m = ldo(bxi, off - PSb), by_m = ldo(byi, off - PSb), s_m = ldo(sig, off - PSb);
-  T exo_n = ldo(exi, off - PSb), eyo_n = ldo(eyi, off - PSb), ezo_n = ldo(ezi, off - PSb);
-  int p = (N - 1) & 1;
-  sBx[p][ly][lx] = bx_k;
-  sBy[p][ly][lx] = by_k;
-  sS[p][ly][lx] = s_k;
+  // A ring of PF+2 register slots holds level k (cur), level k-1 (nxt) and the PF-1 levels below
+  // it; each level issues the loads of level k-1-PF into the slot cur held one level earlier, so
+  // a load has PF levels of work to arrive.  The level loop is unrolled by the ring length so
+  // every slot is a fixed register set (no moves that would wait on loads in flight).  Levels
+  // below k_lo-1 are never read (the top launch of a step owns them).
+  constexpr int S = PF + 2;
+  const int khi = a.k_hi, klo = a.k_lo, lfloor = klo - 1;
+  const long long po = (long long)slot * a.pstride - (a.PX + 1);
+  auto load = [&](int lv) {
+    const unsigned o = (unsigned)max(lv, lfloor) * PSb + ln.colb;
+    Lv<T> v;
+    v.bx = ldo(bxi, o);
+    v.by = ldo(byi, o);
+    v.s = ldo(sig, o);
+    v.ex = ldo(exi, o);
+    v.ey = ldo(eyi, o);
+    v.ez = ldo(ezi, o);
+    return v;
+  };
+  unsigned off = (unsigned)khi * PSb + ln.colb;
+  Lv<T> R[S];
+#pragma unroll
+  for (int d = 0; d < S - 1; ++d) R[d] = load(khi - d);
+  // dz, 1/dz, 1/(dz[k-1]+dz[k]) of every level in LDS: per-level reads then wait on the LDS
+  // counter only, never on the in-order vector-memory counter that the prefetch loads share
+  T *sZ = (T *)zdyn;
+  const int N = a.N;
+  for (int q = ly * kMarchLanes + lx; q < 3 * N; q += kMarchLanes * BY) sZ[q] = a.dzt[q];
+  int p = khi & 1;
+  sBx[p][ly][lx] = R[0].bx;
+  sBy[p][ly][lx] = R[0].by;
+  sS[p][ly][lx] = R[0].s;
   Acc<T> bz;
   bz.init();
   T ex_n1 = 0, ey_n1 = 0;  // E_new at level k+1 (zero at the bottom wall, level N)
+  if (a.carry == 1) {      // resume: Bz sums of level khi+1 and E_new(khi+1) of the lower launch
+    bz.load(a.cs + po, a.cc + po, ln.colb);
+    ex_n1 = ln.vEx ? ldo((const T *)ex, off + PSb) : zero;
+    ey_n1 = ln.vEy ? ldo((const T *)ey, off + PSb) : zero;
+  }
+  T bx_last = 0, by_last = 0;  // old Bx, By of level k_lo-1 after the loop
   __syncthreads();
+  // Drain the set-up loads here: values loaded before the loop (the carried sums, E_new(k_hi+1))
+  // would otherwise keep the loop-header state pending and cost a full wait every level.
+  __builtin_amdgcn_s_waitcnt(0);
 
-  for (int k = N - 1; k >= 1; --k) {
-    p = k & 1;
-    const int q = p ^ 1;
-    const T dzk = a.dzt[k], rdzk = a.rdz[k], rdz2k = a.rdz2[k];
-    // (a) Bz_old(k) = Bz_old(k+1) + dz(k) div_h(B_old(k))
-    bz.add(dzk * ((sBx[p][ly][lxp] - bx_k) * rdx + (sBy[p][lyp][lx] - by_k) * rdy));
-    const T bzk = bz.val();
-    sBz[ly][lx] = bzk;
-    // (b) level k-1 into LDS; level k-2 into registers (level k-1 again at k = 1)
-    sBx[q][ly][lx] = bx_m;
-    sBy[q][ly][lx] = by_m;
-    sS[q][ly][lx] = s_m;
-    const unsigned o2 = k >= 2 ? off - 2 * PSb : off - PSb;
-    const T bx_n = ldo(bxi, o2), by_n = ldo(byi, o2), s_n = ldo(sig, o2);
-    const T exo_nn = ldo(exi, o2), eyo_nn = ldo(eyi, o2), ezo_nn = ldo(ezi, o2);
-    __syncthreads();
-    // (c) Ez(k) on the tile and its +x/+y halo
-    T ezn;
-    {
-      const T s4 = (T)0.25 * (sS[p][lym][lxm] + sS[p][lym][lx] + sS[p][ly][lxm] + sS[p][ly][lx]);
-      const T curl = (by_k - sBy[p][ly][lxm]) * rdx2 - (bx_k - sBx[p][lym][lx]) * rdy2;
-      ezn = apply_src(df_new(ezo, s4, cf, curl), 2, k, j, i, a, sfirst, scount);
-      ezn = ln.vEzAny ? ezn : zero;
+  for (int kb = khi; kb >= klo; kb -= S) {
+#pragma unroll
+    for (int u = 0; u < S; ++u) {
+      const int k = kb - u;
+      if (k < klo) break;
+      const Lv<T> &cur = R[u];
+      const Lv<T> &nxt = R[(u + 1) % S];
+      p = k & 1;
+      const int q = p ^ 1;
+      const T dzk = sZ[k], rdzk = sZ[N + k], rdz2k = sZ[2 * N + k];
+      const T bx_k = cur.bx, by_k = cur.by, bx_m = nxt.bx, by_m = nxt.by;
+      // (a) Bz_old(k) = Bz_old(k+1) + dz(k) div_h(B_old(k))
+      bz.add(dzk * ((sBx[p][ly][lxp] - bx_k) * rdx + (sBy[p][lyp][lx] - by_k) * rdy));
+      const T bzk = bz.val();
+      sBz[ly][lx] = bzk;
+      // (b) level k-1 into LDS; loads of level k-1-PF
+      sBx[q][ly][lx] = bx_m;
+      sBy[q][ly][lx] = by_m;
+      sS[q][ly][lx] = nxt.s;
+      bx_last = bx_m;
+      by_last = by_m;
+      const T ezo = cur.ez, exo = cur.ex, eyo = cur.ey;
+      R[(u + S - 1) % S] = load(k - 1 - PF);
+      __syncthreads();
+      // (c) Ez(k) on the tile and its +x/+y halo
+      T ezn;
+      {
+        const T s4 = (T)0.25 * (sS[p][lym][lxm] + sS[p][lym][lx] + sS[p][ly][lxm] + sS[p][ly][lx]);
+        const T curl = (by_k - sBy[p][ly][lxm]) * rdx2 - (bx_k - sBx[p][lym][lx]) * rdy2;
+        ezn = apply_src(df_new(ezo, s4, cf, curl), 2, k, j, i, a, sfirst, scount);
+        ezn = ln.vEzAny ? ezn : zero;
+      }
+      sEz[ly][lx] = ezn;
+      if (ln.vEz) sto(ez, off, ezn);
+      // (d) Ex(k), Ey(k)
+      T exn, eyn;
+      {
+        const T sxk = (T)0.5 * (sS[p][lym][lx] + sS[p][ly][lx]);
+        const T sxm = (T)0.5 * (sS[q][lym][lx] + sS[q][ly][lx]);
+        const T ave = (T)0.5 * (sxk + sxm);
+        const T curl = (bzk - sBz[lym][lx]) * rdy2 - (by_k - by_m) * rdz2k;
+        exn = apply_src(df_new(exo, ave, cf, curl), 0, k, j, i, a, sfirst, scount);
+        exn = ln.vEx ? exn : zero;
+      }
+      {
+        const T s4 = (T)0.25 * (sS[p][ly][lxm] + sS[p][ly][lx] + sS[q][ly][lxm] + sS[q][ly][lx]);
+        const T curl = (bx_k - bx_m) * rdz2k - (bzk - sBz[ly][lxm]) * rdx2;
+        eyn = apply_src(df_new(eyo, s4, cf, curl), 1, k, j, i, a, sfirst, scount);
+        eyn = ln.vEy ? eyn : zero;
+      }
+      if (ln.vEx) sto(ex, off, exn);
+      if (ln.vEy) sto(ey, off, eyn);
+      __syncthreads();
+      // (e) Bx(k), By(k) from E_new(k), E_new(k+1)
+      const T bxn = bx_k + dtb * ((ey_n1 - eyn) * rdzk - (sEz[lyp][lx] - ezn) * rdy);
+      const T byn = by_k + dtb * ((sEz[ly][lxp] - ezn) * rdx - (ex_n1 - exn) * rdzk);
+      if (ln.vBx) sto(bx, off, bxn);
+      if (ln.vBy) sto(by, off, byn);
+      ex_n1 = exn;
+      ey_n1 = eyn;
+      off -= PSb;
     }
-    sEz[ly][lx] = ezn;
-    if (ln.vEz) sto(ez, off, ezn);
-    // (d) Ex(k), Ey(k)
-    T exn, eyn;
-    {
-      const T sxk = (T)0.5 * (sS[p][lym][lx] + sS[p][ly][lx]);
-      const T sxm = (T)0.5 * (sS[q][lym][lx] + sS[q][ly][lx]);
-      const T ave = (T)0.5 * (sxk + sxm);
-      const T curl = (bzk - sBz[lym][lx]) * rdy2 - (by_k - by_m) * rdz2k;
-      exn = apply_src(df_new(exo, ave, cf, curl), 0, k, j, i, a, sfirst, scount);
-      exn = ln.vEx ? exn : zero;
-    }
-    {
-      const T s4 = (T)0.25 * (sS[p][ly][lxm] + sS[p][ly][lx] + sS[q][ly][lxm] + sS[q][ly][lx]);
-      const T curl = (bx_k - bx_m) * rdz2k - (bzk - sBz[ly][lxm]) * rdx2;
-      eyn = apply_src(df_new(eyo, s4, cf, curl), 1, k, j, i, a, sfirst, scount);
-      eyn = ln.vEy ? eyn : zero;
-    }
-    if (ln.vEx) sto(ex, off, exn);
-    if (ln.vEy) sto(ey, off, eyn);
-    __syncthreads();
-    // (e) Bx(k), By(k) from E_new(k), E_new(k+1)
-    const T bxn = bx_k + dtb * ((ey_n1 - eyn) * rdzk - (sEz[lyp][lx] - ezn) * rdy);
-    const T byn = by_k + dtb * ((sEz[ly][lxp] - ezn) * rdx - (ex_n1 - exn) * rdzk);
-    if (ln.vBx) sto(bx, off, bxn);
-    if (ln.vBy) sto(by, off, byn);
-    ex_n1 = exn;
-    ey_n1 = eyn;
-    bx_k = bx_m;
-    by_k = by_m;
-    s_k = s_m;
-    exo = exo_n;
-    eyo = eyo_n;
-    ezo = ezo_n;
-    bx_m = bx_n;
-    by_m = by_n;
-    s_m = s_n;
-    exo_n = exo_nn;
-    eyo_n = eyo_nn;
-    ezo_n = ezo_nn;
-    off -= PSb;
+  }
+  if (a.carry == 2) {  // lower launch: hand the running sums of level klo to the top launch
+    if (ln.own) bz.save(a.cs + po, a.cc + po, ln.colb);
+    return;
   }
   // Bz_old(0) for the upward continuation and the surface level (level 0 sits in buffer 0)
-  bz.add(a.dzt[0] * ((sBx[0][ly][lxp] - bx_k) * rdx + (sBy[0][lyp][lx] - by_k) * rdy));
-  if (ln.own && i >= 0 && i < a.L && j >= 0 && j < a.M)
-    sto(a.bz0 + (long long)slot * a.pstride - (a.PX + 1), ln.colb, bz.val());
+  bz.add(sZ[0] * ((sBx[0][ly][lxp] - bx_last) * rdx + (sBy[0][lyp][lx] - by_last) * rdy));
+  if (ln.own && i >= 0 && i < a.L && j >= 0 && j < a.M) sto(a.bz0 + po, ln.colb, bz.val());
 }
 
 // Level 0: the air cells count as sigma = 0 and the level above is BxAir/ByAir at -dz0/2
@@ -473,20 +520,44 @@ __global__ void k_fill_zero(uint32_t *p, size_t n) {
 }
 
 // ---- launchers -----------------------------------------------------------------------------------
+// Raises a march kernel's dynamic-LDS limit once per size (static + dynamic may pass 64 KiB).
+template <typename T, int BY, int PF>
+hipError_t march_lds(size_t bytes) {
+  static size_t allowed = 0;  // per instantiation; only grows
+  if (bytes <= allowed) return hipSuccess;
+  const hipError_t e =
+      hipFuncSetAttribute((const void *)k_march<T, BY, PF>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  if (e == hipSuccess) allowed = bytes;
+  return e;
+}
+template <typename T, int BY, int PF>
+hipError_t launch_march_pf(const MarchArgs<T> &a, dim3 grd, dim3 blk, hipStream_t s) {
+  const size_t zl = (size_t)3 * a.N * sizeof(T);
+  const hipError_t e = march_lds<T, BY, PF>(zl);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_march<T, BY, PF>), grd, blk, zl, s, a);
+  return hipGetLastError();
+}
 template <typename T, int BY>
 hipError_t launch_march_by(const MarchArgs<T> &a, int nslot, hipStream_t s, bool surface) {
   dim3 blk(kMarchLanes, BY), grd(8 * a.nper, nslot);
-  if (surface) hipLaunchKernelGGL((k_surface<T, BY>), grd, blk, 0, s, a);
-  else hipLaunchKernelGGL((k_march<T, BY>), grd, blk, 0, s, a);
-  return hipGetLastError();
+  if (surface) {
+    hipLaunchKernelGGL((k_surface<T, BY>), grd, blk, 0, s, a);
+    return hipGetLastError();
+  }
+  switch (a.pf) {
+    case 1: return launch_march_pf<T, BY, 1>(a, grd, blk, s);
+    case 2: return launch_march_pf<T, BY, 2>(a, grd, blk, s);
+    case 3: return launch_march_pf<T, BY, 3>(a, grd, blk, s);
+    case 4: return launch_march_pf<T, BY, 4>(a, grd, blk, s);
+  }
+  return hipErrorInvalidValue;
 }
 template <typename T>
 hipError_t launch_by(const MarchArgs<T> &a, int nslot, hipStream_t s, bool surface) {
   switch (a.rows) {
     case 4: return launch_march_by<T, 4>(a, nslot, s, surface);
-    case 6: return launch_march_by<T, 6>(a, nslot, s, surface);
     case 8: return launch_march_by<T, 8>(a, nslot, s, surface);
-    case 12: return launch_march_by<T, 12>(a, nslot, s, surface);
     case 16: return launch_march_by<T, 16>(a, nslot, s, surface);
   }
   return hipErrorInvalidValue;

@@ -2,18 +2,30 @@
 // BxAir/ByAir at z = -dz0/2 (UpContinuation_gpu.cu:166-274).
 //
 // The chain is linear in Bz(0) and runs in FP64 whatever the field precision:
-//   k_up_cols   one workgroup per regular-grid column: natural-spline IR->R resampling
-//               (SplineGridConvIR2R.cu:539-752) as precomputed window weights -- x pass for the
-//               column, then y pass -- and the forward FFT along y in LDS;
-//   k_up_rows   one workgroup per wavenumber row: forward FFT along x, the wavenumber filter of
+//   k_up_xpass  one workgroup per irregular row: natural-spline IR->R resampling along x
+//               (SplineGridConvIR2R.cu:539-752) as precomputed window weights, tap-major so that
+//               lanes over columns read them coalesced; the row is staged in LDS;
+//   k_up_ypass  one workgroup per y-padding row of the regular plane: the y pass of the same
+//               resampling, lanes over columns, one weight per tap for the whole workgroup (the
+//               interior rows are the x-pass rows themselves and are not copied);
+//   k_up_ycols  one wavefront per regular column: forward FFT along y in LDS;
+//   k_up_xrows  one wavefront per wavenumber row: forward FFT along x, the wavenumber filter of
 //               FBz_to_FBxFBy (UpContinuation_gpu.cu:113-164) continued to the full spectrum by
-//               Hermitian symmetry (so the C2R of the reference becomes Re of a complex
-//               inverse), and the two inverse FFTs along x;
-//   k_up_cols2  one workgroup per column: inverse FFTs along y, real parts;
+//               Hermitian symmetry, and one inverse FFT along x of Z = FBx + i FBy;
+//   k_up_ycols2 one wavefront per column: inverse FFT along y; Re Z = Bx, Im Z = By;
 //   k_up_bilin  bilinear regular -> irregular with the 1/(L1*M1) normalisation
 //               (UpContinuation_gpu.cu:42-110).
-// FFTs are radix-2 in LDS; other lengths use Bluestein's chirp-z algorithm on a power-of-two
-// length, with FP64 twiddle and chirp tables built on the host.
+// The reference's C2R transform returns the inverse of the Hermitian part of its half spectrum:
+// at a self-conjugate bin it keeps the real part only, and i*g*F is purely imaginary there
+// because F is the transform of a real plane.  The bins where the half-spectrum gains are not
+// already Hermitian-consistent are: the DC bin, the whole x-Nyquist column of the x gain (even in
+// ky, so its Hermitian part is zero) and the y-Nyquist bins of the y gain at kx = 0 and
+// kx = L1/2.  With those gains set to zero both spectra are exactly Hermitian, both inverses are
+// real, and one complex inverse of FBx + i FBy yields Bx and By at once.
+// FFTs: working lengths up to 2048 run Stockham radix-8/4/2 passes between two LDS buffers with
+// the twiddle table staged in LDS, one wavefront per line; longer ones run in-place radix-2 with
+// 256 threads per line and the table in global memory.  Lengths that are not powers of two use
+// Bluestein's chirp-z algorithm on a power-of-two length.  Tables are FP64 from the host.
 #include "tcsem_internal.h"
 
 namespace tcsem {
@@ -21,172 +33,322 @@ namespace tcsem {
 __device__ inline double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
+__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ inline double2 conj2(double2 a) { return make_double2(a.x, -a.y); }
+// a * (s i) for real s: (x + i y) s i = -s y + i s x
+__device__ inline double2 muli(double2 a, double s) { return make_double2(-s * a.y, s * a.x); }
 
-// In-place radix-2 decimation-in-time FFT of buf[0..m), m = 2^log2m, sign -1 forward.
-__device__ void radix2(double2 *buf, int m, int log2m, const double2 *tw, int sign) {
-  const int nt = blockDim.x, tid = threadIdx.x;
-  for (int idx = tid; idx < m; idx += nt) {
-    const int r = (int)(__brev((unsigned)idx) >> (32 - log2m));
-    if (idx < r) {
-      const double2 t = buf[idx];
-      buf[idx] = buf[r];
-      buf[r] = t;
-    }
-  }
-  __syncthreads();
-  for (int s = 1; s <= log2m; ++s) {
-    const int half = 1 << (s - 1);
-    const int stride = m >> s;  // twiddle index step
-    for (int b = tid; b < (m >> 1); b += nt) {
-      const int grp = b >> (s - 1), pos = b & (half - 1);
-      const int i0 = (grp << s) + pos, i1 = i0 + half;
-      double2 w = tw[pos * stride];
-      if (sign > 0) w.y = -w.y;
-      const double2 u = buf[i0], v = cmul(buf[i1], w);
-      buf[i0] = make_double2(u.x + v.x, u.y + v.y);
-      buf[i1] = make_double2(u.x - v.x, u.y - v.y);
-    }
-    __syncthreads();
+// exp(sign 2 pi i q / m) from the m-entry table of exp(-2 pi i q / m), 0 <= q < m
+__device__ inline double2 twid(const double2 *tw, int q, int sign) {
+  double2 w = tw[q];
+  if (sign > 0) w.y = -w.y;
+  return w;
+}
+
+// in-register DFTs of size R with exp(sign 2 pi i / R) as the root
+template <int R>
+__device__ inline void dft(double2 *v, int sign);
+template <>
+__device__ inline void dft<2>(double2 *v, int) {
+  const double2 a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+template <>
+__device__ inline void dft<4>(double2 *v, int sign) {
+  const double s = (double)sign;
+  const double2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+  const double2 t2 = cadd(v[1], v[3]), t3 = muli(csub(v[1], v[3]), s);
+  v[0] = cadd(t0, t2);
+  v[2] = csub(t0, t2);
+  v[1] = cadd(t1, t3);
+  v[3] = csub(t1, t3);
+}
+template <>
+__device__ inline void dft<8>(double2 *v, int sign) {
+  const double s = (double)sign, h = 0.70710678118654752440;  // cos(pi/4)
+  double2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+  dft<4>(e, sign);
+  dft<4>(o, sign);
+  // w^k, w = exp(sign i pi / 4): w = (h, s h), w^2 = s i, w^3 = (-h, s h)
+  o[1] = cmul(o[1], make_double2(h, s * h));
+  o[2] = muli(o[2], s);
+  o[3] = cmul(o[3], make_double2(-h, s * h));
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    v[k] = cadd(e[k], o[k]);
+    v[k + 4] = csub(e[k], o[k]);
   }
 }
 
-// Unnormalised DFT of buf[0..n) in place (buf has room for P.m entries).  The caller has
-// written buf[0..n) and synchronised.
-__device__ void fft_line(double2 *buf, const FftPlan &P, int sign) {
-  const double2 *tw = (const double2 *)P.tw;
-  if (P.m == P.n) {
-    radix2(buf, P.m, P.log2m, tw, sign);
-    return;
+// One Stockham autosort pass of radix R over length m, reading a and writing b; Ns is the
+// length of the sub-transforms already done.  Butterfly j reads a[j + r m/R], applies the
+// twiddles exp(sign 2 pi i r (j mod Ns) / (Ns R)), and writes b[(j - j mod Ns) R + j mod Ns + r Ns].
+template <int R>
+__device__ inline void stockham_pass(const double2 *a, double2 *b, int m, int Ns, const double2 *tw, int sign,
+                                     int tid, int nthr) {
+  const int nb = m / R, span = m / (Ns * R);
+  for (int j = tid; j < nb; j += nthr) {
+    const int k = j & (Ns - 1);
+    double2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) v[r] = a[j + r * nb];
+    if (Ns > 1) {
+#pragma unroll
+      for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twid(tw, r * k * span, sign));
+    }
+    dft<R>(v, sign);
+    const int d = (j - k) * R + k;
+#pragma unroll
+    for (int r = 0; r < R; ++r) b[d + r * Ns] = v[r];
   }
+}
+
+// Unnormalised DFT of length m = 2^log2m of the line in a; with PP the passes alternate between
+// a and b (m entries each), otherwise they run in place in a.  Returns the buffer holding the
+// result.  Every thread of the workgroup calls it with the same m (it synchronises the workgroup).
+template <bool PP>
+__device__ double2 *fft_pow2(double2 *a, double2 *b, int m, int log2m, const double2 *tw, int sign, int tid,
+                             int nthr) {
+  if constexpr (PP) {
+    int Ns = 1, left = log2m;
+    while (left > 0) {
+      if (left >= 3) {
+        stockham_pass<8>(a, b, m, Ns, tw, sign, tid, nthr);
+        Ns <<= 3;
+        left -= 3;
+      } else if (left == 2) {
+        stockham_pass<4>(a, b, m, Ns, tw, sign, tid, nthr);
+        Ns <<= 2;
+        left -= 2;
+      } else {
+        stockham_pass<2>(a, b, m, Ns, tw, sign, tid, nthr);
+        Ns <<= 1;
+        left -= 1;
+      }
+      __syncthreads();
+      double2 *t = a;
+      a = b;
+      b = t;
+    }
+    return a;
+  } else {
+    // in-place radix-2 decimation in time
+    for (int idx = tid; idx < m; idx += nthr) {
+      const int r = (int)(__brev((unsigned)idx) >> (32 - log2m));
+      if (idx < r) {
+        const double2 t = a[idx];
+        a[idx] = a[r];
+        a[r] = t;
+      }
+    }
+    __syncthreads();
+    for (int s = 1; s <= log2m; ++s) {
+      const int half = 1 << (s - 1), stride = m >> s;
+      for (int bf = tid; bf < (m >> 1); bf += nthr) {
+        const int grp = bf >> (s - 1), pos = bf & (half - 1);
+        const int i0 = (grp << s) + pos, i1 = i0 + half;
+        const double2 u = a[i0], v = cmul(a[i1], twid(tw, pos * stride, sign));
+        a[i0] = cadd(u, v);
+        a[i1] = csub(u, v);
+      }
+      __syncthreads();
+    }
+    return a;
+  }
+}
+
+// Unnormalised DFT of length P.n of a[0..n) (a and, with PP, b hold P.m entries; tw is the
+// m-entry twiddle table, staged in LDS with PP).  The caller has written a[0..n) and
+// synchronised; returns the buffer holding the result in [0, n).
+template <bool PP>
+__device__ double2 *fft_line(double2 *a, double2 *b, const FftPlan &P, const double2 *tw, int sign, int tid,
+                             int nthr) {
+  if (P.m == P.n) return fft_pow2<PP>(a, b, P.m, P.log2m, tw, sign, tid, nthr);
   const double2 *w = (const double2 *)P.chirp, *bh = (const double2 *)P.bhat;
-  const int nt = blockDim.x, tid = threadIdx.x;
-  for (int k = tid; k < P.m; k += nt) {
+  for (int k = tid; k < P.m; k += nthr) {
     if (k < P.n) {
-      double2 x = buf[k];
+      double2 x = a[k];
       if (sign > 0) x = conj2(x);  // inverse = conj(forward(conj))
-      buf[k] = cmul(x, w[k]);
+      a[k] = cmul(x, w[k]);
     } else {
-      buf[k] = make_double2(0.0, 0.0);
+      a[k] = make_double2(0.0, 0.0);
     }
   }
   __syncthreads();
-  radix2(buf, P.m, P.log2m, tw, -1);
-  for (int k = tid; k < P.m; k += nt) buf[k] = cmul(buf[k], bh[k]);
+  double2 *r = fft_pow2<PP>(a, b, P.m, P.log2m, tw, -1, tid, nthr);
+  double2 *o = (r == a) ? b : a;
+  for (int k = tid; k < P.m; k += nthr) r[k] = cmul(r[k], bh[k]);
   __syncthreads();
-  radix2(buf, P.m, P.log2m, tw, +1);
-  for (int k = tid; k < P.n; k += nt) {
-    double2 y = cmul(buf[k], w[k]);
+  r = fft_pow2<PP>(r, o, P.m, P.log2m, tw, +1, tid, nthr);
+  for (int k = tid; k < P.n; k += nthr) {
+    double2 y = cmul(r[k], w[k]);
     if (sign > 0) y = conj2(y);
-    buf[k] = y;
+    r[k] = y;
   }
   __syncthreads();
+  return r;
+}
+
+// LDS (double2 entries) of one line and of the staged twiddle table
+__host__ __device__ inline int line_span(int m, bool pp) { return pp ? 2 * m : m; }
+__host__ __device__ inline int table_span(int m, bool pp) { return pp ? m : 0; }
+
+// Per-workgroup LDS set-up of a line kernel: the line buffers of this wavefront and the twiddle
+// table (copied by the whole workgroup; the caller synchronises before use).
+template <bool PP>
+__device__ inline const double2 *stage_table(double2 *lds, const FftPlan &P, int lines) {
+  if constexpr (PP) {
+    double2 *t = lds + (long long)lines * line_span(P.m, true);
+    const double2 *g = (const double2 *)P.tw;
+    const int flat = threadIdx.y * blockDim.x + threadIdx.x, nt = blockDim.x * blockDim.y;
+    for (int q = flat; q < P.m; q += nt) t[q] = g[q];
+    return t;
+  } else {
+    return (const double2 *)P.tw;
+  }
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_up_cols(const UpcontArgs<T> u) {
-  extern __shared__ double2 lds[];
-  const int c = blockIdx.x, slot = blockIdx.y, tid = threadIdx.x, nt = blockDim.x;
-  double2 *buf = lds;                        // FFT buffer, u.py.m entries
-  double *tcol = (double *)(lds + u.py.m);   // x-pass column, M entries
-  const T *bz0 = u.bz0 + (long long)slot * u.pstride;
-  const int M = u.M, PX = u.PX;
-  // x pass for this regular column: T(j, c) for every old row j
-  for (int j = tid; j < M; j += nt) {
-    const T *row = bz0 + (long long)j * PX;
-    double v;
-    if (c >= u.AnLx && c < u.AnLx + u.Nx) {
-      v = (double)row[u.AL + c - u.AnLx];
-    } else if (c < u.AnLx) {
-      const double *w = u.wl + (long long)c * u.n1;
-      v = 0.0;
-      for (int q = 0; q < u.n1; ++q) v += w[q] * (double)row[q];
-    } else {
-      const int cr = c - u.AnLx - u.Nx, ox = u.Nx + u.AL - kNPad;
-      const double *w = u.wr + (long long)cr * u.n2;
-      v = 0.0;
-      for (int q = 0; q < u.n2; ++q) v += w[q] * (double)row[ox + q];
-    }
-    tcol[j] = v;
-  }
+__global__ void __launch_bounds__(256) k_up_xpass(const UpcontArgs<T> u) {
+  extern __shared__ double srow[];
+  const int j = blockIdx.x, slot = blockIdx.y;
+  const T *row = u.bz0 + (long long)slot * u.pstride + (long long)j * u.PX;
+  for (int i = threadIdx.x; i < u.L; i += blockDim.x) srow[i] = (double)row[i];
   __syncthreads();
-  // y pass: BzAir(r, c) for every regular row r
-  const int oy = u.Ny + u.AY - kNPad;
-  for (int r = tid; r < u.M1; r += nt) {
-    double v;
-    if (r >= u.Any && r < u.Any + u.Ny) {
-      v = tcol[u.AY + r - u.Any];
-    } else if (r < u.Any) {
-      const double *w = u.wf + (long long)r * u.n3;
-      v = 0.0;
-      for (int q = 0; q < u.n3; ++q) v += w[q] * tcol[q];
+  double *trow = u.tx + (long long)slot * u.wstride + (long long)j * u.L1;
+  const int ox = u.Nx + u.AL - kNPad, aL = u.AnLx, aR = u.AnRx;
+  for (int c = threadIdx.x; c < u.L1; c += blockDim.x) {
+    double v = 0.0;
+    if (c >= aL && c < aL + u.Nx) {
+      v = srow[u.AL + c - aL];
+    } else if (c < aL) {
+#pragma unroll 8
+      for (int q = 0; q < u.n1; ++q) v += u.wl[(long long)q * aL + c] * srow[q];
     } else {
-      const double *w = u.wb + (long long)(r - u.Any - u.Ny) * u.n3;
-      v = 0.0;
-      for (int q = 0; q < u.n3; ++q) v += w[q] * tcol[oy + q];
+      const int cr = c - aL - u.Nx;
+#pragma unroll 8
+      for (int q = 0; q < u.n2; ++q) v += u.wr[(long long)q * aR + cr] * srow[ox + q];
     }
-    buf[r] = make_double2(v, 0.0);
+    trow[c] = v;
   }
-  __syncthreads();
-  fft_line(buf, u.py, -1);
-  double2 *out = u.cplx + (long long)slot * u.wstride;
-  for (int r = tid; r < u.M1; r += nt) out[(long long)r * u.L1 + c] = buf[r];
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) k_up_rows(const UpcontArgs<T> u) {
-  extern __shared__ double2 lds[];
-  const int ky = blockIdx.x, slot = blockIdx.y, tid = threadIdx.x, nt = blockDim.x;
-  const int L1 = u.L1, M1 = u.M1, H = u.H;
-  double2 *row = u.cplx + (long long)slot * u.wstride + (long long)ky * L1;
-  double2 *row2 = u.cplx2 + (long long)slot * u.wstride + (long long)ky * L1;
-  double2 *buf = lds;
-  for (int x = tid; x < L1; x += nt) buf[x] = row[x];
-  __syncthreads();
-  fft_line(buf, u.px, -1);
-  // filter: FBx = i g_x FBz on the half spectrum; the Hermitian completion of the half spectrum
-  // gives -i g(L1-kx, -ky) FBz on the other half.
-  const int kym = (M1 - ky) % M1;
-  for (int kx = tid; kx < L1; kx += nt) {
-    const double2 f = buf[kx];
-    double gx, gy, sgn;
-    if (kx < H) {
-      gx = u.gx[(long long)ky * H + kx];
-      gy = u.gy[(long long)ky * H + kx];
-      sgn = 1.0;
-    } else {
-      gx = u.gx[(long long)kym * H + (L1 - kx)];
-      gy = u.gy[(long long)kym * H + (L1 - kx)];
-      sgn = -1.0;
-    }
-    // (i sgn g) * f
-    row2[kx] = make_double2(-sgn * gy * f.y, sgn * gy * f.x);
-    buf[kx] = make_double2(-sgn * gx * f.y, sgn * gx * f.x);
+__global__ void __launch_bounds__(256) k_up_ypass(const UpcontArgs<T> u) {
+  const int slot = blockIdx.y;
+  int r, src;
+  const double *w;
+  if ((int)blockIdx.x < u.Any) {
+    r = blockIdx.x;
+    w = u.wf + (long long)r * u.n3;
+    src = 0;
+  } else {
+    const int rb = blockIdx.x - u.Any;
+    r = u.Any + u.Ny + rb;
+    w = u.wb + (long long)rb * u.n3;
+    src = u.Ny + u.AY - kNPad;
   }
-  __syncthreads();
-  fft_line(buf, u.px, +1);
-  for (int x = tid; x < L1; x += nt) row[x] = buf[x];
-  __syncthreads();
-  for (int x = tid; x < L1; x += nt) buf[x] = row2[x];
-  __syncthreads();
-  fft_line(buf, u.px, +1);
-  for (int x = tid; x < L1; x += nt) row2[x] = buf[x];
+  const double *tx = u.tx + (long long)slot * u.wstride + (long long)src * u.L1;
+  double *out = u.ry + (long long)slot * u.wstride + (long long)r * u.L1;
+  for (int c = threadIdx.x; c < u.L1; c += blockDim.x) {
+    double v = 0.0;
+#pragma unroll 8
+    for (int q = 0; q < u.n3; ++q) v += w[q] * tx[(long long)q * u.L1 + c];
+    out[c] = v;
+  }
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_up_cols2(const UpcontArgs<T> u) {
+template <typename T, bool PP>
+__global__ void __launch_bounds__(256) k_up_ycols(const UpcontArgs<T> u) {
   extern __shared__ double2 lds[];
-  const int c = blockIdx.x, slot = blockIdx.y, tid = threadIdx.x, nt = blockDim.x;
-  const int L1 = u.L1, M1 = u.M1;
-  double2 *buf = lds;
-  for (int pass = 0; pass < 2; ++pass) {
-    const double2 *src = (pass == 0 ? u.cplx : u.cplx2) + (long long)slot * u.wstride;
-    double *dst = (pass == 0 ? u.bxr : u.byr) + (long long)slot * u.wstride;
-    for (int r = tid; r < M1; r += nt) buf[r] = src[(long long)r * L1 + c];
-    __syncthreads();
-    fft_line(buf, u.py, +1);
-    for (int r = tid; r < M1; r += nt) dst[(long long)r * L1 + c] = buf[r].x;
-    __syncthreads();
+  const int ln = threadIdx.y, tid = threadIdx.x, nthr = blockDim.x;
+  const int c = blockIdx.x * blockDim.y + ln, slot = blockIdx.y;
+  const bool live = c < u.L1;
+  const int m = u.py.m;
+  double2 *a = lds + (long long)ln * line_span(m, PP);
+  double2 *b = a + m;
+  const double2 *tw = stage_table<PP>(lds, u.py, blockDim.y);
+  const double *tx = u.tx + (long long)slot * u.wstride, *ry = u.ry + (long long)slot * u.wstride;
+  const int y0 = u.Any, y1 = u.Any + u.Ny, sh = u.AY - u.Any;
+  for (int r = tid; r < u.M1; r += nthr) {
+    double v = 0.0;
+    if (live) v = (r >= y0 && r < y1) ? tx[(long long)(r + sh) * u.L1 + c] : ry[(long long)r * u.L1 + c];
+    a[r] = make_double2(v, 0.0);
+  }
+  __syncthreads();
+  double2 *res = fft_line<PP>(a, b, u.py, tw, -1, tid, nthr);
+  if (live) {
+    double2 *out = u.cplx + (long long)slot * u.wstride;
+    for (int r = tid; r < u.M1; r += nthr) out[(long long)r * u.L1 + c] = res[r];
+  }
+}
+
+template <typename T, bool PP>
+__global__ void __launch_bounds__(256) k_up_xrows(const UpcontArgs<T> u) {
+  extern __shared__ double2 lds[];
+  const int ln = threadIdx.y, tid = threadIdx.x, nthr = blockDim.x;
+  const int ky = blockIdx.x * blockDim.y + ln, slot = blockIdx.y;
+  const int L1 = u.L1, M1 = u.M1, H = u.H, m = u.px.m;
+  const bool live = ky < M1;
+  double2 *a = lds + (long long)ln * line_span(m, PP);
+  double2 *b = a + m;
+  const double2 *tw = stage_table<PP>(lds, u.px, blockDim.y);
+  double2 *row = u.cplx + (long long)slot * u.wstride + (long long)(live ? ky : 0) * L1;
+  for (int x = tid; x < L1; x += nthr) a[x] = live ? row[x] : make_double2(0.0, 0.0);
+  __syncthreads();
+  double2 *f = fft_line<PP>(a, b, u.px, tw, -1, tid, nthr);
+  double2 *o = (f == a) ? b : a;
+  const int kym = live ? (M1 - ky) % M1 : 0;
+  const bool evenL = (L1 & 1) == 0, evenM = (M1 & 1) == 0;
+  // in place: each element is read and written by the same thread
+  for (int kx = tid; kx < L1; kx += nthr) {
+    const double2 fv = f[kx];
+    double gx = 0.0, gy = 0.0, s = 1.0;
+    if (live) {
+      if (kx < H) {
+        gx = u.gx[(long long)ky * H + kx];
+        gy = u.gy[(long long)ky * H + kx];
+      } else {  // Hermitian completion: S(k) = conj(S(-k)) = -i g(-k) F(k)
+        gx = u.gx[(long long)kym * H + (L1 - kx)];
+        gy = u.gy[(long long)kym * H + (L1 - kx)];
+        s = -1.0;
+      }
+      if (ky == 0 && kx == 0) gx = gy = 0.0;
+      if (evenL && kx == L1 / 2) gx = 0.0;
+      if (evenM && ky == M1 / 2 && (kx == 0 || (evenL && kx == L1 / 2))) gy = 0.0;
+    }
+    // Z = (i s gx) F + i (i s gy) F
+    const double2 sx = muli(fv, s * gx), sy = muli(fv, s * gy);
+    f[kx] = make_double2(sx.x - sy.y, sx.y + sy.x);
+  }
+  __syncthreads();
+  double2 *z = fft_line<PP>(f, o, u.px, tw, +1, tid, nthr);
+  if (live)
+    for (int x = tid; x < L1; x += nthr) row[x] = z[x];
+}
+
+template <typename T, bool PP>
+__global__ void __launch_bounds__(256) k_up_ycols2(const UpcontArgs<T> u) {
+  extern __shared__ double2 lds[];
+  const int ln = threadIdx.y, tid = threadIdx.x, nthr = blockDim.x;
+  const int c = blockIdx.x * blockDim.y + ln, slot = blockIdx.y;
+  const int L1 = u.L1, M1 = u.M1, m = u.py.m;
+  const bool live = c < L1;
+  double2 *a = lds + (long long)ln * line_span(m, PP);
+  double2 *b = a + m;
+  const double2 *tw = stage_table<PP>(lds, u.py, blockDim.y);
+  const double2 *src = u.cplx + (long long)slot * u.wstride;
+  for (int r = tid; r < M1; r += nthr) a[r] = live ? src[(long long)r * L1 + c] : make_double2(0.0, 0.0);
+  __syncthreads();
+  double2 *z = fft_line<PP>(a, b, u.py, tw, +1, tid, nthr);
+  if (live) {
+    double *bxr = u.bxr + (long long)slot * u.wstride, *byr = u.byr + (long long)slot * u.wstride;
+    for (int r = tid; r < M1; r += nthr) {
+      bxr[(long long)r * L1 + c] = z[r].x;
+      byr[(long long)r * L1 + c] = z[r].y;
+    }
   }
 }
 
@@ -221,25 +383,57 @@ __global__ void k_up_bilin(const UpcontArgs<T> u) {
   }
 }
 
+namespace {
+// Launch shape of a line kernel: one wavefront per line (4 lines per workgroup when there are
+// enough lines to fill the chip) for Stockham lengths; one 256-thread workgroup per line for the
+// in-place lengths.
+struct LineShape {
+  int lines, threads;
+  size_t lds;
+};
+LineShape line_shape(const FftPlan &P, bool pp, long long nlines) {
+  const size_t per = (size_t)line_span(P.m, pp) * sizeof(double2), tab = (size_t)table_span(P.m, pp) * sizeof(double2);
+  if (!pp) return {1, 256, per};
+  const int lines = (nlines >= 2048 && per * 4 + tab <= 96 * 1024) ? 4 : 1;
+  return {lines, 64, per * lines + tab};
+}
+template <typename K>
+hipError_t allow_lds(K kern, size_t bytes) {
+  constexpr size_t kDefaultLds = 64 * 1024;
+  if (bytes <= kDefaultLds) return hipSuccess;
+  return hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+template <typename K, typename A>
+hipError_t launch_lines(K kern, const LineShape &sh, int nlines, int nslot, hipStream_t s, const A &u) {
+  hipError_t e = allow_lds(kern, sh.lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(kern, dim3((nlines + sh.lines - 1) / sh.lines, nslot), dim3(sh.threads, sh.lines), sh.lds, s, u);
+  return hipGetLastError();
+}
+}  // namespace
+
 template <typename T>
 hipError_t launch_upcont(const UpcontArgs<T> &u, int nslot, hipStream_t s) {
-  const size_t lds1 = sizeof(double2) * u.py.m + sizeof(double) * u.M;
-  constexpr size_t kDefaultLds = 64 * 1024;
-  if (lds1 > kDefaultLds || sizeof(double2) * u.px.m > kDefaultLds || sizeof(double2) * u.py.m > kDefaultLds) {
-    // planes past 4096 points per axis use up to 160 KiB of LDS per workgroup on gfx950
-    (void)hipFuncSetAttribute((const void *)k_up_cols<T>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds1);
-    (void)hipFuncSetAttribute((const void *)k_up_rows<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        (int)(sizeof(double2) * u.px.m));
-    (void)hipFuncSetAttribute((const void *)k_up_cols2<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                        (int)(sizeof(double2) * u.py.m));
+  hipError_t e;
+  hipLaunchKernelGGL((k_up_xpass<T>), dim3(u.M, nslot), dim3(256), (size_t)u.L * sizeof(double), s, u);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  const int npad = u.M1 - u.Ny;
+  if (npad > 0) {
+    hipLaunchKernelGGL((k_up_ypass<T>), dim3(npad, nslot), dim3(256), 0, s, u);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
   }
-  hipLaunchKernelGGL((k_up_cols<T>), dim3(u.L1, nslot), dim3(256), lds1, s, u);
-  hipError_t e = hipGetLastError();
+  const bool ppx = u.px.m <= 2048, ppy = u.py.m <= 2048;
+  const LineShape sc = line_shape(u.py, ppy, (long long)u.L1 * nslot);
+  const LineShape sr = line_shape(u.px, ppx, (long long)u.M1 * nslot);
+  e = ppy ? launch_lines(k_up_ycols<T, true>, sc, u.L1, nslot, s, u)
+          : launch_lines(k_up_ycols<T, false>, sc, u.L1, nslot, s, u);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_up_rows<T>), dim3(u.M1, nslot), dim3(256), sizeof(double2) * u.px.m, s, u);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
-  hipLaunchKernelGGL((k_up_cols2<T>), dim3(u.L1, nslot), dim3(256), sizeof(double2) * u.py.m, s, u);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
+  e = ppx ? launch_lines(k_up_xrows<T, true>, sr, u.M1, nslot, s, u)
+          : launch_lines(k_up_xrows<T, false>, sr, u.M1, nslot, s, u);
+  if (e != hipSuccess) return e;
+  e = ppy ? launch_lines(k_up_ycols2<T, true>, sc, u.L1, nslot, s, u)
+          : launch_lines(k_up_ycols2<T, false>, sc, u.L1, nslot, s, u);
+  if (e != hipSuccess) return e;
   const int n = u.M * (u.L + 1) + (u.M + 1) * u.L;
   hipLaunchKernelGGL((k_up_bilin<T>), dim3((n + 255) / 256, nslot), dim3(256), 0, s, u);
   return hipGetLastError();
