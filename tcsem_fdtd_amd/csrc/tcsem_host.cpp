@@ -435,6 +435,10 @@ struct tcsem_ctx {
   double *d_ef0 = nullptr, *d_efs = nullptr;
   SrcCell *d_src = nullptr;
   int *d_srcidx = nullptr;  // first[nbatch], count[nbatch]
+  // source pass after the march (levels 1..N-1): merged E cells and the B cells they touch
+  SrcCell *d_fixe = nullptr, *d_fixb = nullptr;
+  int *d_fixidx = nullptr;  // E first[nbatch], E count[nbatch], B first[nbatch], B count[nbatch]
+  bool has_fix = false;
   int *d_rxl = nullptr;     // rx_local, rx_first[nbatch], rx_count[nbatch]
   int *d_ops = nullptr;     // it of every op of the run (E ops then B ops)
   double *d_opl1 = nullptr;
@@ -621,6 +625,12 @@ void free_run(tcsem_ctx *c) {
   (void)hipFree(c->d_efs);
   (void)hipFree(c->d_src);
   (void)hipFree(c->d_srcidx);
+  (void)hipFree(c->d_fixe);
+  (void)hipFree(c->d_fixb);
+  (void)hipFree(c->d_fixidx);
+  c->d_fixe = c->d_fixb = nullptr;
+  c->d_fixidx = nullptr;
+  c->has_fix = false;
   (void)hipFree(c->d_rxl);
   (void)hipFree(c->d_ops);
   (void)hipFree(c->d_opl1);
@@ -938,6 +948,8 @@ int do_step(tcsem_ctx *c) {
     }
     HIPCHK(launch_march<T>(hi, nb, st));
   }
+  if (c->has_fix && a.amp != 0.0)
+    HIPCHK(launch_src_fix<T>(a, c->d_fixe, c->d_fixidx, c->d_fixb, c->d_fixidx + 2 * nb, nb, st));
   if (c->timing) HIPCHK(hipEventRecord(c->ev[3], st));
   HIPCHK(hipEventRecord(c->xev[0], st));
   HIPCHK(hipStreamWaitEvent(st2, c->xev[0], 0));
@@ -1368,6 +1380,12 @@ int tcsem_create(const tcsem_setup *s, const double *cond, int precision, tcsem_
     make_plan(M1, c->ply);
   }
   const size_t lds_max = 160 * 1024 - 1024;
+  // the march's masked stores go through a buffer resource per slot field that must stay below
+  // 2 GiB (out-of-range offsets are its "drop" mask)
+  if ((unsigned long long)c->fstride * c->esz >= (1ull << 31)) {
+    delete c;
+    return fail(TCSEM_ESIZE, "one field of %d x %d x %d cells exceeds 2 GiB", L, M, N);
+  }
   // the march keeps 3 N z-coefficients in LDS beside at most 64 KiB of tiles
   if ((size_t)3 * N * c->esz + 64 * 1024 > lds_max) {
     delete c;
@@ -1604,6 +1622,85 @@ int tcsem_reset_fields(tcsem_ctx *c, int first, int n) {
   return TCSEM_OK;
 }
 
+// Lists of the source pass that follows the volume march (levels 1 .. N-1): every E cell of a
+// slot's table once, with the weights of coincident entries summed in table order (the order of
+// the per-cell loop of ElectricalFiled.cu:347-475), then every Bx/By cell whose curl reads one of
+// them (Bderivate.cu:79-200).  Cells outside the update masks are dropped, as the march drops
+// them; level-0 cells stay with the surface kernel.
+static int build_fix_lists(tcsem_ctx *c, int nbatch) {
+  const int L = c->g.info.L, M = c->g.info.M, N = c->g.info.N;
+  auto e_ok = [&](const SrcCell &s) {
+    if (s.k < 1 || s.k > N - 1 || s.i < 0 || s.j < 0) return false;
+    if (s.comp == 0) return s.i < L && s.j >= 1 && s.j < M;
+    if (s.comp == 1) return s.i >= 1 && s.i < L && s.j < M;
+    return s.i >= 1 && s.i < L && s.j >= 1 && s.j < M;
+  };
+  auto b_ok = [&](int comp, int k, int j, int i) {
+    if (k < 1 || k > N - 1 || i < 0 || j < 0) return false;
+    if (comp == 3) return i >= 1 && i < L && j < M;
+    return i < L && j >= 1 && j < M;
+  };
+  std::vector<SrcCell> ecells, bcells;
+  std::vector<int> idx(4 * nbatch, 0);
+  for (int b = 0; b < nbatch; ++b) {
+    idx[b] = (int)ecells.size();
+    idx[2 * nbatch + b] = (int)bcells.size();
+    if (!c->tx.empty()) {
+      const TxLoc &t = c->tx[c->tx_first + b];
+      const size_t e0 = ecells.size(), b0 = bcells.size();
+      for (const SrcCell &s : t.cells) {
+        if (!e_ok(s)) continue;
+        bool merged = false;
+        for (size_t q = e0; q < ecells.size(); ++q) {
+          SrcCell &m = ecells[q];
+          if (m.comp == s.comp && m.k == s.k && m.j == s.j && m.i == s.i) {
+            m.w += s.w;
+            merged = true;
+            break;
+          }
+        }
+        if (!merged) ecells.push_back(s);
+      }
+      auto add_b = [&](int comp, int k, int j, int i) {
+        if (!b_ok(comp, k, j, i)) return;
+        for (size_t q = b0; q < bcells.size(); ++q) {
+          const SrcCell &m = bcells[q];
+          if (m.comp == comp && m.k == k && m.j == j && m.i == i) return;
+        }
+        bcells.push_back({comp, k, j, i, 0.0});
+      };
+      for (size_t q = e0; q < ecells.size(); ++q) {
+        const SrcCell s = ecells[q];
+        if (s.comp == 0) {  // Ex feeds By(k) and By(k-1)
+          add_b(4, s.k, s.j, s.i);
+          add_b(4, s.k - 1, s.j, s.i);
+        } else if (s.comp == 1) {  // Ey feeds Bx(k) and Bx(k-1)
+          add_b(3, s.k, s.j, s.i);
+          add_b(3, s.k - 1, s.j, s.i);
+        } else {  // Ez feeds Bx(j), Bx(j-1), By(i), By(i-1)
+          add_b(3, s.k, s.j, s.i);
+          add_b(3, s.k, s.j - 1, s.i);
+          add_b(4, s.k, s.j, s.i);
+          add_b(4, s.k, s.j, s.i - 1);
+        }
+      }
+    }
+    idx[nbatch + b] = (int)ecells.size() - idx[b];
+    idx[3 * nbatch + b] = (int)bcells.size() - idx[2 * nbatch + b];
+  }
+  c->has_fix = !ecells.empty();
+  if (ecells.empty()) ecells.push_back({-1, -1, -1, -1, 0.0});
+  if (bcells.empty()) bcells.push_back({-1, -1, -1, -1, 0.0});
+  if (hipMalloc((void **)&c->d_fixe, ecells.size() * sizeof(SrcCell)) != hipSuccess ||
+      hipMalloc((void **)&c->d_fixb, bcells.size() * sizeof(SrcCell)) != hipSuccess ||
+      hipMalloc((void **)&c->d_fixidx, idx.size() * sizeof(int)) != hipSuccess)
+    return fail(TCSEM_ENOMEM, "source table allocation");
+  HIPCHK(hipMemcpy(c->d_fixe, ecells.data(), ecells.size() * sizeof(SrcCell), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_fixb, bcells.data(), bcells.size() * sizeof(SrcCell), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_fixidx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
+  return TCSEM_OK;
+}
+
 int tcsem_begin(tcsem_ctx *c, int tx_first, int nbatch) {
   if (check_ctx(c)) return TCSEM_EINVAL;
   if (nbatch < 1) return fail(TCSEM_EINVAL, "batch must hold at least one transmitter");
@@ -1640,6 +1737,7 @@ int tcsem_begin(tcsem_ctx *c, int tx_first, int nbatch) {
     return fail(TCSEM_ENOMEM, "source table allocation");
   HIPCHK(hipMemcpy(c->d_src, cells.data(), cells.size() * sizeof(SrcCell), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_srcidx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
+  if ((rc = build_fix_lists(c, nbatch))) return rc;
   // receivers (per slot)
   c->nrmax = 1;
   if (c->have_rx) {

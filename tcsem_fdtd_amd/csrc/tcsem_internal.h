@@ -23,8 +23,9 @@ constexpr int kMarchLanes = 64;
 constexpr int kMarchOwnX = kMarchLanes - 2;
 
 // One injected E cell: E[comp](k, j, i) -= amp * w each source step (ElectricalFiled.cu:347-475).
+// The same record lists the B cells a source step recomputes (comp 3 Bx, 4 By; w unused).
 struct SrcCell {
-  int comp;  // 0 Ex, 1 Ey, 2 Ez
+  int comp;  // 0 Ex, 1 Ey, 2 Ez (3 Bx, 4 By in the recompute list)
   int k, j, i;
   double w;
 };
@@ -117,6 +118,9 @@ template <typename T>
 hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s);
 template <typename T>
 hipError_t launch_surface(const MarchArgs<T> &a, int nslot, hipStream_t s);
+template <typename T>
+hipError_t launch_src_fix(const MarchArgs<T> &a, const SrcCell *ecells, const int *eidx, const SrcCell *bcells,
+                          const int *bidx, int nslot, hipStream_t s);
 template <typename T>
 hipError_t launch_bz_volume(const MarchArgs<T> &a, T *bz, int slot, hipStream_t s);
 template <typename T>

@@ -142,6 +142,26 @@ __device__ __forceinline__ void sto(T *__restrict__ base, unsigned off, T v) {
   *(T *)((char *)base + off) = v;
 }
 
+// Branch-free masked stores: a raw buffer resource over one slot's field drops any store whose
+// offset lies past its size, so an invalid lane stores at kDrop instead of branching around the
+// store (the host keeps every slot's field below 2 GiB).
+constexpr unsigned kDrop = 0x80000000u;
+constexpr int kRsrcWord3 = 0x00020000;  // gfx9-family raw buffer: 32-bit data format
+__device__ inline __amdgpu_buffer_rsrc_t rsrc_of(void *base, unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)bytes, kRsrcWord3);
+}
+__device__ __forceinline__ void bsto(__amdgpu_buffer_rsrc_t r, unsigned off, bool ok, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)(ok ? off : kDrop), 0, 0);
+}
+__device__ __forceinline__ void bsto(__amdgpu_buffer_rsrc_t r, unsigned off, bool ok, double v) {
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  u32x2 w;
+  w.x = (unsigned)b;
+  w.y = (unsigned)(b >> 32);
+  __builtin_amdgcn_raw_buffer_store_b64(w, r, (int)(ok ? off : kDrop), 0, 0);
+}
+
 // Per-lane geometry shared by the march and the surface kernel.  Every lane computes every
 // quantity (no divergent loads); the validity masks select what is kept and stored.
 template <typename T>
@@ -208,7 +228,9 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> a)
   T *__restrict__ by = a.by + fo;
   const T *__restrict__ sig = a.sig - (a.PX + 1);
   const unsigned PSb = (unsigned)a.PS * (unsigned)sizeof(T);
-  const int sfirst = a.src_first ? a.src_first[slot] : 0, scount = a.src_count ? a.src_count[slot] : 0;
+  const unsigned fbytes = (unsigned)(a.fstride * (long long)sizeof(T));
+  const __amdgpu_buffer_rsrc_t rex = rsrc_of(ex, fbytes), rey = rsrc_of(ey, fbytes), rez = rsrc_of(ez, fbytes);
+  const __amdgpu_buffer_rsrc_t rbx = rsrc_of(bx, fbytes), rby = rsrc_of(by, fbytes);
   const auto cf = make_coef<T>(a.lam, a.dt);
   const T dtb = (T)a.dtb, zero = 0;
   const T rdx = ln.rdx, rdx2 = ln.rdx2, rdy = ln.rdy, rdy2 = ln.rdy2;
@@ -242,10 +264,9 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> a)
   T *sZ = (T *)zdyn;
   const int N = a.N;
   for (int q = ly * kMarchLanes + lx; q < 3 * N; q += kMarchLanes * BY) sZ[q] = a.dzt[q];
-  int p = khi & 1;
-  sBx[p][ly][lx] = R[0].bx;
-  sBy[p][ly][lx] = R[0].by;
-  sS[p][ly][lx] = R[0].s;
+  sBx[khi & 1][ly][lx] = R[0].bx;
+  sBy[khi & 1][ly][lx] = R[0].by;
+  sS[khi & 1][ly][lx] = R[0].s;
   Acc<T> bz;
   bz.init();
   T ex_n1 = 0, ey_n1 = 0;  // E_new at level k+1 (zero at the bottom wall, level N)
@@ -260,69 +281,70 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> a)
   // would otherwise keep the loop-header state pending and cost a full wait every level.
   __builtin_amdgcn_s_waitcnt(0);
 
-  for (int kb = khi; kb >= klo; kb -= S) {
-#pragma unroll
-    for (int u = 0; u < S; ++u) {
-      const int k = kb - u;
-      if (k < klo) break;
-      const Lv<T> &cur = R[u];
-      const Lv<T> &nxt = R[(u + 1) % S];
-      p = k & 1;
-      const int q = p ^ 1;
-      const T dzk = sZ[k], rdzk = sZ[N + k], rdz2k = sZ[2 * N + k];
-      const T bx_k = cur.bx, by_k = cur.by, bx_m = nxt.bx, by_m = nxt.by;
-      // (a) Bz_old(k) = Bz_old(k+1) + dz(k) div_h(B_old(k))
-      bz.add(dzk * ((sBx[p][ly][lxp] - bx_k) * rdx + (sBy[p][lyp][lx] - by_k) * rdy));
-      const T bzk = bz.val();
-      sBz[ly][lx] = bzk;
-      // (b) level k-1 into LDS; loads of level k-1-PF
-      sBx[q][ly][lx] = bx_m;
-      sBy[q][ly][lx] = by_m;
-      sS[q][ly][lx] = nxt.s;
-      bx_last = bx_m;
-      by_last = by_m;
-      const T ezo = cur.ez, exo = cur.ex, eyo = cur.ey;
-      R[(u + S - 1) % S] = load(k - 1 - PF);
-      __syncthreads();
-      // (c) Ez(k) on the tile and its +x/+y halo
-      T ezn;
-      {
-        const T s4 = (T)0.25 * (sS[p][lym][lxm] + sS[p][lym][lx] + sS[p][ly][lxm] + sS[p][ly][lx]);
-        const T curl = (by_k - sBy[p][ly][lxm]) * rdx2 - (bx_k - sBx[p][lym][lx]) * rdy2;
-        ezn = apply_src(df_new(ezo, s4, cf, curl), 2, k, j, i, a, sfirst, scount);
-        ezn = ln.vEzAny ? ezn : zero;
-      }
-      sEz[ly][lx] = ezn;
-      if (ln.vEz) sto(ez, off, ezn);
-      // (d) Ex(k), Ey(k)
-      T exn, eyn;
-      {
-        const T sxk = (T)0.5 * (sS[p][lym][lx] + sS[p][ly][lx]);
-        const T sxm = (T)0.5 * (sS[q][lym][lx] + sS[q][ly][lx]);
-        const T ave = (T)0.5 * (sxk + sxm);
-        const T curl = (bzk - sBz[lym][lx]) * rdy2 - (by_k - by_m) * rdz2k;
-        exn = apply_src(df_new(exo, ave, cf, curl), 0, k, j, i, a, sfirst, scount);
-        exn = ln.vEx ? exn : zero;
-      }
-      {
-        const T s4 = (T)0.25 * (sS[p][ly][lxm] + sS[p][ly][lx] + sS[q][ly][lxm] + sS[q][ly][lx]);
-        const T curl = (bx_k - bx_m) * rdz2k - (bzk - sBz[ly][lxm]) * rdx2;
-        eyn = apply_src(df_new(eyo, s4, cf, curl), 1, k, j, i, a, sfirst, scount);
-        eyn = ln.vEy ? eyn : zero;
-      }
-      if (ln.vEx) sto(ex, off, exn);
-      if (ln.vEy) sto(ey, off, eyn);
-      __syncthreads();
-      // (e) Bx(k), By(k) from E_new(k), E_new(k+1)
-      const T bxn = bx_k + dtb * ((ey_n1 - eyn) * rdzk - (sEz[lyp][lx] - ezn) * rdy);
-      const T byn = by_k + dtb * ((sEz[ly][lxp] - ezn) * rdx - (ex_n1 - exn) * rdzk);
-      if (ln.vBx) sto(bx, off, bxn);
-      if (ln.vBy) sto(by, off, byn);
-      ex_n1 = exn;
-      ey_n1 = eyn;
-      off -= PSb;
+  // One level.  cur = level k, nxt = level k-1, dst = the free slot for the loads of level
+  // k-1-PF.  Source cells are not applied here (a separate pass patches them), which keeps the
+  // loop free of data-dependent branches, so the compiler's wait counts stay exact.
+  auto level = [&](const int k, const Lv<T> &cur, const Lv<T> &nxt, Lv<T> &dst) {
+    const int p = k & 1, q = p ^ 1;
+    const T dzk = sZ[k], rdzk = sZ[N + k], rdz2k = sZ[2 * N + k];
+    const T bx_k = cur.bx, by_k = cur.by, bx_m = nxt.bx, by_m = nxt.by;
+    // (a) Bz_old(k) = Bz_old(k+1) + dz(k) div_h(B_old(k))
+    bz.add(dzk * ((sBx[p][ly][lxp] - bx_k) * rdx + (sBy[p][lyp][lx] - by_k) * rdy));
+    const T bzk = bz.val();
+    sBz[ly][lx] = bzk;
+    // (b) level k-1 into LDS; loads of level k-1-PF
+    sBx[q][ly][lx] = bx_m;
+    sBy[q][ly][lx] = by_m;
+    sS[q][ly][lx] = nxt.s;
+    bx_last = bx_m;
+    by_last = by_m;
+    const T ezo = cur.ez, exo = cur.ex, eyo = cur.ey;
+    dst = load(k - 1 - PF);
+    __syncthreads();
+    // (c) Ez(k) on the tile and its +x/+y halo
+    T ezn;
+    {
+      const T s4 = (T)0.25 * (sS[p][lym][lxm] + sS[p][lym][lx] + sS[p][ly][lxm] + sS[p][ly][lx]);
+      const T curl = (by_k - sBy[p][ly][lxm]) * rdx2 - (bx_k - sBx[p][lym][lx]) * rdy2;
+      ezn = ln.vEzAny ? df_new(ezo, s4, cf, curl) : zero;
     }
+    sEz[ly][lx] = ezn;
+    bsto(rez, off, ln.vEz, ezn);
+    // (d) Ex(k), Ey(k)
+    T exn, eyn;
+    {
+      const T sxk = (T)0.5 * (sS[p][lym][lx] + sS[p][ly][lx]);
+      const T sxm = (T)0.5 * (sS[q][lym][lx] + sS[q][ly][lx]);
+      const T ave = (T)0.5 * (sxk + sxm);
+      const T curl = (bzk - sBz[lym][lx]) * rdy2 - (by_k - by_m) * rdz2k;
+      exn = ln.vEx ? df_new(exo, ave, cf, curl) : zero;
+    }
+    {
+      const T s4 = (T)0.25 * (sS[p][ly][lxm] + sS[p][ly][lx] + sS[q][ly][lxm] + sS[q][ly][lx]);
+      const T curl = (bx_k - bx_m) * rdz2k - (bzk - sBz[ly][lxm]) * rdx2;
+      eyn = ln.vEy ? df_new(eyo, s4, cf, curl) : zero;
+    }
+    bsto(rex, off, ln.vEx, exn);
+    bsto(rey, off, ln.vEy, eyn);
+    __syncthreads();
+    // (e) Bx(k), By(k) from E_new(k), E_new(k+1)
+    const T bxn = bx_k + dtb * ((ey_n1 - eyn) * rdzk - (sEz[lyp][lx] - ezn) * rdy);
+    const T byn = by_k + dtb * ((sEz[ly][lxp] - ezn) * rdx - (ex_n1 - exn) * rdzk);
+    bsto(rbx, off, ln.vBx, bxn);
+    bsto(rby, off, ln.vBy, byn);
+    ex_n1 = exn;
+    ey_n1 = eyn;
+    off -= PSb;
+  };
+  // whole rounds of S levels (the ring returns to its starting phase), then the remainder
+  int k = khi;
+  for (; k - (S - 1) >= klo; k -= S) {
+#pragma unroll
+    for (int u = 0; u < S; ++u) level(k - u, R[u], R[(u + 1) % S], R[(u + S - 1) % S]);
   }
+#pragma unroll
+  for (int u = 0; u < S - 1; ++u)
+    if (k - u >= klo) level(k - u, R[u], R[(u + 1) % S], R[(u + S - 1) % S]);
   if (a.carry == 2) {  // lower launch: hand the running sums of level klo to the top launch
     if (ln.own) bz.save(a.cs + po, a.cc + po, ln.colb);
     return;
@@ -399,6 +421,43 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_surface(const MarchArgs<T> 
   const T byn = by0 + dtb * ((sEz[ly][lxp] - ezn) * rdx - (ex1 - exn) * rdz0);
   if (ln.vBx) sto(bx, off, bxn);
   if (ln.vBy) sto(by, off, byn);
+}
+
+// Source cells of one step at levels 1 .. N-1, applied after the volume march (level 0 stays in
+// k_surface).  First the E cells take E - amp w (ElectricalFiled.cu:347-475; the weights of
+// coincident cells are summed in table order, as the per-cell loop of the reference does), then
+// every Bx/By cell whose curl reads one of them is recomputed from B_old and the patched E with the
+// march's own expression (Bderivate.cu:79-200).  One workgroup per slot; the host lists each cell
+// once, so no two threads touch one cell.
+template <typename T>
+__global__ void k_src_e(const MarchArgs<T> a, const SrcCell *cells, const int *first, const int *count) {
+  const int slot = blockIdx.x;
+  const long long fo = (long long)slot * a.fstride;
+  T *const f[3] = {a.ex + fo, a.ey + fo, a.ez + fo};
+  for (int c = threadIdx.x; c < count[slot]; c += blockDim.x) {
+    const SrcCell sc = cells[first[slot] + c];
+    T *e = f[sc.comp] + (long long)sc.k * a.PS + (long long)sc.j * a.PX + sc.i;
+    *e = (T)((double)*e - a.amp * sc.w);
+  }
+}
+template <typename T>
+__global__ void k_src_b(const MarchArgs<T> a, const SrcCell *cells, const int *first, const int *count) {
+  const int slot = blockIdx.x;
+  const long long fo = (long long)slot * a.fstride;
+  const T dtb = (T)a.dtb;
+  const long long PS = a.PS, PX = a.PX;
+  for (int c = threadIdx.x; c < count[slot]; c += blockDim.x) {
+    const SrcCell sc = cells[first[slot] + c];
+    const long long o = fo + (long long)sc.k * PS + (long long)sc.j * PX + sc.i;
+    const T rdzk = a.rdz[sc.k];
+    if (sc.comp == 3) {
+      const T rdy = a.rdy[sc.j + 1];
+      a.bx[o] = a.bxi[o] + dtb * ((a.ey[o + PS] - a.ey[o]) * rdzk - (a.ez[o + PX] - a.ez[o]) * rdy);
+    } else {
+      const T rdx = a.rdx[sc.i + 1];
+      a.by[o] = a.byi[o] + dtb * ((a.ez[o + 1] - a.ez[o]) * rdx - (a.ex[o + PS] - a.ex[o]) * rdzk);
+    }
+  }
 }
 
 // Derived Bz volume (N+1, M, L dense) of one slot (Bderivate.cu:202-265), for field read-back.
@@ -571,6 +630,15 @@ hipError_t launch_surface(const MarchArgs<T> &a, int nslot, hipStream_t s) {
   return launch_by<T>(a, nslot, s, true);
 }
 template <typename T>
+hipError_t launch_src_fix(const MarchArgs<T> &a, const SrcCell *ecells, const int *eidx, const SrcCell *bcells,
+                          const int *bidx, int nslot, hipStream_t s) {
+  hipLaunchKernelGGL((k_src_e<T>), dim3(nslot), dim3(64), 0, s, a, ecells, eidx, eidx + nslot);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_src_b<T>), dim3(nslot), dim3(64), 0, s, a, bcells, bidx, bidx + nslot);
+  return hipGetLastError();
+}
+template <typename T>
 hipError_t launch_bz_volume(const MarchArgs<T> &a, T *bz, int slot, hipStream_t s) {
   dim3 blk(64), grd((a.L + 63) / 64, a.M);
   hipLaunchKernelGGL((k_bz_volume<T>), grd, blk, 0, s, a, bz, slot);
@@ -597,6 +665,10 @@ template hipError_t launch_march<float>(const MarchArgs<float> &, int, hipStream
 template hipError_t launch_march<double>(const MarchArgs<double> &, int, hipStream_t);
 template hipError_t launch_surface<float>(const MarchArgs<float> &, int, hipStream_t);
 template hipError_t launch_surface<double>(const MarchArgs<double> &, int, hipStream_t);
+template hipError_t launch_src_fix<float>(const MarchArgs<float> &, const SrcCell *, const int *, const SrcCell *,
+                                          const int *, int, hipStream_t);
+template hipError_t launch_src_fix<double>(const MarchArgs<double> &, const SrcCell *, const int *, const SrcCell *,
+                                           const int *, int, hipStream_t);
 template hipError_t launch_bz_volume<float>(const MarchArgs<float> &, float *, int, hipStream_t);
 template hipError_t launch_bz_volume<double>(const MarchArgs<double> &, double *, int, hipStream_t);
 template hipError_t launch_gather<float>(const GatherArgs &, int, int, const MarchArgs<float> &, hipStream_t);
