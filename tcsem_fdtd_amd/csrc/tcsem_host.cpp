@@ -393,7 +393,9 @@ struct tcsem_ctx {
   std::vector<double> cond;
   int PX = 0, PY = 0;
   long long PS = 0, fstride = 0, pstride = 0, wstride = 0;
-  int by = 8, pf = 2, ntx = 0, nty = 0, nper = 0;
+  int by = 8, pf = 2, ntx = 0, nty = 0, nper = 0;  // march tiling
+  int stx = 0, sty = 0, sper = 0;                   // surface tiling
+  int xm = 1;                                       // x margin of the layout in elements
   int nslots = 0;
   // stream: the volume march and every host-ordered operation; stream2: upward continuation,
   // surface level and receiver gathers of a step, overlapping the lower march of the next step
@@ -456,10 +458,10 @@ namespace {
 
 // Workgroup height of the volume march.  Tall tiles waste fewer lanes on the +-1 halo rows but
 // give fewer workgroups: 16 rows (14 owned) once the batch offers at least two workgroups per CU,
-// else 8 rows (6 owned), which measured best for one 256 x 256 model on MI355X (4-16 rows swept).
-// TCSEM_MARCH_ROWS overrides the choice (4, 8 or 16).  The prefetch depth (levels of loads in
-// flight per lane) follows the same sweep: 1 for 16-row tiles (enough waves already), else 2 for
-// FP32 and 3 for FP64 fields; TCSEM_MARCH_PF (1 .. 4) overrides it.
+// else 4 rows (2 owned), which measured best for one 256 x 256 model on MI355X (4-16 rows swept);
+// FP64 fields stay at 8 rows or fewer (16 rows would spill registers).  TCSEM_MARCH_ROWS
+// overrides the choice (4, 8 or 16).  The prefetch depth (levels of loads in flight per lane) is
+// 2 for FP32 fields in short tiles and 1 otherwise; TCSEM_MARCH_PF (1 .. 4) overrides it.
 void choose_tiling(tcsem_ctx *c, int nbatch) {
   const int L = c->g.info.L, M = c->g.info.M;
   int cu = 256;
@@ -467,30 +469,34 @@ void choose_tiling(tcsem_ctx *c, int nbatch) {
   hipDeviceProp_t prop;
   if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
     cu = prop.multiProcessorCount;
-  auto tiles = [&](int b) {
-    return (long long)((L + 1 + kMarchOwnX - 1) / kMarchOwnX) * ((M + 1 + (b - 2) - 1) / (b - 2));
-  };
-  int by = tiles(16) * nbatch >= 2LL * cu ? 16 : 8;
+  auto tiles = [&](int b) { return (long long)((L + kMarchLanes - 1) / kMarchLanes) * ((M + (b - 2) - 1) / (b - 2)); };
+  int by = tiles(16) * nbatch >= 2LL * cu ? 16 : 4;
   if (const char *e = std::getenv("TCSEM_MARCH_ROWS")) {
     const int v = std::atoi(e);
     if (v == 4 || v == 8 || v == 16) by = v;
   }
-  int pf = by == 16 ? 1 : (c->prec == 64 ? 3 : 2);
+  if (c->prec == 64 && by > 8) by = 8;  // FP64 16-row tiles do not fit their registers
+  int pf = (by == 16 || c->prec == 64) ? 1 : 2;
   if (const char *e = std::getenv("TCSEM_MARCH_PF")) {
     const int v = std::atoi(e);
     if (v >= 1 && v <= 4) pf = v;
   }
   c->pf = pf;
   c->by = by;
-  c->ntx = (L + 1 + kMarchOwnX - 1) / kMarchOwnX;
-  c->nty = (M + 1 + (by - 2) - 1) / (by - 2);
+  // march: 64-column blocks over columns 0..L-1 and rows 0..M-1 (column L and row M hold no
+  // updated component); surface: 62 owned columns over 0..L, rows 0..M
+  c->ntx = (L + kMarchLanes - 1) / kMarchLanes;
+  c->nty = (M + (by - 2) - 1) / (by - 2);
   c->nper = (c->ntx * c->nty + 7) / 8;
+  c->stx = (L + 1 + kSurfaceOwnX - 1) / kSurfaceOwnX;
+  c->sty = (M + 1 + (kSurfaceRows - 2) - 1) / (kSurfaceRows - 2);
+  c->sper = (c->stx * c->sty + 7) / 8;
 }
 
 template <typename T>
 MarchArgs<T> march_args(tcsem_ctx *c) {
   MarchArgs<T> a{};
-  const long long org = c->PX + 1;
+  const long long org = c->PX + c->xm;
   const long long fs = c->fstride * c->nslots;
   T *fi = (T *)c->d_fields + (long long)c->cur * 5 * fs;
   T *fo = (T *)c->d_fields + (long long)(1 - c->cur) * 5 * fs;
@@ -522,11 +528,15 @@ MarchArgs<T> march_args(tcsem_ctx *c) {
   a.M = c->g.info.M;
   a.N = c->g.info.N;
   a.PX = c->PX;
+  a.xm = c->xm;
   a.rows = c->by;
   a.pf = c->pf;
   a.ntx = c->ntx;
   a.nty = c->nty;
   a.nper = c->nper;
+  a.stx = c->stx;
+  a.sty = c->sty;
+  a.sper = c->sper;
   const T *rd = (const T *)c->d_rd;
   const int L = a.L, M = a.M, N = a.N;
   a.rdx = rd;
@@ -547,7 +557,7 @@ MarchArgs<T> march_args(tcsem_ctx *c) {
 template <typename T>
 UpcontArgs<T> upcont_args(tcsem_ctx *c) {
   UpcontArgs<T> u{};
-  const long long org = c->PX + 1;
+  const long long org = c->PX + c->xm;
   T *pl = (T *)c->d_planes;
   const long long ps = c->pstride * c->nslots;
   u.bz0 = pl + org;
@@ -971,7 +981,7 @@ int do_step(tcsem_ctx *c) {
   if (c->have_rx) {
     const double t0 = c->toT + c->dt1;
     GatherArgs gA{};
-    const long long org = c->PX + 1;
+    const long long org = c->PX + c->xm;
     const long long fs = c->fstride * c->nslots;
     const T *f = (const T *)c->d_fields + (long long)c->cur * 5 * fs;
     for (int q = 0; q < 5; ++q) gA.f[q] = f + q * fs + org;
@@ -1332,7 +1342,8 @@ int tcsem_create(const tcsem_setup *s, const double *cond, int precision, tcsem_
       delete c;
       return fail(TCSEM_EINVAL, "conductivity must be finite and >= 0 (cell %zu)", q);
     }
-  c->PX = ((L + 3) + 63) / 64 * 64;
+  c->xm = kLineBytes / (int)c->esz;
+  c->PX = ((L + 2 + c->xm) + 63) / 64 * 64;
   c->PY = M + 3;
   c->PS = (long long)c->PX * c->PY;
   c->fstride = (long long)(N + 1) * c->PS;
@@ -1430,7 +1441,7 @@ int tcsem_create(const tcsem_setup *s, const double *cond, int precision, tcsem_
     for (int k = 0; k < N; ++k)
       for (int j = 0; j < M; ++j)
         for (int i = 0; i < L; ++i)
-          pad[(size_t)k * c->PS + (size_t)(j + 1) * c->PX + (i + 1)] = c->cond[((size_t)k * M + j) * L + i];
+          pad[(size_t)k * c->PS + (size_t)(j + 1) * c->PX + (i + c->xm)] = c->cond[((size_t)k * M + j) * L + i];
     if (hipMalloc(&c->d_sig, pad.size() * c->esz) != hipSuccess) return bail(fail(TCSEM_ENOMEM, "sigma allocation"));
     if ((rc = upload_double_as(c, c->d_sig, pad))) return bail(rc);
   }
@@ -1882,7 +1893,7 @@ int tcsem_get_field(tcsem_ctx *c, int slot, int comp, double *out) {
   for (int k = 0; k < nk; ++k)
     for (int j = 0; j < nj; ++j)
       for (int i = 0; i < ni; ++i) {
-        const size_t src = (size_t)k * c->PS + (size_t)(j + 1) * c->PX + (i + 1);
+        const size_t src = (size_t)k * c->PS + (size_t)(j + 1) * c->PX + (i + c->xm);
         out[((size_t)k * nj + j) * ni + i] = c->prec == 64 ? ((double *)h.data())[src] : ((float *)h.data())[src];
       }
   return TCSEM_OK;
@@ -1915,7 +1926,7 @@ int tcsem_set_field(tcsem_ctx *c, int slot, int comp, const double *in) {
   for (int k = 0; k < nk; ++k)
     for (int j = 0; j < nj; ++j)
       for (int i = 0; i < ni; ++i) {
-        const size_t dst = (size_t)k * c->PS + (size_t)(j + 1) * c->PX + (i + 1);
+        const size_t dst = (size_t)k * c->PS + (size_t)(j + 1) * c->PX + (i + c->xm);
         const double v = in[((size_t)k * nj + j) * ni + i];
         if (c->prec == 64) ((double *)h.data())[dst] = v;
         else ((float *)h.data())[dst] = (float)v;
@@ -1931,7 +1942,7 @@ int tcsem_upcont_plane(tcsem_ctx *c, const double *bz0, double *bxair, double *b
   std::vector<char> h((size_t)c->pstride * c->esz, 0);
   for (int j = 0; j < M; ++j)
     for (int i = 0; i < L; ++i) {
-      const size_t dst = (size_t)(j + 1) * c->PX + (i + 1);
+      const size_t dst = (size_t)(j + 1) * c->PX + (i + c->xm);
       if (c->prec == 64) ((double *)h.data())[dst] = bz0[(size_t)j * L + i];
       else ((float *)h.data())[dst] = (float)bz0[(size_t)j * L + i];
     }

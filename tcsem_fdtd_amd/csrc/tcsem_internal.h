@@ -1,11 +1,12 @@
 // tcsem_internal.h -- types shared by the host runtime (tcsem_host.cpp) and the gfx950 kernels.
 //
-// Field layout in HBM (per field, per batch slot): a zero margin ring around a pitched
-// (k, j, i) volume, x fastest.  Element (k, j, i) lives at origin[k*PS + j*PX + i] with
-// origin = allocation + PX + 1, so the halo indices i = -1 and j = -1 are addressable and
-// read as zero.  PX >= L + 3 (rounded to 64 elements), PY = M + 3 rows per level,
-// PS = PX * PY; every field has N + 1 levels.  The conductivity uses the same layout (N levels,
-// shared by all slots).  The surface planes (Bz0, BxAir, ByAir) use one level of the same
+// Field layout in HBM (per field, per batch slot): a zero margin around a pitched (k, j, i)
+// volume, x fastest.  Element (k, j, i) lives at origin[k*PS + j*PX + i] with
+// origin = allocation + PX + XM, XM = 128 bytes of elements, so column 0 of every row starts a
+// 128-byte line (whole-line stores from the march) and the halo indices i = -1 and j = -1 are
+// addressable and read as zero.  PX >= L + 2 + XM (rounded to 64 elements), PY = M + 3 rows per
+// level, PS = PX * PY; every field has N + 1 levels.  The conductivity uses the same layout (N
+// levels, shared by all slots).  The surface planes (Bz0, BxAir, ByAir) use one level of the same
 // pitch.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -17,10 +18,15 @@ constexpr double kPi = 3.141592653589793238;  // FTDT.h:39
 constexpr double kMu0 = 4 * kPi * 1.e-7;      // main.cu:55
 constexpr int kNPad = 4;                      // FTDT.h:40
 
-// Volume march tile: 64 lanes in x (62 owned columns + one halo column each side) by BY rows in
-// y (BY - 2 owned rows + one halo row each side); BY is chosen per grid and batch size.
+constexpr int kLineBytes = 128;               // x margin of the layout: one cache line
+
+// Volume march tile: 64 aligned columns (no halo lanes in x; two edge lanes fetch the
+// neighbouring columns) by BY rows in y (BY - 2 owned rows + one halo row each side); BY is chosen
+// per grid and batch size.  The surface kernel tiles 64 lanes as 62 owned columns + one halo
+// column each side by kSurfaceRows rows.
 constexpr int kMarchLanes = 64;
-constexpr int kMarchOwnX = kMarchLanes - 2;
+constexpr int kSurfaceOwnX = kMarchLanes - 2;
+constexpr int kSurfaceRows = 8;
 
 // One injected E cell: E[comp](k, j, i) -= amp * w each source step (ElectricalFiled.cu:347-475).
 // The same record lists the B cells a source step recomputes (comp 3 Bx, 4 By; w unused).
@@ -43,9 +49,11 @@ struct MarchArgs {
   long long pstride;          // elements between slots of one plane
   long long PS;               // plane stride of a volume
   int L, M, N, PX;
-  int rows;                   // workgroup rows (tile height + 2): 4, 8 or 16
+  int xm;                     // x margin in elements (origin = allocation + PX + xm)
+  int rows;                   // workgroup rows of the march (tile height + 2): 4, 8 or 16
   int pf;                     // march prefetch depth in levels: 1 .. 4
-  int ntx, nty, nper;         // tiles in x, y; tiles per XCD group
+  int ntx, nty, nper;         // march tiles in x, y; tiles per XCD group
+  int stx, sty, sper;         // the same for the surface kernel
   const T *rdx, *rdx2, *rdy, *rdy2;  // 1/dx[i], 1/(dx[i-1]+dx[i]), same for y (index+1 origin)
   const T *dzt, *rdz, *rdz2;         // dz[k], 1/dz[k], 1/(dz[k-1]+dz[k]) (rdz2[0] = 1/(2 dz[0]))
   double lam, dt, dtb, amp;          // step scalars (FP64, main.cu:223-260)

@@ -180,7 +180,7 @@ __device__ inline void lane_setup(const MarchArgs<T> &a, const Tile &t, Lane<T> 
   ln.i = i;
   ln.j = j;
   const int ic = min(i, L + 1), jc = min(j, M + 1);
-  ln.colb = (unsigned)((jc + 1) * a.PX + (ic + 1)) * (unsigned)sizeof(T);
+  ln.colb = (unsigned)((jc + 1) * a.PX + (ic + a.xm)) * (unsigned)sizeof(T);
   ln.own = t.own && i <= L && j <= M;
   ln.vEx = ln.own && i < L && j >= 1 && j < M;
   ln.vEy = ln.own && i >= 1 && i < L && j < M;
@@ -198,24 +198,55 @@ __device__ inline void lane_setup(const MarchArgs<T> &a, const Tile &t, Lane<T> 
   ln.lyp = min(t.ly + 1, BY - 1);
 }
 
-// One level of one column: old B, sigma and old E.
+// One level of one column: old B, sigma and old E, and the same fields of the halo column the
+// lane fetches (lane 0: column i0-1, lane 63: column i0+64, other lanes: their own column again).
 template <typename T>
 struct Lv {
   T bx, by, s, ex, ey, ez;
+  T hbx, hby, hs, hez;
 };
 
+// Volume march, levels k_hi .. k_lo of every column (tiling and per-level steps in the header
+// comment of this file).  A workgroup is one 128-byte-aligned block of 64 columns (no halo lanes
+// in x) by BY rows (BY-2 owned rows and one halo row on each side).  The x-neighbours a block
+// needs come from two edge lanes: lane 0 keeps a second running Bz sum for column i0-1 and
+// fetches its sigma and By; lane 63 fetches column i0+64 and computes its new Ez.  They go to the
+// halo slots 0 and 65 of the LDS rows.
 template <typename T, int BY, int PF>
 __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> a) {
   extern __shared__ __align__(16) unsigned char zdyn[];  // 3 N entries of T
-  __shared__ T sBx[2][BY][kMarchLanes], sBy[2][BY][kMarchLanes], sS[2][BY][kMarchLanes];
-  __shared__ T sBz[BY][kMarchLanes], sEz[BY][kMarchLanes];
-  Tile t;
-  if (!tile_of<BY>(a.ntx, a.nty, a.nper, t)) return;
-  Lane<T> ln;
-  lane_setup<T, BY>(a, t, ln);
-  const int lx = t.lx, ly = t.ly, i = ln.i, j = ln.j;
+  constexpr int LW = kMarchLanes + 2;                   // LDS row: halo, 64 lanes, halo
+  __shared__ T sBx[2][BY][LW], sBy[2][BY][LW], sS[2][BY][LW];
+  __shared__ T sBz[BY][LW], sEz[BY][LW];
+  const int b = blockIdx.x;
+  const int tile = (b & 7) * a.nper + (b >> 3);  // XCD-contiguous bands of tiles
+  if (tile >= a.ntx * a.nty) return;
+  const int tx = tile % a.ntx, ty = tile / a.ntx;
+  const int lx = threadIdx.x, ly = threadIdx.y;
+  const int L = a.L, M = a.M, N = a.N;
+  const int i0 = tx * kMarchLanes, i = i0 + lx, iR = i0 + kMarchLanes;
+  const int j = ty * (BY - 2) - 1 + ly;
+  const bool own = ly >= 1 && ly <= BY - 2 && i < L && j >= 0 && j < M;
+  const bool vEx = own && j >= 1;
+  const bool vEy = own && i >= 1;
+  const bool vEzAny = ly >= 1 && i >= 1 && i < L && j >= 1 && j < M;  // tile rows and the +y halo row
+  const bool vEz = own && vEzAny;
+  const bool vBx = own && i >= 1;
+  const bool vBy = own && j >= 1;
+  const bool vEzR = ly >= 1 && iR < L && j >= 1 && j < M;  // column i0+64 (used by lane 63)
+  const int ic = min(i, L + 1), jc = min(max(j, -1), M + 1);
+  const int ih = lx == 0 ? i0 - 1 : (lx == kMarchLanes - 1 ? min(iR, L + 1) : ic);
+  const unsigned rowe = (unsigned)((jc + 1) * a.PX + a.xm);  // element of column 0 in this row
+  const unsigned colb = (rowe + ic) * (unsigned)sizeof(T);
+  const unsigned hcolb = (unsigned)((int)rowe + ih) * (unsigned)sizeof(T);
+  const int sl = lx + 1, slm = lx, slp = lx + 2;
+  const int hsl = lx == 0 ? 0 : (lx == kMarchLanes - 1 ? LW - 1 : sl);
+  const bool left = lx == 0, right = lx == kMarchLanes - 1;
+  const int lym = max(ly - 1, 0), lyp = min(ly + 1, BY - 1);
+  const T rdx = a.rdx[ic + 1], rdx2 = a.rdx2[ic + 1], rdy = a.rdy[jc + 1], rdy2 = a.rdy2[jc + 1];
+  const T rdxL = a.rdx[max(i0 - 1, -1) + 1], rdx2R = a.rdx2[min(iR, L + 1) + 1];
   const int slot = blockIdx.y;
-  const long long fo = (long long)slot * a.fstride - (a.PX + 1);  // allocation base of the slot
+  const long long fo = (long long)slot * a.fstride - (a.PX + a.xm);  // allocation base of the slot
   const T *__restrict__ exi = a.exi + fo;
   const T *__restrict__ eyi = a.eyi + fo;
   const T *__restrict__ ezi = a.ezi + fo;
@@ -226,15 +257,13 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> a)
   T *__restrict__ ez = a.ez + fo;
   T *__restrict__ bx = a.bx + fo;
   T *__restrict__ by = a.by + fo;
-  const T *__restrict__ sig = a.sig - (a.PX + 1);
+  const T *__restrict__ sig = a.sig - (a.PX + a.xm);
   const unsigned PSb = (unsigned)a.PS * (unsigned)sizeof(T);
   const unsigned fbytes = (unsigned)(a.fstride * (long long)sizeof(T));
   const __amdgpu_buffer_rsrc_t rex = rsrc_of(ex, fbytes), rey = rsrc_of(ey, fbytes), rez = rsrc_of(ez, fbytes);
   const __amdgpu_buffer_rsrc_t rbx = rsrc_of(bx, fbytes), rby = rsrc_of(by, fbytes);
   const auto cf = make_coef<T>(a.lam, a.dt);
   const T dtb = (T)a.dtb, zero = 0;
-  const T rdx = ln.rdx, rdx2 = ln.rdx2, rdy = ln.rdy, rdy2 = ln.rdy2;
-  const int lxm = ln.lxm, lxp = ln.lxp, lym = ln.lym, lyp = ln.lyp;
 
   // A ring of PF+2 register slots holds level k (cur), level k-1 (nxt) and the PF-1 levels below
   // it; each level issues the loads of level k-1-PF into the slot cur held one level earlier, so
@@ -243,9 +272,10 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> a)
   // below k_lo-1 are never read (the top launch of a step owns them).
   constexpr int S = PF + 2;
   const int khi = a.k_hi, klo = a.k_lo, lfloor = klo - 1;
-  const long long po = (long long)slot * a.pstride - (a.PX + 1);
+  const long long po = (long long)slot * a.pstride - (a.PX + a.xm);
   auto load = [&](int lv) {
-    const unsigned o = (unsigned)max(lv, lfloor) * PSb + ln.colb;
+    const unsigned lb = (unsigned)max(lv, lfloor) * PSb;
+    const unsigned o = lb + colb, h = lb + hcolb;
     Lv<T> v;
     v.bx = ldo(bxi, o);
     v.by = ldo(byi, o);
@@ -253,27 +283,40 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> a)
     v.ex = ldo(exi, o);
     v.ey = ldo(eyi, o);
     v.ez = ldo(ezi, o);
+    v.hbx = ldo(bxi, h);
+    v.hby = ldo(byi, h);
+    v.hs = ldo(sig, h);
+    v.hez = ldo(ezi, h);
     return v;
   };
-  unsigned off = (unsigned)khi * PSb + ln.colb;
+  unsigned off = (unsigned)khi * PSb + colb;
   Lv<T> R[S];
 #pragma unroll
   for (int d = 0; d < S - 1; ++d) R[d] = load(khi - d);
   // dz, 1/dz, 1/(dz[k-1]+dz[k]) of every level in LDS: per-level reads then wait on the LDS
   // counter only, never on the in-order vector-memory counter that the prefetch loads share
   T *sZ = (T *)zdyn;
-  const int N = a.N;
   for (int q = ly * kMarchLanes + lx; q < 3 * N; q += kMarchLanes * BY) sZ[q] = a.dzt[q];
-  sBx[khi & 1][ly][lx] = R[0].bx;
-  sBy[khi & 1][ly][lx] = R[0].by;
-  sS[khi & 1][ly][lx] = R[0].s;
-  Acc<T> bz;
+  // level k_hi into the LDS buffer of its parity; the edge lanes fill the halo slots (the other
+  // lanes store their own value again, so the stores need no branch)
+  auto stage = [&](const int pb, const Lv<T> &v) {
+    sBx[pb][ly][sl] = v.bx;
+    sBy[pb][ly][sl] = v.by;
+    sS[pb][ly][sl] = v.s;
+    sBx[pb][ly][hsl] = right ? v.hbx : v.bx;
+    sBy[pb][ly][hsl] = left ? v.hby : v.by;
+    sS[pb][ly][hsl] = (left || right) ? v.hs : v.s;
+  };
+  stage(khi & 1, R[0]);
+  Acc<T> bz, bzL;  // running Bz of the lane's column and (lane 0) of column i0-1
   bz.init();
+  bzL.init();
   T ex_n1 = 0, ey_n1 = 0;  // E_new at level k+1 (zero at the bottom wall, level N)
   if (a.carry == 1) {      // resume: Bz sums of level khi+1 and E_new(khi+1) of the lower launch
-    bz.load(a.cs + po, a.cc + po, ln.colb);
-    ex_n1 = ln.vEx ? ldo((const T *)ex, off + PSb) : zero;
-    ey_n1 = ln.vEy ? ldo((const T *)ey, off + PSb) : zero;
+    bz.load(a.cs + po, a.cc + po, colb);
+    bzL.load(a.cs + po, a.cc + po, hcolb);
+    ex_n1 = vEx ? ldo((const T *)ex, off + PSb) : zero;
+    ey_n1 = vEy ? ldo((const T *)ey, off + PSb) : zero;
   }
   T bx_last = 0, by_last = 0;  // old Bx, By of level k_lo-1 after the loop
   __syncthreads();
@@ -288,50 +331,56 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> a)
     const int p = k & 1, q = p ^ 1;
     const T dzk = sZ[k], rdzk = sZ[N + k], rdz2k = sZ[2 * N + k];
     const T bx_k = cur.bx, by_k = cur.by, bx_m = nxt.bx, by_m = nxt.by;
-    // (a) Bz_old(k) = Bz_old(k+1) + dz(k) div_h(B_old(k))
-    bz.add(dzk * ((sBx[p][ly][lxp] - bx_k) * rdx + (sBy[p][lyp][lx] - by_k) * rdy));
+    // (a) Bz_old(k) = Bz_old(k+1) + dz(k) div_h(B_old(k)), for the lane's column and column i0-1
+    bz.add(dzk * ((sBx[p][ly][slp] - bx_k) * rdx + (sBy[p][lyp][sl] - by_k) * rdy));
+    bzL.add(dzk * ((bx_k - cur.hbx) * rdxL + (sBy[p][lyp][0] - cur.hby) * rdy));
     const T bzk = bz.val();
-    sBz[ly][lx] = bzk;
+    sBz[ly][sl] = bzk;
+    sBz[ly][left ? 0 : sl] = left ? bzL.val() : bzk;
     // (b) level k-1 into LDS; loads of level k-1-PF
-    sBx[q][ly][lx] = bx_m;
-    sBy[q][ly][lx] = by_m;
-    sS[q][ly][lx] = nxt.s;
+    stage(q, nxt);
     bx_last = bx_m;
     by_last = by_m;
     const T ezo = cur.ez, exo = cur.ex, eyo = cur.ey;
     dst = load(k - 1 - PF);
     __syncthreads();
-    // (c) Ez(k) on the tile and its +x/+y halo
-    T ezn;
+    // (c) Ez(k) on the tile rows and the +y halo row; lane 63 also for column i0+64
+    T ezn, ezr;
     {
-      const T s4 = (T)0.25 * (sS[p][lym][lxm] + sS[p][lym][lx] + sS[p][ly][lxm] + sS[p][ly][lx]);
-      const T curl = (by_k - sBy[p][ly][lxm]) * rdx2 - (bx_k - sBx[p][lym][lx]) * rdy2;
-      ezn = ln.vEzAny ? df_new(ezo, s4, cf, curl) : zero;
+      const T s4 = (T)0.25 * (sS[p][lym][slm] + sS[p][lym][sl] + sS[p][ly][slm] + sS[p][ly][sl]);
+      const T curl = (by_k - sBy[p][ly][slm]) * rdx2 - (bx_k - sBx[p][lym][sl]) * rdy2;
+      ezn = vEzAny ? df_new(ezo, s4, cf, curl) : zero;
     }
-    sEz[ly][lx] = ezn;
-    bsto(rez, off, ln.vEz, ezn);
+    {
+      const T s4 = (T)0.25 * (sS[p][lym][LW - 2] + sS[p][lym][LW - 1] + sS[p][ly][LW - 2] + sS[p][ly][LW - 1]);
+      const T curl = (cur.hby - by_k) * rdx2R - (cur.hbx - sBx[p][lym][LW - 1]) * rdy2;
+      ezr = vEzR ? df_new(cur.hez, s4, cf, curl) : zero;
+    }
+    sEz[ly][sl] = ezn;
+    sEz[ly][right ? LW - 1 : sl] = right ? ezr : ezn;
+    bsto(rez, off, vEz, ezn);
     // (d) Ex(k), Ey(k)
     T exn, eyn;
     {
-      const T sxk = (T)0.5 * (sS[p][lym][lx] + sS[p][ly][lx]);
-      const T sxm = (T)0.5 * (sS[q][lym][lx] + sS[q][ly][lx]);
+      const T sxk = (T)0.5 * (sS[p][lym][sl] + sS[p][ly][sl]);
+      const T sxm = (T)0.5 * (sS[q][lym][sl] + sS[q][ly][sl]);
       const T ave = (T)0.5 * (sxk + sxm);
-      const T curl = (bzk - sBz[lym][lx]) * rdy2 - (by_k - by_m) * rdz2k;
-      exn = ln.vEx ? df_new(exo, ave, cf, curl) : zero;
+      const T curl = (bzk - sBz[lym][sl]) * rdy2 - (by_k - by_m) * rdz2k;
+      exn = vEx ? df_new(exo, ave, cf, curl) : zero;
     }
     {
-      const T s4 = (T)0.25 * (sS[p][ly][lxm] + sS[p][ly][lx] + sS[q][ly][lxm] + sS[q][ly][lx]);
-      const T curl = (bx_k - bx_m) * rdz2k - (bzk - sBz[ly][lxm]) * rdx2;
-      eyn = ln.vEy ? df_new(eyo, s4, cf, curl) : zero;
+      const T s4 = (T)0.25 * (sS[p][ly][slm] + sS[p][ly][sl] + sS[q][ly][slm] + sS[q][ly][sl]);
+      const T curl = (bx_k - bx_m) * rdz2k - (bzk - sBz[ly][slm]) * rdx2;
+      eyn = vEy ? df_new(eyo, s4, cf, curl) : zero;
     }
-    bsto(rex, off, ln.vEx, exn);
-    bsto(rey, off, ln.vEy, eyn);
+    bsto(rex, off, vEx, exn);
+    bsto(rey, off, vEy, eyn);
     __syncthreads();
     // (e) Bx(k), By(k) from E_new(k), E_new(k+1)
-    const T bxn = bx_k + dtb * ((ey_n1 - eyn) * rdzk - (sEz[lyp][lx] - ezn) * rdy);
-    const T byn = by_k + dtb * ((sEz[ly][lxp] - ezn) * rdx - (ex_n1 - exn) * rdzk);
-    bsto(rbx, off, ln.vBx, bxn);
-    bsto(rby, off, ln.vBy, byn);
+    const T bxn = bx_k + dtb * ((ey_n1 - eyn) * rdzk - (sEz[lyp][sl] - ezn) * rdy);
+    const T byn = by_k + dtb * ((sEz[ly][slp] - ezn) * rdx - (ex_n1 - exn) * rdzk);
+    bsto(rbx, off, vBx, bxn);
+    bsto(rby, off, vBy, byn);
     ex_n1 = exn;
     ey_n1 = eyn;
     off -= PSb;
@@ -346,12 +395,12 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> a)
   for (int u = 0; u < S - 1; ++u)
     if (k - u >= klo) level(k - u, R[u], R[(u + 1) % S], R[(u + S - 1) % S]);
   if (a.carry == 2) {  // lower launch: hand the running sums of level klo to the top launch
-    if (ln.own) bz.save(a.cs + po, a.cc + po, ln.colb);
+    if (own) bz.save(a.cs + po, a.cc + po, colb);
     return;
   }
   // Bz_old(0) for the upward continuation and the surface level (level 0 sits in buffer 0)
-  bz.add(sZ[0] * ((sBx[0][ly][lxp] - bx_last) * rdx + (sBy[0][lyp][lx] - by_last) * rdy));
-  if (ln.own && i >= 0 && i < a.L && j >= 0 && j < a.M) sto(a.bz0 + po, ln.colb, bz.val());
+  bz.add(sZ[0] * ((sBx[0][ly][slp] - bx_last) * rdx + (sBy[0][lyp][sl] - by_last) * rdy));
+  if (own) sto(a.bz0 + po, colb, bz.val());
 }
 
 // Level 0: the air cells count as sigma = 0 and the level above is BxAir/ByAir at -dz0/2
@@ -361,21 +410,21 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_surface(const MarchArgs<T> 
   __shared__ T sBx[BY][kMarchLanes], sBy[BY][kMarchLanes], sS[BY][kMarchLanes];
   __shared__ T sBz[BY][kMarchLanes], sEz[BY][kMarchLanes];
   Tile t;
-  if (!tile_of<BY>(a.ntx, a.nty, a.nper, t)) return;
+  if (!tile_of<BY>(a.stx, a.sty, a.sper, t)) return;
   Lane<T> ln;
   lane_setup<T, BY>(a, t, ln);
   const int lx = t.lx, ly = t.ly, i = ln.i, j = ln.j;
   const int L = a.L, M = a.M;
   const int slot = blockIdx.y;
-  const long long fo = (long long)slot * a.fstride - (a.PX + 1);
-  const long long po = (long long)slot * a.pstride - (a.PX + 1);
+  const long long fo = (long long)slot * a.fstride - (a.PX + a.xm);
+  const long long po = (long long)slot * a.pstride - (a.PX + a.xm);
   const unsigned off = ln.colb, PSb = (unsigned)a.PS * (unsigned)sizeof(T);
   T *__restrict__ ex = a.ex + fo;
   T *__restrict__ ey = a.ey + fo;
   T *__restrict__ ez = a.ez + fo;
   T *__restrict__ bx = a.bx + fo;
   T *__restrict__ by = a.by + fo;
-  const T bx0 = ldo(a.bxi + fo, off), by0 = ldo(a.byi + fo, off), s0 = ldo(a.sig - (a.PX + 1), off);
+  const T bx0 = ldo(a.bxi + fo, off), by0 = ldo(a.byi + fo, off), s0 = ldo(a.sig - (a.PX + a.xm), off);
   const bool bzv = t.need_bz && i >= 0 && i < L && j >= 0 && j < M;
   const T bzc = bzv ? ldo(a.bz0 + po, off) : (T)0;
   const T exo = ldo(a.exi + fo, off), eyo = ldo(a.eyi + fo, off), ezo = ldo(a.ezi + fo, off);
@@ -598,12 +647,8 @@ hipError_t launch_march_pf(const MarchArgs<T> &a, dim3 grd, dim3 blk, hipStream_
   return hipGetLastError();
 }
 template <typename T, int BY>
-hipError_t launch_march_by(const MarchArgs<T> &a, int nslot, hipStream_t s, bool surface) {
+hipError_t launch_march_by(const MarchArgs<T> &a, int nslot, hipStream_t s) {
   dim3 blk(kMarchLanes, BY), grd(8 * a.nper, nslot);
-  if (surface) {
-    hipLaunchKernelGGL((k_surface<T, BY>), grd, blk, 0, s, a);
-    return hipGetLastError();
-  }
   switch (a.pf) {
     case 1: return launch_march_pf<T, BY, 1>(a, grd, blk, s);
     case 2: return launch_march_pf<T, BY, 2>(a, grd, blk, s);
@@ -613,21 +658,19 @@ hipError_t launch_march_by(const MarchArgs<T> &a, int nslot, hipStream_t s, bool
   return hipErrorInvalidValue;
 }
 template <typename T>
-hipError_t launch_by(const MarchArgs<T> &a, int nslot, hipStream_t s, bool surface) {
+hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s) {
   switch (a.rows) {
-    case 4: return launch_march_by<T, 4>(a, nslot, s, surface);
-    case 8: return launch_march_by<T, 8>(a, nslot, s, surface);
-    case 16: return launch_march_by<T, 16>(a, nslot, s, surface);
+    case 4: return launch_march_by<T, 4>(a, nslot, s);
+    case 8: return launch_march_by<T, 8>(a, nslot, s);
+    case 16: return launch_march_by<T, 16>(a, nslot, s);
   }
   return hipErrorInvalidValue;
 }
 template <typename T>
-hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s) {
-  return launch_by<T>(a, nslot, s, false);
-}
-template <typename T>
 hipError_t launch_surface(const MarchArgs<T> &a, int nslot, hipStream_t s) {
-  return launch_by<T>(a, nslot, s, true);
+  dim3 blk(kMarchLanes, kSurfaceRows), grd(8 * a.sper, nslot);
+  hipLaunchKernelGGL((k_surface<T, kSurfaceRows>), grd, blk, 0, s, a);
+  return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_src_fix(const MarchArgs<T> &a, const SrcCell *ecells, const int *eidx, const SrcCell *bcells,
