@@ -47,10 +47,9 @@ def build(verbose=True):
     os.replace(tmp, LIB)
     _run([hipcc()] + HIP_FLAGS + ["-x", "hip", os.path.join(CSRC, "hbm_copy.hip"), "-o",
                                   os.path.join(HERE, "hbm_copy")])
-    drv = os.path.join(CSRC, "tcsem_driver.cpp")
-    if os.path.exists(drv):
-        _run([hipcc()] + HIP_FLAGS + [drv, "-o", DRIVER, "-L" + HERE, "-ltcsem",
-                                      "-Wl,-rpath,$ORIGIN"])
+    # the driver is host code over the C ABI: g++ only, linked against the library beside it
+    _run(["g++", "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, "tcsem_driver.cpp"), "-o", DRIVER,
+          "-L" + HERE, "-ltcsem", "-Wl,-rpath,$ORIGIN"])
     return LIB
 
 

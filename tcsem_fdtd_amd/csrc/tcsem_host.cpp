@@ -395,6 +395,7 @@ struct tcsem_ctx {
   long long PS = 0, fstride = 0, pstride = 0, wstride = 0;
   int by = 8, pf = 2, ntx = 0, nty = 0, nper = 0;  // march tiling
   int stx = 0, sty = 0, sper = 0;                   // surface tiling
+  int nchunk = 0, chunk_len = 0;                    // z-chunks of the lower march (0: one pass)
   int xm = 1;                                       // x margin of the layout in elements
   int nslots = 0;
   // stream: the volume march and every host-ordered operation; stream2: upward continuation,
@@ -406,6 +407,7 @@ struct tcsem_ctx {
   int cur = 0;               // state holding the current fields
   char *d_sig = nullptr;
   char *d_planes = nullptr;  // bz0, bxair, byair, carry sum, carry compensation: 5 x nslots x pstride
+  char *d_zs = nullptr;      // z-chunk Bz partial sums: (kMaxChunks - 1) x nslots x pstride
   char *d_rd = nullptr;      // rdx, rdx2, rdy, rdy2, dzt, rdz, rdz2 (field type)
   double *d_grid = nullptr;  // X Y Z dx dy dz Xn Yn
   int *d_maps = nullptr;     // xBx2N yo2N xo2N yBy2N
@@ -483,6 +485,25 @@ void choose_tiling(tcsem_ctx *c, int nbatch) {
   }
   c->pf = pf;
   c->by = by;
+  // z-chunks of the lower march (levels N-1 .. 2) when the tiles alone leave CUs idle: enough
+  // chunks for two workgroups per CU, each chunk at least 8 levels deep
+  const int N = c->g.info.N, nlev = N - 2;
+  int nch = 1;
+  const long long wg = tiles(by) * nbatch;
+  if (N >= 6 && wg < 2LL * cu) nch = (int)std::min<long long>((2LL * cu + wg - 1) / wg, std::max(1, nlev / 8));
+  if (const char *e = std::getenv("TCSEM_MARCH_ZCHUNKS")) {
+    const int v = std::atoi(e);
+    if (v >= 1) nch = v;
+  }
+  nch = std::max(1, std::min({nch, kMaxChunks, N >= 6 ? nlev : 1}));
+  if (nch > 1) {
+    c->chunk_len = (nlev + nch - 1) / nch;
+    c->nchunk = (nlev + c->chunk_len - 1) / c->chunk_len;  // no empty chunk
+  } else {
+    c->nchunk = 0;
+    c->chunk_len = 0;
+  }
+  if (c->nchunk == 1) c->nchunk = 0;
   // march: 64-column blocks over columns 0..L-1 and rows 0..M-1 (column L and row M hold no
   // updated component); surface: 62 owned columns over 0..L, rows 0..M
   c->ntx = (L + kMarchLanes - 1) / kMarchLanes;
@@ -518,6 +539,10 @@ MarchArgs<T> march_args(tcsem_ctx *c) {
   a.byair = pl + 2 * ps + org;
   a.cs = pl + 3 * ps + org;
   a.cc = pl + 4 * ps + org;
+  a.zs = (T *)c->d_zs + org;
+  a.zstride = ps;
+  a.nchunk = 0;
+  a.chunk_len = 0;
   a.k_hi = c->g.info.N - 1;
   a.k_lo = 1;
   a.carry = 0;
@@ -657,7 +682,8 @@ int alloc_slots(tcsem_ctx *c, int nslots) {
   (void)hipFree(c->d_fields);
   (void)hipFree(c->d_planes);
   (void)hipFree(c->d_work);
-  c->d_fields = c->d_planes = nullptr;
+  (void)hipFree(c->d_zs);
+  c->d_fields = c->d_planes = c->d_zs = nullptr;
   c->d_work = nullptr;
   c->nslots = 0;
   const size_t fb = (size_t)2 * 5 * nslots * c->fstride * c->esz;
@@ -666,6 +692,9 @@ int alloc_slots(tcsem_ctx *c, int nslots) {
   if (hipMalloc(&c->d_fields, fb) != hipSuccess) return fail(TCSEM_ENOMEM, "field allocation of %zu bytes failed", fb);
   if (hipMalloc(&c->d_planes, pb) != hipSuccess) return fail(TCSEM_ENOMEM, "plane allocation failed");
   if (hipMalloc((void **)&c->d_work, wb) != hipSuccess) return fail(TCSEM_ENOMEM, "work allocation failed");
+  const size_t zb = (size_t)(kMaxChunks - 1) * nslots * c->pstride * c->esz;
+  if (hipMalloc(&c->d_zs, zb) != hipSuccess) return fail(TCSEM_ENOMEM, "chunk-sum allocation failed");
+  HIPCHK(launch_fill_zero(c->d_zs, zb, c->stream));
   HIPCHK(launch_fill_zero(c->d_fields, fb, c->stream));
   HIPCHK(launch_fill_zero(c->d_planes, pb, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -945,6 +974,11 @@ int do_step(tcsem_ctx *c) {
     MarchArgs<T> lo = a;
     lo.k_lo = 2;
     lo.carry = 2;
+    if (c->nchunk > 1) {
+      lo.nchunk = c->nchunk;
+      lo.chunk_len = c->chunk_len;
+      HIPCHK(launch_zsum<T>(lo, nb, st));
+    }
     HIPCHK(launch_march<T>(lo, nb, st));
   }
   if (c->timing) HIPCHK(hipEventRecord(c->ev[1], st));
@@ -1513,6 +1547,7 @@ void tcsem_destroy(tcsem_ctx *c) {
   (void)hipFree(c->d_fields);
   (void)hipFree(c->d_sig);
   (void)hipFree(c->d_planes);
+  (void)hipFree(c->d_zs);
   (void)hipFree(c->d_rd);
   (void)hipFree(c->d_grid);
   (void)hipFree(c->d_maps);

@@ -27,6 +27,7 @@ constexpr int kLineBytes = 128;               // x margin of the layout: one cac
 constexpr int kMarchLanes = 64;
 constexpr int kSurfaceOwnX = kMarchLanes - 2;
 constexpr int kSurfaceRows = 8;
+constexpr int kMaxChunks = 16;  // z-chunks of one lower march launch
 
 // One injected E cell: E[comp](k, j, i) -= amp * w each source step (ElectricalFiled.cu:347-475).
 // The same record lists the B cells a source step recomputes (comp 3 Bx, 4 By; w unused).
@@ -65,6 +66,12 @@ struct MarchArgs {
   // from them (carry 1) and writes Bz_old(0); carry 0 marches N-1 .. 1 in one launch.
   int k_hi, k_lo, carry;
   T *cs, *cc;                        // carry planes (sum, FP32 compensation), plane origin, slot 0
+  // The lower launch may split its levels into nchunk z-chunks of chunk_len levels (grid z):
+  // chunk c stores levels k_hi - c*chunk_len down, starting from the Bz partial sums of the
+  // chunks below it (k_zsum, planes zs + c*zstride, plane origin, slot 0).
+  int nchunk, chunk_len;
+  T *zs;
+  long long zstride;
 };
 
 // Receiver gather descriptor (EFInterp2Rx.cu:60-470) for one field group (E or B) of one event.
@@ -126,6 +133,8 @@ template <typename T>
 hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s);
 template <typename T>
 hipError_t launch_surface(const MarchArgs<T> &a, int nslot, hipStream_t s);
+template <typename T>
+hipError_t launch_zsum(const MarchArgs<T> &a, int nslot, hipStream_t s);
 template <typename T>
 hipError_t launch_src_fix(const MarchArgs<T> &a, const SrcCell *ecells, const int *eidx, const SrcCell *bcells,
                           const int *bidx, int nslot, hipStream_t s);

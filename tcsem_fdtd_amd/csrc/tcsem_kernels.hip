@@ -271,7 +271,16 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> a)
   // every slot is a fixed register set (no moves that would wait on loads in flight).  Levels
   // below k_lo-1 are never read (the top launch of a step owns them).
   constexpr int S = PF + 2;
-  const int khi = a.k_hi, klo = a.k_lo, lfloor = klo - 1;
+  // level range: k_hi .. k_lo, or one z-chunk of it; a chunk above the bottom one starts one
+  // level lower than it stores (kst) to recompute E_new there, which its first B update needs
+  int khi = a.k_hi, klo = a.k_lo, kst = a.k_hi;
+  const int chunk = blockIdx.z;
+  if (a.nchunk > 1) {
+    kst = a.k_hi - chunk * a.chunk_len;
+    klo = max(kst - a.chunk_len + 1, a.k_lo);
+    khi = chunk ? kst + 1 : kst;
+  }
+  const int lfloor = klo - 1;
   const long long po = (long long)slot * a.pstride - (a.PX + a.xm);
   auto load = [&](int lv) {
     const unsigned lb = (unsigned)max(lv, lfloor) * PSb;
@@ -318,6 +327,10 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> a)
     ex_n1 = vEx ? ldo((const T *)ex, off + PSb) : zero;
     ey_n1 = vEy ? ldo((const T *)ey, off + PSb) : zero;
   }
+  for (int c = 0; c < chunk; ++c) {  // Bz partial sums of the chunks below
+    bz.s += ldo((const T *)(a.zs + c * a.zstride + po), colb);
+    bzL.s += ldo((const T *)(a.zs + c * a.zstride + po), hcolb);
+  }
   T bx_last = 0, by_last = 0;  // old Bx, By of level k_lo-1 after the loop
   __syncthreads();
   // Drain the set-up loads here: values loaded before the loop (the carried sums, E_new(k_hi+1))
@@ -329,9 +342,11 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> a)
   // loop free of data-dependent branches, so the compiler's wait counts stay exact.
   auto level = [&](const int k, const Lv<T> &cur, const Lv<T> &nxt, Lv<T> &dst) {
     const int p = k & 1, q = p ^ 1;
-    const T dzk = sZ[k], rdzk = sZ[N + k], rdz2k = sZ[2 * N + k];
+    const bool st = k <= kst;  // false only on the E-only level that opens an upper chunk
+    const T dzk = st ? sZ[k] : zero, rdzk = sZ[N + k], rdz2k = sZ[2 * N + k];
     const T bx_k = cur.bx, by_k = cur.by, bx_m = nxt.bx, by_m = nxt.by;
     // (a) Bz_old(k) = Bz_old(k+1) + dz(k) div_h(B_old(k)), for the lane's column and column i0-1
+    // (the opening level of an upper chunk starts from Bz_old(k) itself: its increment is zero)
     bz.add(dzk * ((sBx[p][ly][slp] - bx_k) * rdx + (sBy[p][lyp][sl] - by_k) * rdy));
     bzL.add(dzk * ((bx_k - cur.hbx) * rdxL + (sBy[p][lyp][0] - cur.hby) * rdy));
     const T bzk = bz.val();
@@ -358,7 +373,7 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> a)
     }
     sEz[ly][sl] = ezn;
     sEz[ly][right ? LW - 1 : sl] = right ? ezr : ezn;
-    bsto(rez, off, vEz, ezn);
+    bsto(rez, off, vEz && st, ezn);
     // (d) Ex(k), Ey(k)
     T exn, eyn;
     {
@@ -373,14 +388,14 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> a)
       const T curl = (bx_k - bx_m) * rdz2k - (bzk - sBz[ly][slm]) * rdx2;
       eyn = vEy ? df_new(eyo, s4, cf, curl) : zero;
     }
-    bsto(rex, off, vEx, exn);
-    bsto(rey, off, vEy, eyn);
+    bsto(rex, off, vEx && st, exn);
+    bsto(rey, off, vEy && st, eyn);
     __syncthreads();
     // (e) Bx(k), By(k) from E_new(k), E_new(k+1)
     const T bxn = bx_k + dtb * ((ey_n1 - eyn) * rdzk - (sEz[lyp][sl] - ezn) * rdy);
     const T byn = by_k + dtb * ((sEz[ly][slp] - ezn) * rdx - (ex_n1 - exn) * rdzk);
-    bsto(rbx, off, vBx, bxn);
-    bsto(rby, off, vBy, byn);
+    bsto(rbx, off, vBx && st, bxn);
+    bsto(rby, off, vBy && st, byn);
     ex_n1 = exn;
     ey_n1 = eyn;
     off -= PSb;
@@ -395,7 +410,7 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> a)
   for (int u = 0; u < S - 1; ++u)
     if (k - u >= klo) level(k - u, R[u], R[(u + 1) % S], R[(u + S - 1) % S]);
   if (a.carry == 2) {  // lower launch: hand the running sums of level klo to the top launch
-    if (own) bz.save(a.cs + po, a.cc + po, colb);
+    if (own && chunk == max(a.nchunk, 1) - 1) bz.save(a.cs + po, a.cc + po, colb);
     return;
   }
   // Bz_old(0) for the upward continuation and the surface level (level 0 sits in buffer 0)
@@ -509,6 +524,29 @@ __global__ void k_src_b(const MarchArgs<T> a, const SrcCell *cells, const int *f
   }
 }
 
+// Bz partial sums of the z-chunks of a split lower march: for chunk c (levels k_hi - c*chunk_len
+// down to the next chunk) and every column, the sum over the chunk of dz(k) div_h(B_old(k)) in
+// the order the march adds it (Bderivate.cu:227-253).  Chunk c of the march starts from the sum
+// of the partials of chunks 0 .. c-1.  Grid: x over columns, y rows, z = chunk + (nchunk-1) slot.
+template <typename T>
+__global__ void __launch_bounds__(256) k_zsum(const MarchArgs<T> a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+  const int c = blockIdx.z % (a.nchunk - 1), slot = blockIdx.z / (a.nchunk - 1);
+  if (i >= a.L) return;
+  const long long fo = (long long)slot * a.fstride + (long long)j * a.PX + i;
+  const T *bx = a.bxi + fo, *by = a.byi + fo;
+  const T rdx = a.rdx[i + 1], rdy = a.rdy[j + 1];
+  const int hi = a.k_hi - c * a.chunk_len, lo = hi - a.chunk_len + 1;
+  Acc<T> acc;
+  acc.init();
+#pragma unroll 4
+  for (int k = hi; k >= lo; --k) {
+    const long long o = (long long)k * a.PS;
+    acc.add(a.dzt[k] * ((bx[o + 1] - bx[o]) * rdx + (by[o + a.PX] - by[o]) * rdy));
+  }
+  a.zs[c * a.zstride + (long long)slot * a.pstride + (long long)j * a.PX + i] = acc.val();
+}
+
 // Derived Bz volume (N+1, M, L dense) of one slot (Bderivate.cu:202-265), for field read-back.
 template <typename T>
 __global__ void k_bz_volume(const MarchArgs<T> a, T *bzout, int slot) {
@@ -544,13 +582,22 @@ __device__ inline double spline4(const double *x, const double *y, double t) {
          (y[2] / h1 - z2 / 6.0 * h1) * pa + (y[1] / h1 - z1 / 6.0 * h1) * pb;
 }
 
+// One workgroup (kGatherThreads lanes) per receiver and component of the group.  The 4 x 4 x 4
+// stencil values are loaded by 64 lanes at once; for Bz (never stored) each of the 16 stencil
+// columns gets 16 lanes that split the bottom-up sum of dz div_h(Bx, By) below the stencil, and one
+// lane per column then adds the four stencil levels.  The 16 x-, 4 y- and 1 z-splines run on 16,
+// 4 and 1 lanes.
+constexpr int kGatherThreads = 256;
 template <typename T>
-__global__ void k_gather(const GatherArgs g, const MarchArgs<T> a) {
-  const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(kGatherThreads) k_gather(const GatherArgs g, const MarchArgs<T> a) {
+  __shared__ double sv[64];        // stencil values [ik][ij][q]
+  __shared__ double spart[256];    // Bz partial sums [column][part]
+  __shared__ double sy1[16], sy2[4];
   const int slot = blockIdx.y;
   const int nr = g.rx_count[slot];
-  if (tid >= nr * g.ncomp_group) return;
-  const int r = tid / g.ncomp_group, cg = tid % g.ncomp_group;
+  const int r = blockIdx.x / g.ncomp_group, cg = blockIdx.x % g.ncomp_group;
+  if (r >= nr) return;  // uniform over the workgroup
+  const int lane = threadIdx.x;
   const int comp = g.comps[cg];
   const int ir = g.rx_local[g.rx_first[slot] + r];
   const double x = g.rxx[ir], y = g.rxy[ir], z = g.rxz[ir];
@@ -566,6 +613,40 @@ __global__ void k_gather(const GatherArgs g, const MarchArgs<T> a) {
   if (ze) s3 = (iz >= 1 && (Z[iz - 1] + 0.5 * dz[iz - 1]) > z) ? iz - 2 : iz - 1;
   else s3 = iz - 2;
   if (s3 < 0) s3 = 0;
+  const long long fo = (long long)slot * g.fstride;
+  if (comp < 5) {
+    if (lane < 64) {
+      const int ik = lane >> 4, ij = (lane >> 2) & 3, q = lane & 3;
+      const T *f = (const T *)g.f[comp] + fo;
+      sv[lane] = (double)f[(long long)(ik + s3) * g.PS + (long long)(ij + s2) * g.PX + s1 + q];
+    }
+  } else {
+    // Bz at levels s3..s3+3 of the 4 x 4 columns: bottom-up running sums of div_h(Bx, By)
+    const T *bxf = (const T *)g.f[3] + fo, *byf = (const T *)g.f[4] + fo;
+    const int col = lane >> 4, part = lane & 15, ij = col >> 2, q = col & 3;
+    const int ii = s1 + q, jj = s2 + ij;
+    const T rdx = a.rdx[ii + 1], rdy = a.rdy[jj + 1];
+    const long long base = (long long)jj * g.PX + ii;
+    auto div = [&](int k) {
+      const long long o = (long long)k * g.PS + base;
+      return (T)(a.dzt[k] * ((bxf[o + 1] - bxf[o]) * rdx + (byf[o + g.PX] - byf[o]) * rdy));
+    };
+    Acc<T> acc;
+    acc.init();
+    for (int k = g.N - 1 - part; k >= s3 + 4; k -= 16) acc.add(div(k));
+    spart[lane] = (double)acc.val();
+    __syncthreads();
+    if (part == 0) {
+      double run = 0.0;
+      for (int p = 15; p >= 0; --p) run += spart[col * 16 + p];
+      for (int ik = 3; ik >= 0; --ik) {
+        const int k = s3 + ik;
+        if (k < g.N) run += (double)div(k);
+        sv[ik * 16 + ij * 4 + q] = k < g.N ? run : 0.0;
+      }
+    }
+  }
+  __syncthreads();
   double tx[4], ty[4], tz[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -573,45 +654,12 @@ __global__ void k_gather(const GatherArgs g, const MarchArgs<T> a) {
     ty[q] = ye ? Y[s2 + q] - 0.5 * dy[s2 + q] : Y[s2 + q];
     tz[q] = ze ? Z[s3 + q] - 0.5 * dz[s3 + q] : Z[s3 + q];
   }
-  const long long fo = (long long)slot * g.fstride;
-  double v;
-  double Y1[4], Y2[4], F[4];
-  if (comp < 5) {
-    const T *f = (const T *)g.f[comp] + fo;
-    for (int ik = 0; ik < 4; ++ik) {
-      for (int ij = 0; ij < 4; ++ij) {
-        const long long o = (long long)(ik + s3) * g.PS + (long long)(ij + s2) * g.PX + s1;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) F[q] = (double)f[o + q];
-        Y1[ij] = spline4(tx, F, x);
-      }
-      Y2[ik] = spline4(ty, Y1, y);
-    }
-    v = spline4(tz, Y2, z);
-  } else {
-    // Bz at levels s3..s3+3 of the 4x4 columns: bottom-up running sums of div_h(Bx, By)
-    const T *bxf = (const T *)g.f[3] + fo, *byf = (const T *)g.f[4] + fo;
-    double Bzv[4][4][4];  // [ik][ij][q]
-    for (int ij = 0; ij < 4; ++ij)
-      for (int q = 0; q < 4; ++q) {
-        const int ii = s1 + q, jj = s2 + ij;
-        const T rdx = a.rdx[ii + 1], rdy = a.rdy[jj + 1];
-        Acc<T> acc;
-        acc.init();
-        for (int ik = 0; ik < 4; ++ik)
-          if (s3 + ik >= g.N) Bzv[ik][ij][q] = 0.0;
-        for (int k = g.N - 1; k >= s3; --k) {
-          const long long o = (long long)k * g.PS + (long long)jj * g.PX + ii;
-          acc.add(a.dzt[k] * ((bxf[o + 1] - bxf[o]) * rdx + (byf[o + g.PX] - byf[o]) * rdy));
-          if (k <= s3 + 3) Bzv[k - s3][ij][q] = (double)acc.val();
-        }
-      }
-    for (int ik = 0; ik < 4; ++ik) {
-      for (int ij = 0; ij < 4; ++ij) Y1[ij] = spline4(tx, Bzv[ik][ij], x);
-      Y2[ik] = spline4(ty, Y1, y);
-    }
-    v = spline4(tz, Y2, z);
-  }
+  if (lane < 16) sy1[lane] = spline4(tx, &sv[lane * 4], x);  // lane = ik*4 + ij
+  __syncthreads();
+  if (lane < 4) sy2[lane] = spline4(ty, &sy1[lane * 4], y);
+  __syncthreads();
+  if (lane != 0) return;
+  const double v = spline4(tz, sy2, z);
   const int c = g.comp_base + cg;
   double *e0 = g.ef0 + (long long)slot * 2 * g.ncomp * g.nrmax;
   e0[(long long)g.slot_store * g.ncomp * g.nrmax + (long long)c * g.nrmax + r] = v;
@@ -648,7 +696,7 @@ hipError_t launch_march_pf(const MarchArgs<T> &a, dim3 grd, dim3 blk, hipStream_
 }
 template <typename T, int BY>
 hipError_t launch_march_by(const MarchArgs<T> &a, int nslot, hipStream_t s) {
-  dim3 blk(kMarchLanes, BY), grd(8 * a.nper, nslot);
+  dim3 blk(kMarchLanes, BY), grd(8 * a.nper, nslot, max(a.nchunk, 1));
   switch (a.pf) {
     case 1: return launch_march_pf<T, BY, 1>(a, grd, blk, s);
     case 2: return launch_march_pf<T, BY, 2>(a, grd, blk, s);
@@ -665,6 +713,11 @@ hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s) {
     case 16: return launch_march_by<T, 16>(a, nslot, s);
   }
   return hipErrorInvalidValue;
+}
+template <typename T>
+hipError_t launch_zsum(const MarchArgs<T> &a, int nslot, hipStream_t s) {
+  hipLaunchKernelGGL((k_zsum<T>), dim3((a.L + 255) / 256, a.M, (a.nchunk - 1) * nslot), dim3(256), 0, s, a);
+  return hipGetLastError();
 }
 template <typename T>
 hipError_t launch_surface(const MarchArgs<T> &a, int nslot, hipStream_t s) {
@@ -690,8 +743,7 @@ hipError_t launch_bz_volume(const MarchArgs<T> &a, T *bz, int slot, hipStream_t 
 template <typename T>
 hipError_t launch_gather(const GatherArgs &g, int is_b, int nslot, const MarchArgs<T> &a, hipStream_t s) {
   (void)is_b;
-  const int n = g.nrmax * g.ncomp_group;
-  dim3 blk(64), grd((n + 63) / 64, nslot);
+  dim3 blk(kGatherThreads), grd(g.nrmax * g.ncomp_group, nslot);
   hipLaunchKernelGGL((k_gather<T>), grd, blk, 0, s, g, a);
   return hipGetLastError();
 }
@@ -712,6 +764,8 @@ template hipError_t launch_src_fix<float>(const MarchArgs<float> &, const SrcCel
                                           const int *, int, hipStream_t);
 template hipError_t launch_src_fix<double>(const MarchArgs<double> &, const SrcCell *, const int *, const SrcCell *,
                                            const int *, int, hipStream_t);
+template hipError_t launch_zsum<float>(const MarchArgs<float> &, int, hipStream_t);
+template hipError_t launch_zsum<double>(const MarchArgs<double> &, int, hipStream_t);
 template hipError_t launch_bz_volume<float>(const MarchArgs<float> &, float *, int, hipStream_t);
 template hipError_t launch_bz_volume<double>(const MarchArgs<double> &, double *, int, hipStream_t);
 template hipError_t launch_gather<float>(const GatherArgs &, int, int, const MarchArgs<float> &, hipStream_t);
