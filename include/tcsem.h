@@ -174,9 +174,11 @@ int tcsem_set_field(tcsem_ctx *ctx, int slot, int comp, const double *in);
 /* UpContinuation_gpu on a caller plane: bz0 (M*L) -> bxair (M*(L+1)), byair ((M+1)*L). */
 int tcsem_upcont_plane(tcsem_ctx *ctx, const double *bz0, double *bxair, double *byair);
 
-/* Average device time per launch [ms] of the kernels of the last tcsem_step/tcsem_run, measured
- * with HIP events on the context's stream: t[0] volume march, t[1] surface level,
- * t[2] upward continuation chain, t[3] receiver gather; n[i] launches. */
+/* Average device time [ms] of the kernels of the steps taken while timing was enabled, from HIP
+ * events that the kernels' dispatch packets record on the context's streams as the steps run
+ * (every 8th step; no extra synchronisation, reading waits for them): t[0] volume march per step
+ * (lower and top launch), t[1] surface level, t[2] upward continuation chain, t[3] receiver
+ * gather per launch; n[i] counts. */
 int tcsem_timing(const tcsem_ctx *ctx, double t[4], long n[4]);
 int tcsem_enable_timing(tcsem_ctx *ctx, int on);
 int tcsem_sync(tcsem_ctx *ctx);

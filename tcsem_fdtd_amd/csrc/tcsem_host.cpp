@@ -449,9 +449,16 @@ struct tcsem_ctx {
   int src_kmin = 1, src_kmax = 0;
   std::vector<int> op_it;
   std::vector<double> op_l1;
-  // timing
+  // timing: intervals between pooled events recorded on the streams while stepping (no extra
+  // synchronisation), folded into tacc/tn when read
   bool timing = false;
-  hipEvent_t ev[8] = {};
+  struct Span {
+    int kind, count;  // kind: 0 march, 1 surface, 2 upward continuation, 3 receiver gathers
+    hipEvent_t a, b;
+  };
+  std::vector<hipEvent_t> evpool;
+  size_t evused = 0;
+  std::vector<Span> spans;
   double tacc[4] = {0, 0, 0, 0};
   long tn[4] = {0, 0, 0, 0};
 };
@@ -951,6 +958,33 @@ UpcontArgs<float> &up_of<float>(tcsem_ctx *c) {
   return c->up32;
 }
 
+constexpr int kTimingStride = 8;
+
+hipEvent_t timing_event(tcsem_ctx *c) {
+  if (c->evused == c->evpool.size()) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    c->evpool.push_back(e);
+  }
+  return c->evpool[c->evused++];
+}
+
+// Waits for the recorded intervals and adds them to the per-kind totals.
+int rc_collect(tcsem_ctx *c) {
+  if (c->spans.empty()) return TCSEM_OK;
+  HIPCHK(hipStreamSynchronize(c->stream2));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (const auto &sp : c->spans) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, sp.a, sp.b));
+    c->tacc[sp.kind] += ms;
+    c->tn[sp.kind] += sp.count;
+  }
+  c->spans.clear();
+  c->evused = 0;
+  return TCSEM_OK;
+}
+
 template <typename T>
 int do_step(tcsem_ctx *c) {
   const tcsem_setup &s = c->setup;
@@ -969,39 +1003,42 @@ int do_step(tcsem_ctx *c) {
   // the old state and still reads the Bz(0) plane), the top march (level 1 and Bz_old(0)).
   // stream2: upward continuation, surface level and receiver gathers of this step.
   const bool split = N >= 6;
-  if (c->timing) HIPCHK(hipEventRecord(c->ev[0], st));
+  // timing events are recorded by the dispatch packets of the kernels themselves, on every
+  // kTimingStride-th step (each timed step costs a few microseconds of queue time)
+  const bool timed = c->timing && c->steps % kTimingStride == 0;
+  hipEvent_t te[8] = {};
+  if (timed) {
+    if (c->spans.size() > 50000 && (rc_collect(c))) return TCSEM_EDEVICE;
+    for (auto &e : te)
+      if (!(e = timing_event(c))) return fail(TCSEM_EDEVICE, "timing event creation failed");
+  }
   if (split) {
     MarchArgs<T> lo = a;
     lo.k_lo = 2;
     lo.carry = 2;
-    if (c->nchunk > 1) {
+    const bool chunked = c->nchunk > 1;
+    if (chunked) {
       lo.nchunk = c->nchunk;
       lo.chunk_len = c->chunk_len;
-      HIPCHK(launch_zsum<T>(lo, nb, st));
+      HIPCHK(launch_zsum<T>(lo, nb, st, te[0]));
     }
-    HIPCHK(launch_march<T>(lo, nb, st));
+    HIPCHK(launch_march<T>(lo, nb, st, chunked ? nullptr : te[0], te[1]));
   }
-  if (c->timing) HIPCHK(hipEventRecord(c->ev[1], st));
   HIPCHK(hipStreamWaitEvent(st, c->xev[1], 0));
-  if (c->timing) HIPCHK(hipEventRecord(c->ev[2], st));
   {
     MarchArgs<T> hi = a;
     if (split) {
       hi.k_hi = 1;
       hi.carry = 1;
     }
-    HIPCHK(launch_march<T>(hi, nb, st));
+    HIPCHK(launch_march<T>(hi, nb, st, te[2], te[3]));
   }
   if (c->has_fix && a.amp != 0.0)
     HIPCHK(launch_src_fix<T>(a, c->d_fixe, c->d_fixidx, c->d_fixb, c->d_fixidx + 2 * nb, nb, st));
-  if (c->timing) HIPCHK(hipEventRecord(c->ev[3], st));
   HIPCHK(hipEventRecord(c->xev[0], st));
   HIPCHK(hipStreamWaitEvent(st2, c->xev[0], 0));
-  if (c->timing) HIPCHK(hipEventRecord(c->ev[4], st2));
-  HIPCHK(launch_upcont<T>(u, nb, st2));
-  if (c->timing) HIPCHK(hipEventRecord(c->ev[5], st2));
-  HIPCHK(launch_surface<T>(a, nb, st2));
-  if (c->timing) HIPCHK(hipEventRecord(c->ev[6], st2));
+  HIPCHK(launch_upcont<T>(u, nb, st2, te[4], te[5]));
+  HIPCHK(launch_surface<T>(a, nb, st2, te[6], te[7]));
   c->cur ^= 1;  // the new state is current
   // time advance (main.cu:253-260)
   c->toT += c->dt1;
@@ -1011,7 +1048,6 @@ int do_step(tcsem_ctx *c) {
   else c->dt2 = std::min(0.15 * s.sfa1 * std::sqrt(miu * c->toT * minsita / 8.) * mindx, 1.2 * c->dt1);
   c->steps++;
   // receiver sampling (EFInterp2Rx(t0 = toT + dt1, dt = dt2))
-  int ngather = 0;
   if (c->have_rx) {
     const double t0 = c->toT + c->dt1;
     GatherArgs gA{};
@@ -1070,32 +1106,24 @@ int do_step(tcsem_ctx *c) {
       gA.op_it = c->d_ops + ev.op_first;
       gA.op_l1 = c->d_opl1 + ev.op_first;
       gA.nops = ev.op_count;
-      HIPCHK(launch_gather<T>(gA, fld, nb, march_args<T>(c), st2));
-      ++ngather;
+      hipEvent_t g0 = nullptr, g1 = nullptr;
+      if (timed) {
+        g0 = timing_event(c);
+        g1 = timing_event(c);
+        if (!g0 || !g1) return fail(TCSEM_EDEVICE, "timing event creation failed");
+        c->spans.push_back({3, 1, g0, g1});
+      }
+      HIPCHK(launch_gather<T>(gA, fld, nb, march_args<T>(c), st2, g0, g1));
     }
-    if (c->timing && ngather) HIPCHK(hipEventRecord(c->ev[7], st2));
   }
   HIPCHK(hipEventRecord(c->xev[1], st2));
-  if (c->timing) {
-    // timed steps run one at a time, so every interval is a kernel's own time
-    HIPCHK(hipStreamSynchronize(st2));
-    HIPCHK(hipStreamSynchronize(st));
-    float ms, ms2;
-    HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-    HIPCHK(hipEventElapsedTime(&ms2, c->ev[2], c->ev[3]));
-    c->tacc[0] += ms + ms2;
-    c->tn[0]++;
-    HIPCHK(hipEventElapsedTime(&ms, c->ev[5], c->ev[6]));
-    c->tacc[1] += ms;
-    c->tn[1]++;
-    HIPCHK(hipEventElapsedTime(&ms, c->ev[4], c->ev[5]));
-    c->tacc[2] += ms;
-    c->tn[2]++;
-    if (ngather) {
-      HIPCHK(hipEventElapsedTime(&ms, c->ev[6], c->ev[7]));
-      c->tacc[3] += ms;
-      c->tn[3] += ngather;
-    }
+  if (timed) {
+    // the march of a step is its lower and top launches (not the wait between them for the
+    // previous step's stream2 work); the step is counted once
+    if (split) c->spans.push_back({0, 0, te[0], te[1]});
+    c->spans.push_back({0, 1, te[2], te[3]});
+    c->spans.push_back({2, 1, te[4], te[5]});
+    c->spans.push_back({1, 1, te[6], te[7]});
   }
   return TCSEM_OK;
 }
@@ -1450,11 +1478,6 @@ int tcsem_create(const tcsem_setup *s, const double *cond, int precision, tcsem_
     tcsem_destroy(c);
     return fail(TCSEM_EDEVICE, "stream creation failed");
   }
-  for (auto &e : c->ev)
-    if (hipEventCreate(&e) != hipSuccess) {
-      tcsem_destroy(c);
-      return fail(TCSEM_EDEVICE, "event creation failed");
-    }
   for (auto &e : c->xev)
     if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
       tcsem_destroy(c);
@@ -1556,9 +1579,9 @@ void tcsem_destroy(tcsem_ctx *c) {
   (void)hipFree(c->d_scratch);
   (void)hipFree(c->d_rx);
   (void)hipFree(c->d_rxi);
-  for (auto &e : c->ev)
-    if (e) (void)hipEventDestroy(e);
   for (auto &e : c->xev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto &e : c->evpool)
     if (e) (void)hipEventDestroy(e);
   if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -1993,16 +2016,26 @@ int tcsem_upcont_plane(tcsem_ctx *c, const double *bz0, double *bxair, double *b
 
 int tcsem_enable_timing(tcsem_ctx *c, int on) {
   if (check_ctx(c)) return TCSEM_EINVAL;
-  c->timing = on != 0;
-  for (int q = 0; q < 4; ++q) {
-    c->tacc[q] = 0;
-    c->tn[q] = 0;
+  if (on) {
+    int rc = rc_collect(c);
+    if (rc) return rc;
+    for (int q = 0; q < 4; ++q) {
+      c->tacc[q] = 0;
+      c->tn[q] = 0;
+    }
+  } else {
+    int rc = rc_collect(c);  // keep what was recorded readable
+    if (rc) return rc;
   }
+  c->timing = on != 0;
   return TCSEM_OK;
 }
 
-int tcsem_timing(const tcsem_ctx *c, double t[4], long n[4]) {
-  if (!c || !t || !n) return fail(TCSEM_EINVAL, "null argument");
+int tcsem_timing(const tcsem_ctx *cc, double t[4], long n[4]) {
+  if (!cc || !t || !n) return fail(TCSEM_EINVAL, "null argument");
+  tcsem_ctx *c = const_cast<tcsem_ctx *>(cc);  // folding recorded intervals does not change the state
+  int rc = rc_collect(c);
+  if (rc) return rc;
   for (int q = 0; q < 4; ++q) {
     n[q] = c->tn[q];
     t[q] = c->tn[q] ? c->tacc[q] / c->tn[q] : 0.0;

@@ -9,6 +9,7 @@
 // levels, shared by all slots).  The surface planes (Bz0, BxAir, ByAir) use one level of the same
 // pitch.
 #pragma once
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
@@ -129,12 +130,16 @@ struct UpcontArgs {
 };
 
 // ---- launchers (tcsem_kernels.hip, tcsem_upcont.hip) ------------------------------------------
+// Optional events e0/e1 are recorded by the dispatch packets themselves (start of the first and
+// end of the last kernel of the launcher), so timing adds no packets to the stream.
 template <typename T>
-hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s);
+hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEvent_t e0 = nullptr,
+                        hipEvent_t e1 = nullptr);
 template <typename T>
-hipError_t launch_surface(const MarchArgs<T> &a, int nslot, hipStream_t s);
+hipError_t launch_surface(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEvent_t e0 = nullptr,
+                          hipEvent_t e1 = nullptr);
 template <typename T>
-hipError_t launch_zsum(const MarchArgs<T> &a, int nslot, hipStream_t s);
+hipError_t launch_zsum(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEvent_t e0 = nullptr);
 template <typename T>
 hipError_t launch_src_fix(const MarchArgs<T> &a, const SrcCell *ecells, const int *eidx, const SrcCell *bcells,
                           const int *bidx, int nslot, hipStream_t s);
@@ -142,9 +147,10 @@ template <typename T>
 hipError_t launch_bz_volume(const MarchArgs<T> &a, T *bz, int slot, hipStream_t s);
 template <typename T>
 hipError_t launch_gather(const GatherArgs &g, int is_b, int nslot, const MarchArgs<T> &a,
-                         hipStream_t s);
+                         hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 template <typename T>
-hipError_t launch_upcont(const UpcontArgs<T> &u, int nslot, hipStream_t s);
+hipError_t launch_upcont(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEvent_t e0 = nullptr,
+                         hipEvent_t e1 = nullptr);
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);
 
 }  // namespace tcsem

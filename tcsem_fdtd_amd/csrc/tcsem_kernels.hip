@@ -687,42 +687,43 @@ hipError_t march_lds(size_t bytes) {
   return e;
 }
 template <typename T, int BY, int PF>
-hipError_t launch_march_pf(const MarchArgs<T> &a, dim3 grd, dim3 blk, hipStream_t s) {
+hipError_t launch_march_pf(const MarchArgs<T> &a, dim3 grd, dim3 blk, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   const size_t zl = (size_t)3 * a.N * sizeof(T);
   const hipError_t e = march_lds<T, BY, PF>(zl);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_march<T, BY, PF>), grd, blk, zl, s, a);
+  hipExtLaunchKernelGGL((k_march<T, BY, PF>), grd, blk, (std::uint32_t)zl, s, e0, e1, 0u, a);
   return hipGetLastError();
 }
 template <typename T, int BY>
-hipError_t launch_march_by(const MarchArgs<T> &a, int nslot, hipStream_t s) {
+hipError_t launch_march_by(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   dim3 blk(kMarchLanes, BY), grd(8 * a.nper, nslot, max(a.nchunk, 1));
   switch (a.pf) {
-    case 1: return launch_march_pf<T, BY, 1>(a, grd, blk, s);
-    case 2: return launch_march_pf<T, BY, 2>(a, grd, blk, s);
-    case 3: return launch_march_pf<T, BY, 3>(a, grd, blk, s);
-    case 4: return launch_march_pf<T, BY, 4>(a, grd, blk, s);
+    case 1: return launch_march_pf<T, BY, 1>(a, grd, blk, s, e0, e1);
+    case 2: return launch_march_pf<T, BY, 2>(a, grd, blk, s, e0, e1);
+    case 3: return launch_march_pf<T, BY, 3>(a, grd, blk, s, e0, e1);
+    case 4: return launch_march_pf<T, BY, 4>(a, grd, blk, s, e0, e1);
   }
   return hipErrorInvalidValue;
 }
 template <typename T>
-hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s) {
+hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   switch (a.rows) {
-    case 4: return launch_march_by<T, 4>(a, nslot, s);
-    case 8: return launch_march_by<T, 8>(a, nslot, s);
-    case 16: return launch_march_by<T, 16>(a, nslot, s);
+    case 4: return launch_march_by<T, 4>(a, nslot, s, e0, e1);
+    case 8: return launch_march_by<T, 8>(a, nslot, s, e0, e1);
+    case 16: return launch_march_by<T, 16>(a, nslot, s, e0, e1);
   }
   return hipErrorInvalidValue;
 }
 template <typename T>
-hipError_t launch_zsum(const MarchArgs<T> &a, int nslot, hipStream_t s) {
-  hipLaunchKernelGGL((k_zsum<T>), dim3((a.L + 255) / 256, a.M, (a.nchunk - 1) * nslot), dim3(256), 0, s, a);
+hipError_t launch_zsum(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEvent_t e0) {
+  hipExtLaunchKernelGGL((k_zsum<T>), dim3((a.L + 255) / 256, a.M, (a.nchunk - 1) * nslot), dim3(256), 0u, s, e0,
+                        nullptr, 0u, a);
   return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_surface(const MarchArgs<T> &a, int nslot, hipStream_t s) {
+hipError_t launch_surface(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   dim3 blk(kMarchLanes, kSurfaceRows), grd(8 * a.sper, nslot);
-  hipLaunchKernelGGL((k_surface<T, kSurfaceRows>), grd, blk, 0, s, a);
+  hipExtLaunchKernelGGL((k_surface<T, kSurfaceRows>), grd, blk, 0u, s, e0, e1, 0u, a);
   return hipGetLastError();
 }
 template <typename T>
@@ -741,10 +742,11 @@ hipError_t launch_bz_volume(const MarchArgs<T> &a, T *bz, int slot, hipStream_t 
   return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_gather(const GatherArgs &g, int is_b, int nslot, const MarchArgs<T> &a, hipStream_t s) {
+hipError_t launch_gather(const GatherArgs &g, int is_b, int nslot, const MarchArgs<T> &a, hipStream_t s,
+                         hipEvent_t e0, hipEvent_t e1) {
   (void)is_b;
   dim3 blk(kGatherThreads), grd(g.nrmax * g.ncomp_group, nslot);
-  hipLaunchKernelGGL((k_gather<T>), grd, blk, 0, s, g, a);
+  hipExtLaunchKernelGGL((k_gather<T>), grd, blk, 0u, s, e0, e1, 0u, g, a);
   return hipGetLastError();
 }
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s) {
@@ -756,19 +758,19 @@ hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s) {
   return hipGetLastError();
 }
 
-template hipError_t launch_march<float>(const MarchArgs<float> &, int, hipStream_t);
-template hipError_t launch_march<double>(const MarchArgs<double> &, int, hipStream_t);
-template hipError_t launch_surface<float>(const MarchArgs<float> &, int, hipStream_t);
-template hipError_t launch_surface<double>(const MarchArgs<double> &, int, hipStream_t);
+template hipError_t launch_march<float>(const MarchArgs<float> &, int, hipStream_t, hipEvent_t, hipEvent_t);
+template hipError_t launch_march<double>(const MarchArgs<double> &, int, hipStream_t, hipEvent_t, hipEvent_t);
+template hipError_t launch_surface<float>(const MarchArgs<float> &, int, hipStream_t, hipEvent_t, hipEvent_t);
+template hipError_t launch_surface<double>(const MarchArgs<double> &, int, hipStream_t, hipEvent_t, hipEvent_t);
 template hipError_t launch_src_fix<float>(const MarchArgs<float> &, const SrcCell *, const int *, const SrcCell *,
                                           const int *, int, hipStream_t);
 template hipError_t launch_src_fix<double>(const MarchArgs<double> &, const SrcCell *, const int *, const SrcCell *,
                                            const int *, int, hipStream_t);
-template hipError_t launch_zsum<float>(const MarchArgs<float> &, int, hipStream_t);
-template hipError_t launch_zsum<double>(const MarchArgs<double> &, int, hipStream_t);
+template hipError_t launch_zsum<float>(const MarchArgs<float> &, int, hipStream_t, hipEvent_t);
+template hipError_t launch_zsum<double>(const MarchArgs<double> &, int, hipStream_t, hipEvent_t);
 template hipError_t launch_bz_volume<float>(const MarchArgs<float> &, float *, int, hipStream_t);
 template hipError_t launch_bz_volume<double>(const MarchArgs<double> &, double *, int, hipStream_t);
-template hipError_t launch_gather<float>(const GatherArgs &, int, int, const MarchArgs<float> &, hipStream_t);
-template hipError_t launch_gather<double>(const GatherArgs &, int, int, const MarchArgs<double> &, hipStream_t);
+template hipError_t launch_gather<float>(const GatherArgs &, int, int, const MarchArgs<float> &, hipStream_t, hipEvent_t, hipEvent_t);
+template hipError_t launch_gather<double>(const GatherArgs &, int, int, const MarchArgs<double> &, hipStream_t, hipEvent_t, hipEvent_t);
 
 }  // namespace tcsem

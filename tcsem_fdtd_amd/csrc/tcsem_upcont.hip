@@ -413,9 +413,10 @@ hipError_t launch_lines(K kern, const LineShape &sh, int nlines, int nslot, hipS
 }  // namespace
 
 template <typename T>
-hipError_t launch_upcont(const UpcontArgs<T> &u, int nslot, hipStream_t s) {
+hipError_t launch_upcont(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   hipError_t e;
-  hipLaunchKernelGGL((k_up_xpass<T>), dim3(u.M, nslot), dim3(256), (size_t)u.L * sizeof(double), s, u);
+  hipExtLaunchKernelGGL((k_up_xpass<T>), dim3(u.M, nslot), dim3(256), (std::uint32_t)(u.L * sizeof(double)), s, e0,
+                        nullptr, 0u, u);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const int npad = u.M1 - u.Ny;
   if (npad > 0) {
@@ -435,11 +436,11 @@ hipError_t launch_upcont(const UpcontArgs<T> &u, int nslot, hipStream_t s) {
           : launch_lines(k_up_ycols2<T, false>, sc, u.L1, nslot, s, u);
   if (e != hipSuccess) return e;
   const int n = u.M * (u.L + 1) + (u.M + 1) * u.L;
-  hipLaunchKernelGGL((k_up_bilin<T>), dim3((n + 255) / 256, nslot), dim3(256), 0, s, u);
+  hipExtLaunchKernelGGL((k_up_bilin<T>), dim3((n + 255) / 256, nslot), dim3(256), 0u, s, nullptr, e1, 0u, u);
   return hipGetLastError();
 }
 
-template hipError_t launch_upcont<float>(const UpcontArgs<float> &, int, hipStream_t);
-template hipError_t launch_upcont<double>(const UpcontArgs<double> &, int, hipStream_t);
+template hipError_t launch_upcont<float>(const UpcontArgs<float> &, int, hipStream_t, hipEvent_t, hipEvent_t);
+template hipError_t launch_upcont<double>(const UpcontArgs<double> &, int, hipStream_t, hipEvent_t, hipEvent_t);
 
 }  // namespace tcsem
