@@ -1,0 +1,36 @@
+"""Full-length accuracy of the FP32 path: the 256 x 256 x 160 marine model (seeded bathymetry and
+resistive slab, 101 seafloor receivers, samples from 10 ms to 50 s, about 19 000 DuFort-Frankel
+steps) run to its last sample with FP32 fields and with FP64 fields.  The FP64 path itself matches
+the FP64 oracle at round-off (test_gpu_parity.py); here the FP32 traces are held to the 1e-4
+relative trace error per receiver (components below 1e-6 of a receiver's largest norm, the
+symmetry nulls of an inline x-dipole, excluded)."""
+import numpy as np
+import pytest
+
+import tcsem_fdtd_amd as T
+from tcsem_fdtd_amd import workloads as W
+from tests import decks
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(deck, precision):
+    with T.Context(T.make_setup(**deck.setup), deck.cond, precision) as c:
+        r = deck.rxtime
+        c.set_receivers(T.make_rxtime(r["T0"], r["Tn"], r["Nt"], r["is_log"], r["comps"]), deck.rx)
+        c.set_transmitters([T.make_tx(**t) for t in deck.txs])
+        tr = c.run(0, 1)
+        return tr[0], c.steps_done
+
+
+def test_marine_3d_full_run_fp32_vs_fp64():
+    deck = W.marine_3d(T)
+    print("marine_3d full run FP64", flush=True)
+    ref, n64 = _run(deck, T.FP64)
+    print("marine_3d full run FP32 (%d steps)" % n64, flush=True)
+    got, n32 = _run(deck, T.FP32)
+    assert n32 == n64 > 15000
+    err = decks.trace_error(got, ref)
+    print("steps %d, per-receiver FP32 trace error: max %.3e median %.3e" % (n32, err.max(), np.median(err)),
+          flush=True)
+    assert np.all(np.isfinite(got)) and np.all(err < 1e-4), err
