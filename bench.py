@@ -5,11 +5,13 @@ A step is one full DF step (volume march, upward continuation of Bz(0), surface 
 receiver gathers the sampling schedule asks for) over the marine_3d workload: 256 x 256 x 160
 cells, seeded bathymetry and resistive slab, one x-dipole, 101 receivers.  Inputs are resident in
 HBM before the timed region.  The value is FP32 Gcell-updates/s of the single model; the same line
-carries the plain-FP64 and the FP32 transmitter-batched throughput, the roofline of the volume
-march (HIP events on the library's stream) and the FP64 CPU oracle timed on this host as the
-baseline.
+carries the plain-FP64 and the FP32 transmitter-batched (32 x-dipoles) throughput, the large
+512 x 512 x 256 model in FP32 and FP64 over a window of steps, the roofline of the volume march
+(HIP events on the library's stream), the LDS and HBM rates of the air-plane chain and of the
+receiver gathers, and the FP64 CPU oracle timed on this host as the baseline.  Every GPU run first
+lets the library's autotuner pick the march tiling for its batch size.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu] [--no-tune]
 """
 import argparse
 import json
@@ -23,12 +25,18 @@ sys.path.insert(0, ROOT)
 METRIC = "Gcell-updates/s + % HBM roofline, FP32-batched vs plain FP64 at 1e-4 trace err"
 PEAK_HBM_GBS = 8000.0          # MI355X HBM3E, AMD published
 WORDS_PER_CELL = 13            # read Ex Ey Ez Bx By Bz sigma, write the six fields
+# LDS rate assumed for the air-plane FFT figures: 128 B per clock per CU (the CDNA LDS read rate)
+# x 256 CUs x 2.4 GHz
+PEAK_LDS_GBS = 128 * 256 * 2.4
 
 
-def gpu_run(T, deck, precision, nbatch, steps, warmup):
+def gpu_run(T, deck, precision, nbatch, steps, warmup, tune=True):
     import numpy as np
     st = T.make_setup(**deck.setup)
     with T.Context(st, deck.cond, precision) as c:
+        # the library's autotuner times every march tiling for this batch size and keeps the
+        # fastest (outside the timed region, on zeroed fields)
+        tiling = c.autotune(nbatch, 5) if tune else None
         r = deck.rxtime
         c.set_receivers(T.make_rxtime(r["T0"], r["Tn"], r["Nt"], r["is_log"], r["comps"]), deck.rx)
         txs = [T.make_tx(**dict(t, txid=i + 1)) for i, t in enumerate(deck.txs)]
@@ -52,7 +60,7 @@ def gpu_run(T, deck, precision, nbatch, steps, warmup):
         "gcell_s": cells * nbatch * steps / sec / 1e9,
         "ms_per_step": 1000.0 * sec / steps,
         "timing": tm, "L": c.L, "M": c.M, "N": c.N, "ex_norm": float(np.linalg.norm(ex)),
-        "ex": ex,
+        "ex": ex, "tiling": tiling,
     }
 
 
@@ -68,6 +76,97 @@ def roofline(res, esz, nbatch, traffic):
             "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
             "kernel": "k_march", "algorithmic_bytes_per_launch": int(algo_bytes),
             "avg_launch_ms": round(ms, 5)}
+
+
+def _fft_len(n):
+    """Working length of a line FFT: n itself for powers of two, else the Bluestein length."""
+    need = n if n & (n - 1) == 0 else 2 * n - 1
+    m = 1
+    while m < need:
+        m <<= 1
+    return m
+
+
+def _fft_lds_entries(n, m):
+    """LDS entries (complex) read and written by one line FFT of length n on working length m:
+    every Stockham pass reads and writes the line, passes after the first read (R-1)/R m
+    twiddles; Bluestein adds the chirp products around two transforms of length m."""
+    def pow2(m):
+        tot, left, first = 0, m.bit_length() - 1, True
+        while left > 0:
+            r = 8 if left >= 3 else (4 if left == 2 else 2)
+            left -= {8: 3, 4: 2, 2: 1}[r]
+            tot += 2 * m + (0 if first else (r - 1) * m // r)
+            first = False
+        return tot
+    if n == m:
+        return pow2(m)
+    return 2 * pow2(m) + (n + m) + 2 * m + 2 * n
+
+
+def upcont_model(deck, esz):
+    """Algorithmic LDS and HBM bytes of one upward continuation (tiled chain), one slot."""
+    import tcsem_fdtd_amd as T
+    s = deck.setup
+    g = T.grid_info(T.make_setup(**s))
+    L, M, L1, M1 = g.L, g.M, g.L1, g.M1
+    H = L1 // 2 + 1
+    mx, my = _fft_len(L1), _fft_len(M1)
+    n1 = s["ALx1"] + s["ALx2"] + 4
+    n2 = s["ARx1"] + s["ARx2"] + 4
+    n3 = s["Ay1"] + s["Ay2"] + 4
+    aL, aR, aY, Nx, Ny = g.AnLx_New, g.AnRx_New, g.Any_New, s["Nx"], s["Ny"]
+    npair = (M + 1) // 2
+    c = 16  # bytes per complex entry
+    k1 = npair * (2 * L * 8 + 2 * (Nx + aL * n1 + aR * n2) * 8 + L1 * c + _fft_lds_entries(L1, mx) * c + 6 * H * c)
+    k2 = H * (M * c + (Ny + 2 * aY * n3) * c + M1 * c + 3 * _fft_lds_entries(M1, my) * c + 5 * M1 * c)
+    k3 = M1 * (L1 * c + _fft_lds_entries(L1, mx) * c + L1 * c)
+    nair = M * (L + 1) + (M + 1) * L
+    hbm = (M * L * esz + H * M * c) + (H * M * c + H * M1 * 2 * c) + (H * M1 * 2 * c + M1 * L1 * 16) + \
+          (4 * nair * 8 + nair * esz)
+    return {"lds_bytes": int(k1 + k2 + k3), "hbm_bytes": int(hbm), "plane": "%dx%d" % (L1, M1)}
+
+
+def gather_model(deck, esz):
+    """Algorithmic bytes of one receiver-gather launch pair (E group and B group) over all the
+    deck's receivers: 4 x 4 x 4 stencil values per component, and for Bz the Bx/By columns
+    under the 4 x 4 stencil columns (5 x 4 + 4 x 5 values per level) from the bottom."""
+    import numpy as np
+    import tcsem_fdtd_amd as T
+    st = T.make_setup(**deck.setup)
+    g = T.grid_info(st)
+    Z = T.grid_arrays(st)["Z"]
+    comps = deck.rxtime["comps"]
+    ne = sum(comps[:3])
+    nb2 = sum(comps[3:5])
+    total = 0
+    for x, y, z in np.asarray(deck.rx):
+        k = int(np.clip(np.searchsorted(Z, z) - 2, 0, g.N - 4))
+        total += (ne + nb2) * 64 * esz
+        if comps[5]:
+            total += 64 * esz + 40 * (g.N - k) * esz
+    return total
+
+
+def side_kernels(res, deck, esz, nbatch):
+    """Upward continuation (LDS and HBM rate of the chain) and receiver gathers (HBM rate) from
+    the measured in-situ kernel times and the algorithmic byte models."""
+    tm = res["timing"]
+    up = upcont_model(deck, esz)
+    out = {}
+    if tm["upcont_ms"] > 0:
+        sec = tm["upcont_ms"] * 1e-3
+        out["upcont"] = {"plane": up["plane"], "ms": round(tm["upcont_ms"], 5),
+                         "lds_bytes": up["lds_bytes"] * nbatch, "hbm_bytes": up["hbm_bytes"] * nbatch,
+                         "lds_GBs": round(up["lds_bytes"] * nbatch / sec / 1e9, 1),
+                         "lds_frac": round(up["lds_bytes"] * nbatch / sec / 1e9 / PEAK_LDS_GBS, 4),
+                         "hbm_GBs": round(up["hbm_bytes"] * nbatch / sec / 1e9, 1)}
+    if tm["gather_ms"] > 0:
+        per = gather_model(deck, esz) * nbatch / 2.0  # an E launch and a B launch per sample event
+        gb = per / (tm["gather_ms"] * 1e-3) / 1e9
+        out["gather"] = {"ms_per_launch": round(tm["gather_ms"], 5), "bytes_per_launch": int(per),
+                         "hbm_GBs": round(gb, 1), "hbm_frac": round(gb / PEAK_HBM_GBS, 4)}
+    return out
 
 
 def cpu_baseline(deck, budget_s=12.0, max_steps=30):
@@ -111,6 +210,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="heuristic march tiling instead of the autotuner")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (multi-GPU sharding is design-only); "
@@ -121,11 +221,20 @@ def main():
     from tcsem_fdtd_amd import workloads as W
 
     deck = W.marine_3d(T)
-    r32 = gpu_run(T, deck, T.FP32, 1, args.steps, args.warmup)
-    r64 = gpu_run(T, deck, T.FP64, 1, args.steps, args.warmup)
+    tune = not args.no_tune
+    r32 = gpu_run(T, deck, T.FP32, 1, args.steps, args.warmup, tune)
+    r64 = gpu_run(T, deck, T.FP64, 1, args.steps, args.warmup, tune)
     field_err = float(np.linalg.norm(r32["ex"] - r64["ex"]) / max(np.linalg.norm(r64["ex"]), 1e-300))
     tow = W.towline(T, ntx=args.batch)
-    rb = gpu_run(T, tow, T.FP32, args.batch, max(args.steps // 5, 1), max(args.warmup // 5, 1))
+    rb = gpu_run(T, tow, T.FP32, args.batch, max(args.steps // 5, 1), max(args.warmup // 5, 1), tune)
+    # the large single-GPU model (512 x 512 x 256 cells, 1024 x 1024 air plane): FP32 and FP64
+    # over a window of steps from the start of the run, and their field difference after it
+    big = W.large_5(T)
+    nl, wl = max(args.steps // 2, 1), max(args.warmup // 2, 1)
+    rl32 = gpu_run(T, big, T.FP32, 1, nl, wl, tune)
+    rl64 = gpu_run(T, big, T.FP64, 1, nl, wl, tune)
+    big_err = float(np.linalg.norm(rl32["ex"] - rl64["ex"]) / max(np.linalg.norm(rl64["ex"]), 1e-300))
+    rl32.pop("ex"), rl64.pop("ex")
     line = {
         "metric": METRIC,
         "value": round(r32["gcell_s"], 3),
@@ -141,17 +250,29 @@ def main():
         "data": "synthetic (seeded model, bathymetry and slab; see tcsem_fdtd_amd/workloads.py)",
         "config": {"workload": "marine_3d 256x256x160, 1 x-dipole, 101 receivers, FP32 fields",
                    "cells": r32["L"] * r32["M"] * r32["N"], "air_plane": "512x512",
-                   "batch": 1, "parallelism": "single GPU"},
+                   "batch": 1, "parallelism": "single GPU", "march_tiling": r32["tiling"]},
         "roofline": roofline(r32, 4, 1, load_traffic("marine_3d_fp32")),
         "kernel_ms": {k: round(v, 5) for k, v in r32["timing"].items() if k.endswith("_ms")},
+        "side_kernels": side_kernels(r32, deck, 4, 1),
         "fp64_single": {"value": round(r64["gcell_s"], 3), "ms_per_step": round(r64["ms_per_step"], 5),
                         "roofline": roofline(r64, 8, 1, load_traffic("marine_3d_fp64")),
+                        "march_tiling": r64["tiling"],
                         "kernel_ms": {k: round(v, 5) for k, v in r64["timing"].items() if k.endswith("_ms")}},
         "fp32_batched": {"value": round(rb["gcell_s"], 3), "batch": args.batch,
                          "workload": "towline %d x-dipoles, marine_3d grid" % args.batch,
                          "ms_per_step": round(rb["ms_per_step"], 5),
                          "roofline": roofline(rb, 4, args.batch, load_traffic("towline32_fp32")),
-                         "kernel_ms": {k: round(v, 5) for k, v in rb["timing"].items() if k.endswith("_ms")}},
+                         "march_tiling": rb["tiling"],
+                         "kernel_ms": {k: round(v, 5) for k, v in rb["timing"].items() if k.endswith("_ms")},
+                         "side_kernels": side_kernels(rb, tow, 4, args.batch)},
+        "large_5": {"workload": "large_5 512x512x256, 1024x1024 air plane, 3 x-dipoles (first one run), 401 receivers",
+                    "steps": nl, "warmup": wl,
+                    "fp32": {"value": round(rl32["gcell_s"], 3), "ms_per_step": round(rl32["ms_per_step"], 5),
+                             "roofline": roofline(rl32, 4, 1, None), "march_tiling": rl32["tiling"],
+                             "side_kernels": side_kernels(rl32, big, 4, 1)},
+                    "fp64": {"value": round(rl64["gcell_s"], 3), "ms_per_step": round(rl64["ms_per_step"], 5),
+                             "roofline": roofline(rl64, 8, 1, None), "march_tiling": rl64["tiling"]},
+                    "fp32_vs_fp64_ex_rel_l2_after_window": big_err},
         "ratio_fp32_batched_vs_fp64": round(rb["gcell_s"] / r64["gcell_s"], 3),
         "fp32_vs_fp64_ex_rel_l2_after_run": field_err,
     }

@@ -151,6 +151,16 @@ int tcsem_reserve_batch(tcsem_ctx *ctx, int nbatch);
 /* set_zeros_EM_arrays (prepare_free_mem.cu:164-186) for field sets [first, first+n). */
 int tcsem_reset_fields(tcsem_ctx *ctx, int first, int n);
 
+/* Autotuner of the volume march (no counterpart in the reference, whose launch shapes are
+ * fixed): times nsteps nbatch-transmitter steps as they run in production (the march beside the
+ * upward continuation and surface level on the second stream; no source, no gathers) for every
+ * tiling the library runs -- rows per workgroup 4/8/16 (FP64: 4/8), prefetch depth 1-4, z-chunks
+ * of the lower launch 1/2/4/8 as the depth allows -- and keeps the fastest for later batches of
+ * that size (TCSEM_MARCH_ROWS / _PF / _ZCHUNKS still override it).  It runs on the fields of
+ * slots [0, nbatch): a begun run is discarded and the fields are left zeroed.  Outputs
+ * (nullable): the chosen rows, prefetch depth, z-chunks and the time per step [ms]. */
+int tcsem_autotune(tcsem_ctx *ctx, int nbatch, int nsteps, int *rows, int *pf, int *zchunks, double *ms);
+
 /* One DuFort-Frankel step for field sets [0, nbatch), each driven by transmitter
  * tx_first+b with the schedule of the step counter.  Equivalent to main.cu:223-260:
  * UpContinuation_gpu, ElectricalField (with source), Bderivate, then time advance. */

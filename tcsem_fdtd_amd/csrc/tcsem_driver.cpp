@@ -9,7 +9,8 @@
 //     uninitialised traces);
 //   * transmitters are advanced --batch at a time in one set of launches (independent field sets
 //     sharing the conductivity);
-//   * --precision 32 runs FP32 fields (FP64 schedule, coefficients, air-plane FFT).
+//   * --precision 32 runs FP32 fields (FP64 schedule, coefficients, air-plane FFT);
+//   * --autotune times every march tiling for the batch size first and keeps the fastest.
 // All device work goes through the C ABI of libtcsem.so; errors print the library message and
 // exit with the (positive) error code.
 #include <algorithm>
@@ -30,10 +31,11 @@ int die(int code, const char *what) {
 
 void usage() {
   std::printf(
-      "usage: tcsem_fdtd [--dir D] [--precision 64|32] [--batch B] [--max-steps S] [--quiet]\n"
+      "usage: tcsem_fdtd [--dir D] [--precision 64|32] [--batch B] [--max-steps S] [--autotune] [--quiet]\n"
       "  Reads FDTD_setup.in, Cond.bin, Receiver.in, Transmitter.in from D (default: .) and writes\n"
       "  Tx<id>.bout there.  --batch: transmitters advanced together (default 8).\n"
-      "  --max-steps: cap on steps per transmitter (default: until the last sample).\n");
+      "  --max-steps: cap on steps per transmitter (default: until the last sample).\n"
+      "  --autotune: time every march tiling for the batch size first and keep the fastest.\n");
 }
 
 }  // namespace
@@ -41,7 +43,7 @@ void usage() {
 int main(int argc, char **argv) {
   std::string dir = ".";
   int precision = TCSEM_FP64, batch = 8, max_steps = 0;
-  bool quiet = false;
+  bool quiet = false, tune = false;
   for (int a = 1; a < argc; ++a) {
     const std::string s = argv[a];
     auto next = [&](const char *name) -> const char * {
@@ -56,6 +58,7 @@ int main(int argc, char **argv) {
     else if (s == "--batch") batch = std::atoi(next("--batch"));
     else if (s == "--max-steps") max_steps = std::atoi(next("--max-steps"));
     else if (s == "--quiet") quiet = true;
+    else if (s == "--autotune") tune = true;
     else if (s == "-h" || s == "--help") {
       usage();
       return 0;
@@ -111,6 +114,17 @@ int main(int argc, char **argv) {
   if ((rc = tcsem_set_transmitters(ctx, txs.data(), ntx))) {
     tcsem_destroy(ctx);
     return die(rc, "transmitter setup");
+  }
+  if (tune && ntx > 0) {
+    int rows = 0, pf = 0, zc = 0;
+    double ms = 0.0;
+    if ((rc = tcsem_autotune(ctx, std::min(batch, ntx), 5, &rows, &pf, &zc, &ms))) {
+      tcsem_destroy(ctx);
+      return die(rc, "autotune");
+    }
+    if (!quiet)
+      std::printf("autotuned march: %d rows per tile, prefetch %d, %d z-chunk(s): %.4f ms per step\n", rows, pf, zc,
+                  ms);
   }
   for (int first = 0; first < ntx; first += batch) {
     const int nb = std::min(batch, ntx - first);

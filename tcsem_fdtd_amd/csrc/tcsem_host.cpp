@@ -394,6 +394,7 @@ struct tcsem_ctx {
   int PX = 0, PY = 0;
   long long PS = 0, fstride = 0, pstride = 0, wstride = 0;
   int by = 8, pf = 2, ntx = 0, nty = 0, nper = 0;  // march tiling
+  int tune_nbatch = 0, tune_by = 0, tune_pf = 0, tune_nch = 0;  // autotuned tiling and its batch size
   int stx = 0, sty = 0, sper = 0;                   // surface tiling
   int nchunk = 0, chunk_len = 0;                    // z-chunks of the lower march (0: one pass)
   int xm = 1;                                       // x margin of the layout in elements
@@ -412,12 +413,17 @@ struct tcsem_ctx {
   double *d_grid = nullptr;  // X Y Z dx dy dz Xn Yn
   int *d_maps = nullptr;     // xBx2N yo2N xo2N yBy2N
   double *d_up = nullptr;    // wl wr wf wb gx gy, plan tables
-  double *d_work = nullptr;  // all slots of: cplx (complex), tx, bxr, byr
+  // upward-continuation work: the tiled chain's A, G, bxr, byr regions (slot strides sA, sG, sX
+  // doubles), or the line chain's cplx (complex), tx, bxr, byr regions (wstride each)
+  double *d_work = nullptr;
+  int up_tiled = 0, MA = 0, MG = 0, XR = 0;
+  long long sA = 0, sG = 0, sX = 0;
   double *d_scratch = nullptr;
   size_t scratch_bytes = 0;
   UpcontArgs<double> up64{};
   UpcontArgs<float> up32{};
   HostPlan plx, ply;
+  std::vector<double> gxT, gyT;  // gains kx-major with the non-Hermitian bins zeroed (tiled chain)
   std::vector<double> wl, wr, wf, wb, gx, gy;
   int n1 = 0, n2 = 0, n3 = 0;
   // receivers
@@ -471,37 +477,21 @@ namespace {
 // FP64 fields stay at 8 rows or fewer (16 rows would spill registers).  TCSEM_MARCH_ROWS
 // overrides the choice (4, 8 or 16).  The prefetch depth (levels of loads in flight per lane) is
 // 2 for FP32 fields in short tiles and 1 otherwise; TCSEM_MARCH_PF (1 .. 4) overrides it.
-void choose_tiling(tcsem_ctx *c, int nbatch) {
-  const int L = c->g.info.L, M = c->g.info.M;
-  int cu = 256;
-  int dev = 0;
+int device_cus() {
+  int cu = 256, dev = 0;
   hipDeviceProp_t prop;
   if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
     cu = prop.multiProcessorCount;
-  auto tiles = [&](int b) { return (long long)((L + kMarchLanes - 1) / kMarchLanes) * ((M + (b - 2) - 1) / (b - 2)); };
-  int by = tiles(16) * nbatch >= 2LL * cu ? 16 : 4;
-  if (const char *e = std::getenv("TCSEM_MARCH_ROWS")) {
-    const int v = std::atoi(e);
-    if (v == 4 || v == 8 || v == 16) by = v;
-  }
+  return cu;
+}
+
+// Launch geometry of a march tiling: `by` rows per workgroup (by - 2 owned), prefetch depth pf,
+// nch z-chunks of the lower launch (1: one pass).
+void apply_tiling(tcsem_ctx *c, int by, int pf, int nch) {
+  const int L = c->g.info.L, M = c->g.info.M, N = c->g.info.N, nlev = N - 2;
   if (c->prec == 64 && by > 8) by = 8;  // FP64 16-row tiles do not fit their registers
-  int pf = (by == 16 || c->prec == 64) ? 1 : 2;
-  if (const char *e = std::getenv("TCSEM_MARCH_PF")) {
-    const int v = std::atoi(e);
-    if (v >= 1 && v <= 4) pf = v;
-  }
-  c->pf = pf;
   c->by = by;
-  // z-chunks of the lower march (levels N-1 .. 2) when the tiles alone leave CUs idle: enough
-  // chunks for two workgroups per CU, each chunk at least 8 levels deep
-  const int N = c->g.info.N, nlev = N - 2;
-  int nch = 1;
-  const long long wg = tiles(by) * nbatch;
-  if (N >= 6 && wg < 2LL * cu) nch = (int)std::min<long long>((2LL * cu + wg - 1) / wg, std::max(1, nlev / 8));
-  if (const char *e = std::getenv("TCSEM_MARCH_ZCHUNKS")) {
-    const int v = std::atoi(e);
-    if (v >= 1) nch = v;
-  }
+  c->pf = pf;
   nch = std::max(1, std::min({nch, kMaxChunks, N >= 6 ? nlev : 1}));
   if (nch > 1) {
     c->chunk_len = (nlev + nch - 1) / nch;
@@ -519,6 +509,41 @@ void choose_tiling(tcsem_ctx *c, int nbatch) {
   c->stx = (L + 1 + kSurfaceOwnX - 1) / kSurfaceOwnX;
   c->sty = (M + 1 + (kSurfaceRows - 2) - 1) / (kSurfaceRows - 2);
   c->sper = (c->stx * c->sty + 7) / 8;
+}
+
+// Tiling of an nbatch-transmitter step: the autotuned one for this batch size when there is
+// one, else the heuristic (16-row tiles once the batch gives two workgroups per CU, else 4;
+// prefetch 2 for FP32 short tiles; z-chunks when the tiles alone leave CUs idle); the
+// TCSEM_MARCH_ROWS / _PF / _ZCHUNKS environment variables override either.
+void choose_tiling(tcsem_ctx *c, int nbatch) {
+  const int L = c->g.info.L, M = c->g.info.M, N = c->g.info.N, nlev = N - 2;
+  const int cu = device_cus();
+  auto tiles = [&](int b) { return (long long)((L + kMarchLanes - 1) / kMarchLanes) * ((M + (b - 2) - 1) / (b - 2)); };
+  int by = tiles(16) * nbatch >= 2LL * cu ? 16 : 4;
+  if (c->prec == 64 && by > 8) by = 8;
+  int pf = (by == 16 || c->prec == 64) ? 1 : 2;
+  // enough chunks for two workgroups per CU, each chunk at least 8 levels deep
+  int nch = 1;
+  const long long wg = tiles(by) * nbatch;
+  if (N >= 6 && wg < 2LL * cu) nch = (int)std::min<long long>((2LL * cu + wg - 1) / wg, std::max(1, nlev / 8));
+  if (c->tune_nbatch == nbatch) {
+    by = c->tune_by;
+    pf = c->tune_pf;
+    nch = c->tune_nch;
+  }
+  if (const char *e = std::getenv("TCSEM_MARCH_ROWS")) {
+    const int v = std::atoi(e);
+    if (v == 4 || v == 8 || v == 16) by = v;
+  }
+  if (const char *e = std::getenv("TCSEM_MARCH_PF")) {
+    const int v = std::atoi(e);
+    if (v >= 1 && v <= 4) pf = v;
+  }
+  if (const char *e = std::getenv("TCSEM_MARCH_ZCHUNKS")) {
+    const int v = std::atoi(e);
+    if (v >= 1) nch = v;
+  }
+  apply_tiling(c, by, pf, nch);
 }
 
 template <typename T>
@@ -640,13 +665,34 @@ UpcontArgs<T> upcont_args(tcsem_ctx *c) {
   plan(c->ply, u.py);
   u.H = I.L1 / 2 + 1;
   u.wstride = c->wstride;
+  u.gxT = p;
+  p += c->gxT.size();
+  u.gyT = p;
+  p += c->gyT.size();
   double *w = c->d_work;
-  const long long ws = c->wstride * c->nslots;
-  u.cplx = (double2 *)w;
-  u.tx = w + 2 * ws;
-  u.bxr = w + 3 * ws;
-  u.ry = u.bxr;  // consumed by the forward column FFTs before the results are written
-  u.byr = w + 4 * ws;
+  u.tiled = c->up_tiled;
+  if (c->up_tiled) {
+    const long long ns = c->nslots;
+    u.A = (double2 *)w;
+    u.sA = c->sA;
+    u.MA = c->MA;
+    u.G = (double2 *)(w + ns * c->sA);
+    u.sG = c->sG;
+    u.MG = c->MG;
+    u.bxr = w + ns * (c->sA + c->sG);
+    u.byr = u.bxr + ns * c->sX;
+    u.sX = c->sX;
+    u.XR = c->XR;
+  } else {
+    const long long ws = c->wstride * c->nslots;
+    u.cplx = (double2 *)w;
+    u.tx = w + 2 * ws;
+    u.bxr = w + 3 * ws;
+    u.ry = u.bxr;  // consumed by the forward column FFTs before the results are written
+    u.byr = w + 4 * ws;
+    u.sX = c->wstride;
+    u.XR = I.L1;
+  }
   const double *gd = c->d_grid;
   u.X = gd;
   u.Y = gd + I.L;
@@ -695,7 +741,8 @@ int alloc_slots(tcsem_ctx *c, int nslots) {
   c->nslots = 0;
   const size_t fb = (size_t)2 * 5 * nslots * c->fstride * c->esz;
   const size_t pb = (size_t)5 * nslots * c->pstride * c->esz;
-  const size_t wb = (size_t)5 * nslots * c->wstride * sizeof(double);
+  const long long per_slot = c->up_tiled ? c->sA + c->sG + 2 * c->sX : 5 * c->wstride;
+  const size_t wb = (size_t)nslots * per_slot * sizeof(double);
   if (hipMalloc(&c->d_fields, fb) != hipSuccess) return fail(TCSEM_ENOMEM, "field allocation of %zu bytes failed", fb);
   if (hipMalloc(&c->d_planes, pb) != hipSuccess) return fail(TCSEM_ENOMEM, "plane allocation failed");
   if (hipMalloc((void **)&c->d_work, wb) != hipSuccess) return fail(TCSEM_ENOMEM, "work allocation failed");
@@ -985,6 +1032,38 @@ int rc_collect(tcsem_ctx *c) {
   return TCSEM_OK;
 }
 
+// The volume march of one step on stream st: with N >= 6 a lower launch (levels N-1 .. 2, z-chunked
+// when the tiling says so) and, after `wait` (the previous step's stream2 work; may be null), the
+// top launch (level 1 and Bz_old(0)); with fewer levels one launch.  te (null, or 4 events):
+// start/end of the lower and of the top launch.
+template <typename T>
+hipError_t enqueue_march(tcsem_ctx *c, const MarchArgs<T> &a, int nb, hipStream_t st, hipEvent_t wait,
+                         const hipEvent_t *te) {
+  static const hipEvent_t none[4] = {};
+  if (!te) te = none;
+  const bool split = c->g.info.N >= 6;
+  hipError_t e;
+  if (split) {
+    MarchArgs<T> lo = a;
+    lo.k_lo = 2;
+    lo.carry = 2;
+    const bool chunked = c->nchunk > 1;
+    if (chunked) {
+      lo.nchunk = c->nchunk;
+      lo.chunk_len = c->chunk_len;
+      if ((e = launch_zsum<T>(lo, nb, st, te[0])) != hipSuccess) return e;
+    }
+    if ((e = launch_march<T>(lo, nb, st, chunked ? nullptr : te[0], te[1])) != hipSuccess) return e;
+  }
+  if (wait && (e = hipStreamWaitEvent(st, wait, 0)) != hipSuccess) return e;
+  MarchArgs<T> hi = a;
+  if (split) {
+    hi.k_hi = 1;
+    hi.carry = 1;
+  }
+  return launch_march<T>(hi, nb, st, te[2], te[3]);
+}
+
 template <typename T>
 int do_step(tcsem_ctx *c) {
   const tcsem_setup &s = c->setup;
@@ -1012,27 +1091,7 @@ int do_step(tcsem_ctx *c) {
     for (auto &e : te)
       if (!(e = timing_event(c))) return fail(TCSEM_EDEVICE, "timing event creation failed");
   }
-  if (split) {
-    MarchArgs<T> lo = a;
-    lo.k_lo = 2;
-    lo.carry = 2;
-    const bool chunked = c->nchunk > 1;
-    if (chunked) {
-      lo.nchunk = c->nchunk;
-      lo.chunk_len = c->chunk_len;
-      HIPCHK(launch_zsum<T>(lo, nb, st, te[0]));
-    }
-    HIPCHK(launch_march<T>(lo, nb, st, chunked ? nullptr : te[0], te[1]));
-  }
-  HIPCHK(hipStreamWaitEvent(st, c->xev[1], 0));
-  {
-    MarchArgs<T> hi = a;
-    if (split) {
-      hi.k_hi = 1;
-      hi.carry = 1;
-    }
-    HIPCHK(launch_march<T>(hi, nb, st, te[2], te[3]));
-  }
+  HIPCHK(enqueue_march<T>(c, a, nb, st, c->xev[1], te));
   if (c->has_fix && a.amp != 0.0)
     HIPCHK(launch_src_fix<T>(a, c->d_fixe, c->d_fixidx, c->d_fixb, c->d_fixidx + 2 * nb, nb, st));
   HIPCHK(hipEventRecord(c->xev[0], st));
@@ -1151,6 +1210,77 @@ int check_ctx(const tcsem_ctx *c) {
 // ==================================================================================================
 // C ABI
 // ==================================================================================================
+// Times nbatch-slot steps as they run in production -- the volume march on the main stream
+// beside the upward continuation and surface level of the previous step on the second stream,
+// on zeroed fields without a source -- for each candidate march tiling, and returns the fastest
+// (its time per step).  Timing the march alone would favour tilings that lose more bandwidth
+// to the concurrent air-plane chain.
+template <typename T>
+int autotune_march(tcsem_ctx *c, int nbatch, int nsteps, int *best_by, int *best_pf, int *best_nch, double *best_ms) {
+  const int N = c->g.info.N, nlev = N - 2;
+  std::vector<int> rows_c = c->prec == 64 ? std::vector<int>{4, 8} : std::vector<int>{4, 8, 16};
+  std::vector<int> nch_c = {1};
+  if (N >= 6)
+    for (int z : {2, 4, 8})
+      if (z <= std::max(1, nlev / 8)) nch_c.push_back(z);
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(TCSEM_EDEVICE, "event creation failed");
+  }
+  hipStream_t st = c->stream, st2 = c->stream2;
+  const UpcontArgs<T> &u = up_of<T>(c);
+  const int cur0 = c->cur;
+  auto one_step = [&]() -> hipError_t {
+    MarchArgs<T> a = march_args<T>(c);
+    a.lam = 1.0;
+    a.dt = 1e-6;
+    a.dtb = 1e-6;
+    a.amp = 0.0;
+    hipError_t e = enqueue_march<T>(c, a, nbatch, st, c->xev[1], nullptr);
+    if (e == hipSuccess) e = hipEventRecord(c->xev[0], st);
+    if (e == hipSuccess) e = hipStreamWaitEvent(st2, c->xev[0], 0);
+    if (e == hipSuccess) e = launch_upcont<T>(u, nbatch, st2, nullptr, nullptr);
+    if (e == hipSuccess) e = launch_surface<T>(a, nbatch, st2, nullptr, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(c->xev[1], st2);
+    c->cur ^= 1;
+    return e;
+  };
+  double best = 1e300;
+  int rc = TCSEM_OK;
+  for (int by : rows_c)
+    for (int pf = 1; pf <= 4 && rc == TCSEM_OK; ++pf)
+      for (int nch : nch_c) {
+        apply_tiling(c, by, pf, nch);
+        if (c->nchunk == 0 && nch > 1) continue;  // clamped to one pass: already timed
+        hipError_t e = one_step();               // untimed: first launches of this tiling
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, c->xev[1], 0);
+        if (e == hipSuccess) e = hipEventRecord(e0, st);
+        for (int q = 0; q < nsteps && e == hipSuccess; ++q) e = one_step();
+        if (e == hipSuccess) e = hipStreamWaitEvent(st, c->xev[1], 0);
+        if (e == hipSuccess) e = hipEventRecord(e1, st);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) {
+          rc = fail(TCSEM_EDEVICE, "autotune: %s", hipGetErrorString(e));
+          break;
+        }
+        const double per = ms / nsteps;
+        if (per < best) {
+          best = per;
+          *best_by = c->by;
+          *best_pf = pf;
+          *best_nch = c->nchunk > 1 ? c->nchunk : 1;
+        }
+      }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->cur = cur0;
+  *best_ms = best;
+  return rc;
+}
+
 extern "C" {
 
 const char *tcsem_version(void) { return "tcsem-mi355x 0.1 (gfx950)"; }
@@ -1451,6 +1581,41 @@ int tcsem_create(const tcsem_setup *s, const double *cond, int precision, tcsem_
       }
     make_plan(L1, c->plx);
     make_plan(M1, c->ply);
+    // The tiled chain needs both working lengths <= 2048 (one wavefront per line, two LDS buffers)
+    // and stages an irregular row pair (L) or column (M) in one line buffer of m + 2 entries.
+    const char *chain = std::getenv("TCSEM_UPCONT_CHAIN");
+    const bool want_lines = chain && std::strcmp(chain, "lines") == 0;
+    c->up_tiled = !want_lines && c->plx.m <= 2048 && c->ply.m <= 2048 && L <= c->plx.m + 2 && M <= c->ply.m + 2;
+    if (c->up_tiled) {
+      // pitches are whole 128-byte lines and avoid multiples of 1 KiB (HBM channel spread)
+      auto pitch = [](int n, int per_line) {
+        int p = (n + per_line - 1) / per_line * per_line;
+        if (p % (8 * per_line) == 0) p += per_line;
+        return p;
+      };
+      auto slot_stride = [](long long n) {
+        n = (n + 31) / 32 * 32;
+        if (n % 2048 == 0) n += 32;
+        return n;
+      };
+      c->MA = pitch(M, 8);    // complex
+      c->MG = pitch(M1, 4);   // (Gx, Gy) pairs
+      c->XR = pitch(L1, 16);  // doubles
+      c->sA = slot_stride(2LL * H * c->MA);
+      c->sG = slot_stride(4LL * H * c->MG);
+      c->sX = slot_stride((long long)M1 * c->XR);
+      c->gxT.assign((size_t)H * M1, 0.0);
+      c->gyT.assign((size_t)H * M1, 0.0);
+      for (int ky = 0; ky < M1; ++ky)
+        for (int kx = 0; kx < H; ++kx) {
+          double gxv = c->gx[(size_t)ky * H + kx], gyv = c->gy[(size_t)ky * H + kx];
+          if (kx == 0 && ky == 0) gxv = gyv = 0.0;
+          if (L1 % 2 == 0 && 2 * kx == L1) gxv = 0.0;
+          if (M1 % 2 == 0 && 2 * ky == M1 && (kx == 0 || (L1 % 2 == 0 && 2 * kx == L1))) gyv = 0.0;
+          c->gxT[(size_t)kx * M1 + ky] = gxv;
+          c->gyT[(size_t)kx * M1 + ky] = gyv;
+        }
+    }
   }
   const size_t lds_max = 160 * 1024 - 1024;
   // the march's masked stores go through a buffer resource per slot field that must stay below
@@ -1543,6 +1708,8 @@ int tcsem_create(const tcsem_setup *s, const double *cond, int precision, tcsem_
       up.insert(up.end(), v->begin(), v->end());
     for (auto *P : {&c->plx, &c->ply})
       for (auto *v : {&P->tw, &P->chirp, &P->bhat}) up.insert(up.end(), v->begin(), v->end());
+    up.insert(up.end(), c->gxT.begin(), c->gxT.end());
+    up.insert(up.end(), c->gyT.begin(), c->gyT.end());
     if (hipMalloc((void **)&c->d_up, up.size() * 8) != hipSuccess) return bail(fail(TCSEM_ENOMEM, "operator allocation"));
     if (hipMemcpy(c->d_up, up.data(), up.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
       return bail(fail(TCSEM_EDEVICE, "operator upload"));
@@ -1867,6 +2034,33 @@ int tcsem_step(tcsem_ctx *c, int nsteps) {
     if (rc) return rc;
   }
   return join_streams(c);
+}
+
+int tcsem_autotune(tcsem_ctx *c, int nbatch, int nsteps, int *rows, int *pf, int *zchunks, double *ms) {
+  if (check_ctx(c)) return TCSEM_EINVAL;
+  if (nbatch < 1 || nsteps < 1 || nsteps > 1000) return fail(TCSEM_EINVAL, "batch >= 1 and 1 <= steps <= 1000");
+  int rc = join_streams(c);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if ((rc = tcsem_reserve_batch(c, nbatch))) return rc;
+  free_run(c);  // a begun run is discarded; tcsem_begin / tcsem_run start a new one
+  if ((rc = tcsem_reset_fields(c, 0, nbatch))) return rc;
+  int by = 0, p = 0, nch = 0;
+  double t = 0.0;
+  rc = c->prec == 64 ? autotune_march<double>(c, nbatch, nsteps, &by, &p, &nch, &t)
+                     : autotune_march<float>(c, nbatch, nsteps, &by, &p, &nch, &t);
+  if (rc) return rc;
+  if ((rc = tcsem_reset_fields(c, 0, nbatch))) return rc;
+  c->tune_nbatch = nbatch;
+  c->tune_by = by;
+  c->tune_pf = p;
+  c->tune_nch = nch;
+  choose_tiling(c, nbatch);
+  if (rows) *rows = by;
+  if (pf) *pf = p;
+  if (zchunks) *zchunks = nch;
+  if (ms) *ms = t;
+  return TCSEM_OK;
 }
 
 int tcsem_steps_remaining(const tcsem_ctx *c) {

@@ -116,13 +116,26 @@ struct UpcontArgs {
   int n1, n2, n3;          // spline window lengths
   const double *wl, *wr;   // x-window weights, tap-major: n1 x AnLx, n2 x AnRx
   const double *wf, *wb;   // y-window weights: Any x n3 (front, back)
+  // Chain selection: tiled = 1 runs the four-kernel tiled chain (both FFT working lengths
+  // <= 2048), 0 the six-kernel line chain.
+  int tiled;
+  // line chain work planes
   double *tx;              // per slot M x L1: Bz(0) resampled along x (irregular rows)
   double *ry;              // per slot M1 x L1: the y-padding rows of the regular plane
   double2 *cplx;           // per slot M1 x L1 complex work (column spectra / row results)
-  double *bxr, *byr;       // per slot M1 x L1 real results
-  long long wstride;       // elements between slots of the work planes
+  long long wstride;       // elements between slots of the line-chain work planes
+  // tiled chain work planes (slot strides in doubles)
+  double2 *A;              // per slot H x MA: x half-spectra of the irregular rows, kx-major
+  double2 *G;              // per slot H x MG pairs (Gx, Gy): y-inverse columns, kx-major
+  int MA, MG;
+  long long sA, sG;
+  // both chains: real results, rows of pitch XR, slot stride sX (doubles)
+  double *bxr, *byr;
+  int XR;
+  long long sX;
   FftPlan px, py;          // x (length L1) and y (length M1) plans
   const double *gx, *gy;   // filter gains on the half spectrum: M1 x (L1/2+1)
+  const double *gxT, *gyT; // the same, kx-major (H x M1), with the non-Hermitian bins zeroed
   int H;                   // L1/2+1
   // bilinear (UpContinuation_gpu.cu:42-110)
   const double *X, *Y, *Xn, *Yn, *dx, *dy;

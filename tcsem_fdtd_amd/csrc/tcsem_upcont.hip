@@ -1,7 +1,25 @@
 // tcsem_upcont.hip -- gfx950 kernels of the air boundary: upward continuation of Bz(0) to
 // BxAir/ByAir at z = -dz0/2 (UpContinuation_gpu.cu:166-274).
 //
-// The chain is linear in Bz(0) and runs in FP64 whatever the field precision:
+// The operator is linear in Bz(0) and runs in FP64 whatever the field precision.  Two chains
+// compute it; both end in k_up_bilin.
+//
+// Tiled chain (both FFT working lengths <= 2048), four kernels, every global access coalesced:
+//   k_up_rows_fwd  one wavefront per pair of irregular rows: the natural-spline IR->R resampling
+//                  along x (SplineGridConvIR2R.cu:539-752, window weights tap-major) of both rows
+//                  into one complex line, its FFT along x, and the split into the two real-input
+//                  half spectra X(k) = (Z(k) + conj Z(-k)) / 2, (Z(k) - conj Z(-k)) / 2i; the
+//                  workgroup's rows leave through LDS as contiguous runs of a kx-major plane;
+//   k_up_cols      one wavefront per half-spectrum column kx (contiguous in that plane): the
+//                  y-padding rows of the regular plane as window sums of the irregular rows, the
+//                  FFT along y, and the gains of FBz_to_FBxFBy (UpContinuation_gpu.cu:113-164)
+//                  with the bins that are not Hermitian-consistent zeroed (below); the column
+//                  spectrum stays in registers while the two inverse FFTs along y run, and
+//                  (Gx, Gy) go out as one contiguous column;
+//   k_up_rows_inv  one wavefront per regular row: a tile of rows is read kx-major, each row is
+//                  completed to the full spectrum Gx + i Gy by Hermitian symmetry (kx and L1 - kx),
+//                  and one inverse FFT along x gives Bx = Re, By = Im.
+// Line chain (any length), six kernels:
 //   k_up_xpass  one workgroup per irregular row: natural-spline IR->R resampling along x
 //               (SplineGridConvIR2R.cu:539-752) as precomputed window weights, tap-major so that
 //               lanes over columns read them coalesced; the row is staged in LDS;
@@ -21,7 +39,8 @@
 // already Hermitian-consistent are: the DC bin, the whole x-Nyquist column of the x gain (even in
 // ky, so its Hermitian part is zero) and the y-Nyquist bins of the y gain at kx = 0 and
 // kx = L1/2.  With those gains set to zero both spectra are exactly Hermitian, both inverses are
-// real, and one complex inverse of FBx + i FBy yields Bx and By at once.
+// real, and one complex inverse of FBx + i FBy yields Bx and By at once.  (The tiled chain takes
+// the real part at kx = 0 and kx = L1/2 of each row, as a real-output inverse does.)
 // FFTs: working lengths up to 2048 run Stockham radix-8/4/2 passes between two LDS buffers with
 // the twiddle table staged in LDS, one wavefront per line; longer ones run in-place radix-2 with
 // 256 threads per line and the table in global memory.  Lengths that are not powers of two use
@@ -352,12 +371,196 @@ __global__ void __launch_bounds__(256) k_up_ycols2(const UpcontArgs<T> u) {
   }
 }
 
+// ---- tiled chain ---------------------------------------------------------------------------
+// LDS of a tiled kernel: `lines` line regions of two buffers of m + 2 entries each (the two extra
+// entries hold the Nyquist bin of an unpacked half spectrum and shift the lines across LDS
+// banks), then the m-entry twiddle table.
+__host__ __device__ inline int tile_span(int m) { return m + 2; }
+__host__ __device__ inline size_t tile_lds(int m, int lines) {
+  return ((size_t)lines * 2 * tile_span(m) + m) * sizeof(double2);
+}
+
+__device__ inline const double2 *stage_twiddles(double2 *dst, const FftPlan &P) {
+  const double2 *g = (const double2 *)P.tw;
+  const int flat = threadIdx.y * blockDim.x + threadIdx.x, nt = blockDim.x * blockDim.y;
+  for (int q = flat; q < P.m; q += nt) dst[q] = g[q];
+  return dst;
+}
+
+// Regular column c of one irregular row staged in LDS (the x pass of the IR->R resampling).
+template <typename T>
+__device__ inline double resample_x(const UpcontArgs<T> &u, const double *srow, int c) {
+  const int aL = u.AnLx;
+  if (c >= aL && c < aL + u.Nx) return srow[u.AL + c - aL];
+  double v = 0.0;
+  if (c < aL) {
+#pragma unroll 8
+    for (int q = 0; q < u.n1; ++q) v += u.wl[(long long)q * aL + c] * srow[q];
+  } else {
+    const int cr = c - aL - u.Nx, ox = u.Nx + u.AL - kNPad;
+#pragma unroll 8
+    for (int q = 0; q < u.n2; ++q) v += u.wr[(long long)q * u.AnRx + cr] * srow[ox + q];
+  }
+  return v;
+}
+
+// blockDim (W, P), W = 64, 128 or 256 threads per line: line p transforms irregular rows
+// j0 = 2 (blockIdx.x P + p) and j0 + 1.
+template <typename T>
+__global__ void __launch_bounds__(256) k_up_rows_fwd(const UpcontArgs<T> u) {
+  extern __shared__ double2 lds[];
+  const int lane = threadIdx.x, W = blockDim.x, p = threadIdx.y, np = blockDim.y, slot = blockIdx.y;
+  const int m = u.px.m, span = tile_span(m), L1 = u.L1, L = u.L, M = u.M;
+  double2 *a = lds + (long long)p * 2 * span, *b = a + span;
+  const double2 *tw = stage_twiddles(lds + (long long)np * 2 * span, u.px);
+  const int jb = blockIdx.x * 2 * np, j0 = jb + 2 * p;
+  // the two rows are staged in the second buffer (L <= m + 2 doubles each, checked on the host)
+  double *s0 = (double *)b, *s1 = s0 + L;
+  const T *plane = u.bz0 + (long long)slot * u.pstride;
+  for (int i = lane; i < L; i += W) {
+    s0[i] = j0 < M ? (double)plane[(long long)j0 * u.PX + i] : 0.0;
+    s1[i] = j0 + 1 < M ? (double)plane[(long long)(j0 + 1) * u.PX + i] : 0.0;
+  }
+  __syncthreads();
+  for (int c = lane; c < L1; c += W) a[c] = make_double2(resample_x(u, s0, c), resample_x(u, s1, c));
+  __syncthreads();
+  double2 *f = fft_line<true>(a, b, u.px, tw, -1, lane, W);
+  double2 *o = (f == a) ? b : a;
+  for (int k = lane; k < u.H; k += W) {
+    const double2 z = f[k], w = f[k == 0 ? 0 : L1 - k];
+    o[2 * k] = make_double2(0.5 * (z.x + w.x), 0.5 * (z.y - w.y));       // (Z(k) + conj Z(-k)) / 2
+    o[2 * k + 1] = make_double2(0.5 * (z.y + w.y), -0.5 * (z.x - w.x));  // (Z(k) - conj Z(-k)) / 2i
+  }
+  __syncthreads();
+  // out through LDS: for each kx the workgroup's 2 np rows are one contiguous run
+  const int R = 2 * np, ooff = (f == a) ? span : 0;
+  double2 *A = u.A + (long long)slot * (u.sA / 2);
+  for (int idx = p * W + lane; idx < u.H * R; idx += np * W) {
+    const int kx = idx / R, r = idx - kx * R, j = jb + r;
+    if (j < M) A[(long long)kx * u.MA + j] = lds[(long long)(r >> 1) * 2 * span + ooff + 2 * kx + (r & 1)];
+  }
+}
+
+// blockDim (W, Q), W = 64, 128 or 256 threads per line: line q owns half-spectrum column
+// kx = blockIdx.x Q + q.  E >= M1 / W: column entries kept in registers per thread.
+template <typename T, int E>
+__global__ void __launch_bounds__(256) k_up_cols(const UpcontArgs<T> u) {
+  extern __shared__ double2 lds[];
+  const int lane = threadIdx.x, W = blockDim.x, q = threadIdx.y, nq = blockDim.y, slot = blockIdx.y;
+  const int m = u.py.m, span = tile_span(m), M1 = u.M1, M = u.M;
+  double2 *a = lds + (long long)q * 2 * span, *b = a + span;
+  const double2 *tw = stage_twiddles(lds + (long long)nq * 2 * span, u.py);
+  const int kx = blockIdx.x * nq + q;
+  const bool live = kx < u.H;
+  const int kc = live ? kx : 0;
+  const double2 *Acol = u.A + (long long)slot * (u.sA / 2) + (long long)kc * u.MA;
+  for (int j = lane; j < M; j += W) b[j] = live ? Acol[j] : make_double2(0.0, 0.0);
+  __syncthreads();
+  // the regular column: core rows are irregular rows, padding rows the y-window sums
+  // (the y pass of SplineGridConvIR2R.cu:539-752)
+  const int y0 = u.Any, y1 = u.Any + u.Ny, sh = u.AY - u.Any, ob = u.Ny + u.AY - kNPad;
+  for (int r = lane; r < M1; r += W) {
+    double2 v;
+    if (r >= y0 && r < y1) {
+      v = b[r + sh];
+    } else {
+      const double *w = r < y0 ? u.wf + (long long)r * u.n3 : u.wb + (long long)(r - y1) * u.n3;
+      const double2 *s = r < y0 ? b : b + ob;
+      double vx = 0.0, vy = 0.0;
+#pragma unroll 8
+      for (int t = 0; t < u.n3; ++t) {
+        const double wt = w[t];
+        vx += wt * s[t].x;
+        vy += wt * s[t].y;
+      }
+      v = make_double2(vx, vy);
+    }
+    a[r] = v;
+  }
+  __syncthreads();
+  double2 *f = fft_line<true>(a, b, u.py, tw, -1, lane, W);
+  double2 *o = (f == a) ? b : a;
+  const double *gx = u.gxT + (long long)kc * M1, *gy = u.gyT + (long long)kc * M1;
+  double2 F[E], Gx[E];
+  // F stays in registers; each lane rewrites only its own entries
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int ky = lane + W * e;
+    F[e] = ky < M1 ? f[ky] : make_double2(0.0, 0.0);
+    if (ky < M1) f[ky] = muli(F[e], gx[ky]);  // FBx = i gx F
+  }
+  __syncthreads();
+  double2 *rx = fft_line<true>(f, o, u.py, tw, +1, lane, W);
+  double2 *o2 = (rx == f) ? o : f;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    const int ky = lane + W * e;
+    Gx[e] = ky < M1 ? rx[ky] : make_double2(0.0, 0.0);
+    if (ky < M1) rx[ky] = muli(F[e], gy[ky]);  // FBy = i gy F
+  }
+  __syncthreads();
+  double2 *ry = fft_line<true>(rx, o2, u.py, tw, +1, lane, W);
+  if (live) {
+    double2 *Gc = u.G + (long long)slot * (u.sG / 2) + (long long)kx * u.MG * 2;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+      const int y = lane + W * e;
+      if (y < M1) {
+        Gc[2 * y] = Gx[e];
+        Gc[2 * y + 1] = ry[y];
+      }
+    }
+  }
+}
+
+// blockDim (W, Q), W = 64, 128 or 256 threads per line: line q inverts regular row
+// y = blockIdx.x Q + q along x.
+template <typename T>
+__global__ void __launch_bounds__(256) k_up_rows_inv(const UpcontArgs<T> u) {
+  extern __shared__ double2 lds[];
+  const int lane = threadIdx.x, W = blockDim.x, q = threadIdx.y, nq = blockDim.y, slot = blockIdx.y;
+  const int m = u.px.m, span = tile_span(m), L1 = u.L1, H = u.H, M1 = u.M1;
+  const double2 *tw = stage_twiddles(lds + (long long)nq * 2 * span, u.px);
+  const int yb = blockIdx.x * nq;
+  const double2 *G = u.G + (long long)slot * (u.sG / 2);
+  const bool evenL = (L1 & 1) == 0;
+  // tile in: for each kx the nq rows are one contiguous run of (Gx, Gy) pairs
+  for (int idx = q * W + lane; idx < H * nq; idx += nq * W) {
+    const int kx = idx / nq, r = idx - kx * nq, y = yb + r;
+    double2 gx = make_double2(0.0, 0.0), gy = gx;
+    if (y < M1) {
+      gx = G[((long long)kx * u.MG + y) * 2];
+      gy = G[((long long)kx * u.MG + y) * 2 + 1];
+    }
+    double2 *row = lds + (long long)r * 2 * span;
+    if (kx == 0 || (evenL && 2 * kx == L1)) {
+      row[kx] = make_double2(gx.x, gy.x);  // self-conjugate bins: real parts
+    } else {
+      row[kx] = make_double2(gx.x - gy.y, gx.y + gy.x);       // Gx + i Gy
+      row[L1 - kx] = make_double2(gx.x + gy.y, gy.x - gx.y);  // conj Gx + i conj Gy
+    }
+  }
+  __syncthreads();
+  double2 *a = lds + (long long)q * 2 * span, *b = a + span;
+  double2 *z = fft_line<true>(a, b, u.px, tw, +1, lane, W);
+  const int y = yb + q;
+  if (y < M1) {
+    double *bx = u.bxr + (long long)slot * u.sX + (long long)y * u.XR;
+    double *by = u.byr + (long long)slot * u.sX + (long long)y * u.XR;
+    for (int x = lane; x < L1; x += W) {
+      const double2 v = z[x];
+      bx[x] = v.x;
+      by[x] = v.y;
+    }
+  }
+}
+
 template <typename T>
 __global__ void k_up_bilin(const UpcontArgs<T> u) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x, slot = blockIdx.y;
-  const int L0 = u.L, M0 = u.M, L1 = u.L1, M1 = u.M1;
-  const double norm = (double)(L1 * M1);
-  const long long wo = (long long)slot * u.wstride, po = (long long)slot * u.pstride;
+  const int L0 = u.L, M0 = u.M, L1 = u.L1, M1 = u.M1, XR = u.XR;
+  const double norm = (double)u.L1 * (double)M1;
+  const long long wo = (long long)slot * u.sX, po = (long long)slot * u.pstride;
   const int nbx = M0 * (L0 + 1);
   if (t < nbx) {
     const int iy = t / (L0 + 1), ix = t % (L0 + 1);
@@ -365,8 +568,8 @@ __global__ void k_up_bilin(const UpcontArgs<T> u) {
     const double Xp = (ix == L0) ? u.X[L0 - 1] + 0.5 * u.dx[L0 - 1] : u.X[ix] - 0.5 * u.dx[ix];
     const double *F = u.bxr + wo;
     double l1 = (Xp - u.Xn[II - 1]) / (u.Xn[II] - u.Xn[II - 1]);
-    const double a0 = (1.0 - l1) * F[(long long)(JJ - 1) * L1 + II - 1] + l1 * F[(long long)(JJ - 1) * L1 + II];
-    const double a1 = (1.0 - l1) * F[(long long)JJ * L1 + II - 1] + l1 * F[(long long)JJ * L1 + II];
+    const double a0 = (1.0 - l1) * F[(long long)(JJ - 1) * XR + II - 1] + l1 * F[(long long)(JJ - 1) * XR + II];
+    const double a1 = (1.0 - l1) * F[(long long)JJ * XR + II - 1] + l1 * F[(long long)JJ * XR + II];
     l1 = (u.Y[iy] - u.Yn[JJ - 1]) / (u.Yn[JJ] - u.Yn[JJ - 1]);
     u.bxair[po + (long long)iy * u.PX + ix] = (T)(((1.0 - l1) * a0 + l1 * a1) / norm);
   } else if (t < nbx + (M0 + 1) * L0) {
@@ -376,8 +579,8 @@ __global__ void k_up_bilin(const UpcontArgs<T> u) {
     const double Yp = (iy == M0) ? u.Y[M0 - 1] + 0.5 * u.dy[M0 - 1] : u.Y[iy] - 0.5 * u.dy[iy];
     const double *F = u.byr + wo;
     double l1 = (u.X[ix] - u.Xn[II - 1]) / (u.Xn[II] - u.Xn[II - 1]);
-    const double a0 = (1.0 - l1) * F[(long long)(JJ - 1) * L1 + II - 1] + l1 * F[(long long)(JJ - 1) * L1 + II];
-    const double a1 = (1.0 - l1) * F[(long long)JJ * L1 + II - 1] + l1 * F[(long long)JJ * L1 + II];
+    const double a0 = (1.0 - l1) * F[(long long)(JJ - 1) * XR + II - 1] + l1 * F[(long long)(JJ - 1) * XR + II];
+    const double a1 = (1.0 - l1) * F[(long long)JJ * XR + II - 1] + l1 * F[(long long)JJ * XR + II];
     l1 = (Yp - u.Yn[JJ - 1]) / (u.Yn[JJ] - u.Yn[JJ - 1]);
     u.byair[po + (long long)iy * u.PX + ix] = (T)(((1.0 - l1) * a0 + l1 * a1) / norm);
   }
@@ -410,10 +613,63 @@ hipError_t launch_lines(K kern, const LineShape &sh, int nlines, int nslot, hipS
   hipLaunchKernelGGL(kern, dim3((nlines + sh.lines - 1) / sh.lines, nslot), dim3(sh.threads, sh.lines), sh.lds, s, u);
   return hipGetLastError();
 }
-}  // namespace
+
+// Launch shape of a tiled kernel: with enough lines for several waves per SIMD, one wavefront per
+// line and 4 (then 2) lines per workgroup while two such workgroups fit a CU's LDS; with few lines
+// (a single slot), one line per workgroup of 256 threads, so that the resampling sums, the
+// butterflies and the transfers of a line spread over four wavefronts.
+struct Tile {
+  int w, q;
+  size_t lds;
+};
+Tile tile_shape(int m, long long nlines) {
+  constexpr size_t kHalfCu = 80 * 1024;
+  if (nlines >= 1024) {
+    if (tile_lds(m, 4) <= kHalfCu) return {64, 4, tile_lds(m, 4)};
+    if (tile_lds(m, 2) <= kHalfCu) return {64, 2, tile_lds(m, 2)};
+    return {64, 1, tile_lds(m, 1)};
+  }
+  return {256, 1, tile_lds(m, 1)};
+}
+
+template <typename T, int E>
+hipError_t launch_cols_e(const UpcontArgs<T> &u, const Tile &t, int nslot, hipStream_t s) {
+  hipError_t e = allow_lds(k_up_cols<T, E>, t.lds);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL((k_up_cols<T, E>), dim3((u.H + t.q - 1) / t.q, nslot), dim3(t.w, t.q), t.lds, s, u);
+  return hipGetLastError();
+}
 
 template <typename T>
-hipError_t launch_upcont(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+hipError_t launch_cols(const UpcontArgs<T> &u, int nslot, hipStream_t s) {
+  const Tile t = tile_shape(u.py.m, (long long)u.H * nslot);
+  const int need = (u.M1 + t.w - 1) / t.w;  // column entries per thread
+  if (need <= 1) return launch_cols_e<T, 1>(u, t, nslot, s);
+  if (need <= 2) return launch_cols_e<T, 2>(u, t, nslot, s);
+  if (need <= 4) return launch_cols_e<T, 4>(u, t, nslot, s);
+  if (need <= 8) return launch_cols_e<T, 8>(u, t, nslot, s);
+  if (need <= 16) return launch_cols_e<T, 16>(u, t, nslot, s);
+  return launch_cols_e<T, 32>(u, t, nslot, s);
+}
+
+template <typename T>
+hipError_t launch_tiled(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEvent_t e0) {
+  const int npair = (u.M + 1) / 2;
+  Tile t = tile_shape(u.px.m, (long long)npair * nslot);
+  hipError_t e = allow_lds(k_up_rows_fwd<T>, t.lds);
+  if (e != hipSuccess) return e;
+  hipExtLaunchKernelGGL((k_up_rows_fwd<T>), dim3((npair + t.q - 1) / t.q, nslot), dim3(t.w, t.q),
+                        (std::uint32_t)t.lds, s, e0, nullptr, 0u, u);
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = launch_cols<T>(u, nslot, s)) != hipSuccess) return e;
+  t = tile_shape(u.px.m, (long long)u.M1 * nslot);
+  if ((e = allow_lds(k_up_rows_inv<T>, t.lds)) != hipSuccess) return e;
+  hipLaunchKernelGGL((k_up_rows_inv<T>), dim3((u.M1 + t.q - 1) / t.q, nslot), dim3(t.w, t.q), t.lds, s, u);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_lined(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEvent_t e0) {
   hipError_t e;
   hipExtLaunchKernelGGL((k_up_xpass<T>), dim3(u.M, nslot), dim3(256), (std::uint32_t)(u.L * sizeof(double)), s, e0,
                         nullptr, 0u, u);
@@ -434,6 +690,13 @@ hipError_t launch_upcont(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEv
   if (e != hipSuccess) return e;
   e = ppy ? launch_lines(k_up_ycols2<T, true>, sc, u.L1, nslot, s, u)
           : launch_lines(k_up_ycols2<T, false>, sc, u.L1, nslot, s, u);
+  return e;
+}
+}  // namespace
+
+template <typename T>
+hipError_t launch_upcont(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  hipError_t e = u.tiled ? launch_tiled<T>(u, nslot, s, e0) : launch_lined<T>(u, nslot, s, e0);
   if (e != hipSuccess) return e;
   const int n = u.M * (u.L + 1) + (u.M + 1) * u.L;
   hipExtLaunchKernelGGL((k_up_bilin<T>), dim3((n + 255) / 256, nslot), dim3(256), 0u, s, nullptr, e1, 0u, u);

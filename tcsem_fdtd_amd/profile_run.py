@@ -2,8 +2,12 @@
 
     python -m tcsem_fdtd_amd.profile_run --workload marine_3d --precision 32 --batch 1 --steps 20
 
-Builds the deck, advances `warmup` untimed steps, then `steps` steps, and prints one JSON line with
-the wall time per step.  Used by the rocprof summaries under profiles/.
+Builds the deck, runs the library's march autotuner for the batch size (as bench.py does; its
+trial launches precede the first k_surface dispatch), advances `warmup` untimed steps, then
+`steps` steps, and prints one JSON line with the chosen tiling and the wall time per step.
+--every-step-rx replaces the deck's sample times by one sample per step from the first step on,
+so that every step runs the receiver gathers of all receivers and components.  Used by the
+rocprof summaries under profiles/.
 """
 import argparse
 import json
@@ -15,6 +19,7 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 
 
 def main():
+    import numpy as np
     import tcsem_fdtd_amd as T
     from tcsem_fdtd_amd import workloads as W
     ap = argparse.ArgumentParser()
@@ -23,10 +28,22 @@ def main():
     ap.add_argument("--batch", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--every-step-rx", action="store_true")
     a = ap.parse_args()
     deck = W.towline(T, ntx=a.batch) if a.workload == "towline" else getattr(W, a.workload)(T)
     with T.Context(T.make_setup(**deck.setup), deck.cond, a.precision) as c:
-        r = deck.rxtime
+        tiling = None if a.no_tune else c.autotune(a.batch, 5)
+        r = dict(deck.rxtime)
+        if a.every_step_rx:
+            # linear samples about four per step over the run (the source sub-steps), so that
+            # (nearly) every step gathers E and B at every receiver
+            n = a.warmup + a.steps
+            ns = deck.setup["n_src"] // 2
+            if n > 2 * ns:
+                raise SystemExit("--every-step-rx needs warmup + steps <= %d (the source sub-steps)" % (2 * ns))
+            t_end = float(np.cumsum(T.waveform(ns, deck.setup["sigma_t"])[1])[n - 1])
+            r.update(T0=1e-12, Tn=t_end, Nt=4 * n, is_log=0)
         c.set_receivers(T.make_rxtime(r["T0"], r["Tn"], r["Nt"], r["is_log"], r["comps"]), deck.rx)
         c.set_transmitters([T.make_tx(**dict(t, txid=i + 1)) for i, t in enumerate(deck.txs)])
         c.begin(0, a.batch)
@@ -38,7 +55,7 @@ def main():
         dt = time.perf_counter() - t0
         print(json.dumps({"workload": deck.name, "precision": a.precision, "batch": a.batch,
                           "steps": a.steps, "ms_per_step": 1000 * dt / a.steps,
-                          "cells": c.L * c.M * c.N}), flush=True)
+                          "cells": c.L * c.M * c.N, "march_tiling": tiling}), flush=True)
 
 
 if __name__ == "__main__":

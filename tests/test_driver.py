@@ -99,3 +99,22 @@ def test_driver_batch_size_does_not_change_results():
             assert r.returncode == 0, r.stderr
             outs.append([open(os.path.join(d, "Tx%d.bout" % (b + 1)), "rb").read() for b in range(2)])
     assert outs[0] == outs[1]
+
+
+@pytest.mark.gpu
+def test_driver_autotune_keeps_results():
+    """--autotune may pick another march tiling (z-chunks reorder the FP64 Bz sums): the traces
+    stay at round-off from the heuristic tiling's."""
+    deck = _small_deck()
+    outs = []
+    for extra in ([], ["--autotune"]):
+        with tempfile.TemporaryDirectory() as d:
+            deck.write(d)
+            r = _run(["--dir", d, "--batch", "2"] + extra)
+            assert r.returncode == 0, r.stderr
+            if extra:
+                assert "autotuned march" in r.stdout, r.stdout
+            outs.append([T.read_bout(os.path.join(d, "Tx%d.bout" % (b + 1))) for b in range(2)])
+    for got, ref in zip(outs[1], outs[0]):
+        err = decks.trace_error(got, ref)
+        assert np.all(err < 1e-10), err

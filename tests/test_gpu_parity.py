@@ -72,6 +72,70 @@ def test_upcont_plane_bluestein_fp64():
     assert decks.rel_l2(by, byo) < 1e-11
 
 
+# Decks whose regular air planes have power-of-two sides (the Stockham FFT path, not Bluestein):
+# 128 x 64 (L = 82, M = 18) and 64 x 128 (L = 20, M = 84), so that a transposed x/y pass shows.
+POW2_DECKS = {
+    "128x64": {"nx": 72, "ny": 8, "pad": (2, 3), "arg": (1.5, 2.0)},
+    "64x128": {"nx": 10, "ny": 74, "pad": (3, 2), "arg": (1.3, 2.0)},
+}
+
+
+@pytest.mark.parametrize("name", sorted(POW2_DECKS))
+def test_upcont_plane_pow2_fp64(name):
+    so, st, m, cond = _pair(POW2_DECKS[name])
+    assert "%dx%d" % (m.L1, m.M1) == name
+    rng = np.random.default_rng(3)
+    bz0 = rng.standard_normal((m.M, m.L))
+    bxo, byo = m.upcont_plane(bz0)
+    with T.Context(st, cond, T.FP64) as c:
+        bx, by = c.upcont_plane(bz0)
+    assert decks.rel_l2(bx, bxo) < 1e-12
+    assert decks.rel_l2(by, byo) < 1e-12
+
+
+def _upcont_check(kw, seed, tol=1e-12):
+    so, st, m, cond = _pair(kw)
+    bz0 = np.random.default_rng(seed).standard_normal((m.M, m.L))
+    bxo, byo = m.upcont_plane(bz0)
+    with T.Context(st, cond, T.FP64) as c:
+        bx, by = c.upcont_plane(bz0)
+    assert decks.rel_l2(bx, bxo) < tol
+    assert decks.rel_l2(by, byo) < tol
+    return m
+
+
+@pytest.mark.parametrize("kw", [{"nx": 23, "ny": 17}, POW2_DECKS["128x64"]], ids=["bluestein", "128x64"])
+def test_upcont_plane_line_chain_fp64(kw, monkeypatch):
+    """The six-kernel line chain, forced where the tiled chain would run."""
+    monkeypatch.setenv("TCSEM_UPCONT_CHAIN", "lines")
+    _upcont_check(kw, 5, 1e-11)
+
+
+def test_upcont_plane_long_x_fp64():
+    """L1 = 1156: Bluestein working length 4096 along x, beyond the tiled chain (line chain with
+    in-place radix-2 lines of 256 threads)."""
+    m = _upcont_check({"nx": 1100, "ny": 8, "pad": (2, 3), "arg": (1.5, 2.0)}, 6, 1e-11)
+    assert m.L1 == 1156
+
+
+def test_upcont_plane_marine_1d_fp64():
+    """The 256 x 256 air plane of the 64 x 64 x 48 marine model (three radix-8/4 passes)."""
+    from tcsem_fdtd_amd import workloads as W
+    d = W.marine_1d(T)
+    so = O.make_setup(**d.setup)
+    m = O.Model(so, d.cond)
+    assert (m.L1, m.M1) == (256, 256)
+    g = m.grid()
+    X, Y = np.meshgrid(g["X"], g["Y"])
+    rng = np.random.default_rng(4)
+    bz0 = np.exp(-(X ** 2 + Y ** 2) / 1e6) + 0.01 * rng.standard_normal(X.shape)
+    bxo, byo = m.upcont_plane(bz0)
+    with T.Context(T.make_setup(**d.setup), d.cond, T.FP64) as c:
+        bx, by = c.upcont_plane(bz0)
+    assert decks.rel_l2(bx, bxo) < 1e-12
+    assert decks.rel_l2(by, byo) < 1e-12
+
+
 def _random_state(m, seed):
     rng = np.random.default_rng(seed)
     st = {}
@@ -131,6 +195,15 @@ def test_traces_fp32():
     ref, got = _deck_run(T.FP32, [TX_X])
     err = decks.trace_error(got[0], ref[0])
     assert np.all(err < FP32_TRACE_TOL), err
+
+
+@pytest.mark.parametrize("name", sorted(POW2_DECKS))
+@pytest.mark.parametrize("precision,tol", [(T.FP64, 1e-9), (T.FP32, FP32_TRACE_TOL)])
+def test_traces_pow2_decks(name, precision, tol):
+    ref, got = _deck_run(precision, [TX_X], nsteps=400, setup_kw=POW2_DECKS[name])
+    err = decks.trace_error(got[0], ref[0])
+    assert np.any(np.abs(ref[0]) > 0)
+    assert np.all(err < tol), err
 
 
 @pytest.mark.parametrize("tx", [
