@@ -3,7 +3,7 @@
     python -m tcsem_fdtd_amd.profile_run --workload marine_3d --precision 32 --batch 1 --steps 20
 
 Builds the deck, runs the library's march autotuner for the batch size (as bench.py does; its
-trial launches precede the first k_surface dispatch), advances `warmup` untimed steps, then
+trial steps precede the field resets of tcsem_begin, the last k_fill_zero dispatches), advances `warmup` untimed steps, then
 `steps` steps, and prints one JSON line with the chosen tiling and the wall time per step.
 --every-step-rx replaces the deck's sample times by one sample per step from the first step on,
 so that every step runs the receiver gathers of all receivers and components.  Used by the
