@@ -89,6 +89,11 @@ def lib():
         "orc_bilinear": (None, [P, dp, dp, dp, dp]),
         "orc_upcont_plane": (None, [P, dp, dp, dp]),
         "orc_upcont": (None, [P]),
+        "orc_spline_ir2r_cpu": (None, [P, dp, dp]),
+        "orc_upcont_filter_cpu": (None, [P, dp, dp, dp]),
+        "orc_bilinear_cpu": (None, [P, dp, dp, dp, dp]),
+        "orc_upcont_plane_cpu": (None, [P, dp, dp, dp]),
+        "orc_set_cpu_twins": (None, [P, C.c_int]),
         "orc_update_e": (None, [P, C.c_double, C.c_double]),
         "orc_update_b": (None, [P, C.c_double]),
         "orc_update_bz": (None, [P]),
@@ -223,6 +228,37 @@ class Model:
 
     def upcont(self):
         self.L_.orc_upcont(self.h)
+
+    # CPU twins of the air boundary (SplineGridConvIR2R_cpu.cpp, UpContinuation.cpp:70-97,
+    # BilinearGridConvR2IR.cpp)
+    def spline_ir2r_cpu(self, bz0):
+        bz0 = np.ascontiguousarray(bz0, dtype=np.float64)
+        out = np.zeros((self.M1, self.L1))
+        self.L_.orc_spline_ir2r_cpu(self.h, _dp(bz0), _dp(out))
+        return out
+
+    def upcont_filter_cpu(self, bzair):
+        bzair = np.ascontiguousarray(bzair, dtype=np.float64)
+        bx, by = np.zeros((self.M1, self.L1)), np.zeros((self.M1, self.L1))
+        self.L_.orc_upcont_filter_cpu(self.h, _dp(bzair), _dp(bx), _dp(by))
+        return bx, by
+
+    def bilinear_cpu(self, bxr, byr):
+        bxr = np.ascontiguousarray(bxr, dtype=np.float64)
+        byr = np.ascontiguousarray(byr, dtype=np.float64)
+        bxa, bya = np.zeros((self.M, self.L + 1)), np.zeros((self.M + 1, self.L))
+        self.L_.orc_bilinear_cpu(self.h, _dp(bxr), _dp(byr), _dp(bxa), _dp(bya))
+        return bxa, bya
+
+    def upcont_plane_cpu(self, bz0):
+        bz0 = np.ascontiguousarray(bz0, dtype=np.float64)
+        bxa, bya = np.zeros((self.M, self.L + 1)), np.zeros((self.M + 1, self.L))
+        self.L_.orc_upcont_plane_cpu(self.h, _dp(bz0), _dp(bxa), _dp(bya))
+        return bxa, bya
+
+    def set_cpu_twins(self, on=True):
+        """Steps (and run) take their air boundary from the CPU-twin chain."""
+        self.L_.orc_set_cpu_twins(self.h, int(on))
 
     def update_e(self, lam, dt):
         self.L_.orc_update_e(self.h, lam, dt)

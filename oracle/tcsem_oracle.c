@@ -12,12 +12,17 @@
  * receiver gathers and the host time-interpolation state machine.  Each function cites the
  * reference file:line it follows.  It implements the intended, race-free semantics (every
  * level reads same-step values).  The FFT is a plain radix-2 / Bluestein FFT written here
- * (FFTW/cuFFT are absent); tests pin it against numpy.fft.
+ * (FFTW/cuFFT are absent); tests pin it against numpy.fft.  It also restates the reference's CPU
+ * twins of the air boundary (SplineGridConvIR2R_cpu.cpp, the filter of UpContinuation.cpp,
+ * BilinearGridConvR2IR.cpp), selectable for the air boundary of a step, to compare the two
+ * formulations.
  *
  * Pins (see tests/test_oracle_*.py): testwaveform.dat (golden), closed-form grid invariants,
  * the reference's DEBUG==1 mapping bracket checks, scipy natural CubicSpline, numpy.fft,
  * exactness of bilinear / 4-point splines on linear data, div(B)=0, linearity, mirror
- * symmetry, the homogeneous DF decay factor, and whole-space/half-space closed forms.
+ * symmetry, the homogeneous DF decay factor, and whole-space/half-space closed forms; the CPU
+ * twins against scipy natural splines on their own node spacing and against the GPU formulation
+ * where the two coincide (filter with dx = dy, bilinear).
  */
 #include <math.h>
 #include <stdint.h>
@@ -48,6 +53,7 @@ typedef struct orc_model {
   double *cond;
   double *Ex, *Ey, *Ez, *Bx, *By, *Bz, *BxAir, *ByAir;
   double *T, *BzAir, *Fr, *Fi, *Gr, *Gi; /* up-continuation work */
+  int cpu_twins; /* 1: the air boundary of a step uses the CPU-twin chain */
 } orc_model;
 
 /* ModelingSetup.cpp:79-84,100-192 */
@@ -683,7 +689,254 @@ void orc_upcont_plane(orc_model *m, const double *bz0, double *bxair, double *by
   free(bxr); free(byr);
 }
 
-void orc_upcont(orc_model *m) { orc_upcont_plane(m, m->Bz, m->BxAir, m->ByAir); }
+/* ============================================================================================
+ * CPU twins of the air boundary (the reference's CPU formulation; parity spec, compared with the
+ * GPU formulation above per function): SplineGridConvIR2R_cpu.cpp:7-691, the filter of
+ * UpContinuation.cpp:70-97 and BilinearGridConvR2IR.cpp:8-352.
+ * ========================================================================================== */
+
+/* tk::spline coefficients of SplineGridConvIR2R_cpu on n nodes ys with the CPU spacing d[i]
+ * between nodes i and i+1 (the cell width dx_i, not the centre distance): natural ends (b = 0),
+ * tridiagonal system solved by elimination without pivoting (tk::band_matrix::lu_solve,
+ * Spline.h:221-231, for a 1-1 band); a, b, c of f_i(h) = a h^3 + b h^2 + c h + y_i, with the
+ * right end continued as f_{n-1}(h) = b h^2 + c h + y_{n-1}, c_{n-1} = f'_{n-2}(x_{n-1}).
+ * SplineGridConvIR2R_cpu.cpp:52-98. */
+static void cpu_tk_spline(int n, const double *d, const double *ys, double *a, double *b, double *c,
+                          double *work) {
+  double *dl = work, *dd = work + n, *du = work + 2 * n, *r = work + 3 * n;
+  for (int i = 0; i < n; i++) {
+    if (i == 0 || i == n - 1) {
+      dl[i] = 0.0; dd[i] = 2.0; du[i] = 0.0; r[i] = 0.0;
+    } else {
+      dl[i] = 1.0 / 3.0 * d[i - 1];
+      dd[i] = 2.0 / 3.0 * (d[i] + d[i - 1]);
+      du[i] = 1.0 / 3.0 * d[i];
+      r[i] = (ys[i + 1] - ys[i]) / d[i] - (ys[i] - ys[i - 1]) / d[i - 1];
+    }
+  }
+  for (int i = 1; i < n; i++) { /* forward elimination */
+    double f = dl[i] / dd[i - 1];
+    dd[i] -= f * du[i - 1];
+    r[i] -= f * r[i - 1];
+  }
+  b[n - 1] = r[n - 1] / dd[n - 1];
+  for (int i = n - 2; i >= 0; i--) b[i] = (r[i] - du[i] * b[i + 1]) / dd[i];
+  for (int i = 0; i < n - 1; i++) {
+    a[i] = 1.0 / 3.0 * (b[i + 1] - b[i]) / d[i];
+    c[i] = (ys[i + 1] - ys[i]) / d[i] - 1.0 / 3.0 * (2.0 * b[i] + b[i + 1]) * d[i];
+  }
+  double h = d[n - 2];
+  a[n - 1] = 0.0;
+  c[n - 1] = 3.0 * a[n - 2] * h * h + 2.0 * b[n - 2] * h + c[n - 2];
+}
+
+/* Evaluation of SplineGridConvIR2R_cpu at a target: map = the new->old search index of the
+ * target, off = global index of the window's node 0, xo = old node coordinates (global), xt =
+ * the target's coordinate.  The offset h is taken from old node max(map - 1, 0) (global), the
+ * branch from id = map - 1 - off; `last` is the index the reference tests for the beyond-the-end
+ * branch (n - 1, except n1 - 1 in the right-back x pass, SplineGridConvIR2R_cpu.cpp:627). */
+static double cpu_tk_eval(int n, int last, const double *a, const double *b, const double *c,
+                          const double *ys, int map, int off, const double *xo, double xt) {
+  double h = xt - xo[imax(map - 1, 0)];
+  int id = map - 1 - off;
+  if (id < 0) return (b[0] * h + c[0]) * h + ys[0];
+  if (id == last) return (b[n - 1] * h + c[n - 1]) * h + ys[id];
+  return ((a[id] * h + b[id]) * h + c[id]) * h + ys[id];
+}
+
+/* SplineGridConvIR2R_cpu (SplineGridConvIR2R_cpu.cpp:7-691): Bz0 (M x L) -> BzAir (M1 x L1).
+ * x passes over n1 = AL + 2 / n2 = AR + 2 nodes (the padding plus two core nodes) and y passes
+ * over n3 = AY + 2 nodes; the corner blocks go through a y pass of x-pass results whose two core
+ * nodes are the first / last two x-pass rows -- on the right, taken two columns to the left of
+ * the target column, as the reference does (:579-581, :616-618). */
+void orc_spline_ir2r_cpu(orc_model *m, const double *bz0, double *bzair) {
+  const int L = m->L, L1 = m->L1, Nx = m->s.Nx, Ny = m->s.Ny;
+  const int xo = m->AL, xr = m->AR, yo = m->AY; /* irxL_old, irxR_old, iry_old */
+  const int xn = m->AnLx, xrn = m->AnRx, yn = m->Any;
+  const int n1 = xo + 2, n2 = xr + 2, n3 = yo + 2;
+  const double *X = m->X, *Y = m->Y, *Xn = m->Xn, *Yn = m->Yn, *dx = m->dx, *dy = m->dy;
+  const int *xm = m->xN2o, *ym = m->yN2o;
+  int nmax = imax(imax(n1, n2), n3);
+  double *a = malloc(nmax * sizeof(double)), *b = malloc(nmax * sizeof(double));
+  double *c = malloc(nmax * sizeof(double)), *w = malloc(4 * nmax * sizeof(double));
+  double *ys = malloc(nmax * sizeof(double));
+  double *semi = malloc((size_t)imax(imax(xn, xrn), 1) * n3 * sizeof(double));
+#define F_OLD(j, i) bz0[(size_t)(j) * L + (i)]
+#define F_NEW(j, i) bzair[(size_t)(j) * L1 + (i)]
+#define SEMI(i, j) semi[(size_t)(i) * n3 + (j)]
+  /* core copy (:43-50) */
+  for (int j = 0; j < Ny; j++)
+    for (int i = 0; i < Nx; i++) F_NEW(j + yn, i + xn) = F_OLD(j + yo, i + xo);
+
+  /* one x pass of old row j over the window starting at old column x0 (nodes x0 .. x0+nn-1),
+   * evaluated at new columns c0 .. c0+nc-1; results go to dst[k] */
+#define X_PASS(j, x0, nn, c0, nc, last, STORE)                                               \
+  do {                                                                                        \
+    for (int q = 0; q < (nn); q++) ys[q] = F_OLD(j, (x0) + q);                                \
+    cpu_tk_spline((nn), dx + (x0), ys, a, b, c, w);                                           \
+    for (int k = 0; k < (nc); k++) {                                                          \
+      double v = cpu_tk_eval((nn), (last), a, b, c, ys, xm[(c0) + k], (x0), X, Xn[(c0) + k]); \
+      STORE;                                                                                  \
+    }                                                                                         \
+  } while (0)
+  /* one y pass over nodes ys (already filled; window starting at old row y0), evaluated at new
+   * rows r0 .. r0+nr-1 */
+#define Y_PASS(y0, r0, nr, STORE)                                                             \
+  do {                                                                                        \
+    cpu_tk_spline(n3, dy + (y0), ys, a, b, c, w);                                             \
+    for (int k = 0; k < (nr); k++) {                                                          \
+      double v = cpu_tk_eval(n3, n3 - 1, a, b, c, ys, ym[(r0) + k], (y0), Y, Yn[(r0) + k]);   \
+      STORE;                                                                                  \
+    }                                                                                         \
+  } while (0)
+
+  const int yb = yo + Ny - 2; /* old row of the back windows' node 0 */
+  const int xb = Nx + xo - 2; /* old column of the right windows' node 0 */
+  /* ---- left: x passes (front rows into semi, middle rows into the plane), y passes of the
+   * corners (:52-346) ---- */
+  for (int j = 0; j < yo; j++) X_PASS(j, 0, n1, 0, xn, n1 - 1, SEMI(k, j) = v);
+  for (int j = yo; j < yo + Ny; j++) X_PASS(j, 0, n1, 0, xn, n1 - 1, F_NEW(j - yo + yn, k) = v);
+  for (int j = yo; j < yo + 2; j++)
+    for (int i = 0; i < xn; i++) SEMI(i, j) = F_NEW(j - yo + yn, i);
+  for (int i = 0; i < xn; i++) {
+    for (int q = 0; q < n3; q++) ys[q] = SEMI(i, q);
+    Y_PASS(0, 0, yn, F_NEW(k, i) = v);
+  }
+  for (int j = yo + Ny - 2; j < yo + Ny; j++)
+    for (int i = 0; i < xn; i++) SEMI(i, j + 2 - yo - Ny) = F_NEW(j - yo + yn, i);
+  for (int j = yo + Ny; j < Ny + 2 * yo; j++) X_PASS(j, 0, n1, 0, xn, n1 - 1, SEMI(k, j + 2 - yo - Ny) = v);
+  for (int i = 0; i < xn; i++) {
+    for (int q = 0; q < n3; q++) ys[q] = SEMI(i, q);
+    Y_PASS(yb, yn + Ny, yn, F_NEW(yn + Ny + k, i) = v);
+  }
+  /* ---- centre columns: front and back y passes (:348-432) ---- */
+  for (int jc = xo; jc < xo + Nx; jc++) {
+    for (int q = 0; q < n3; q++) ys[q] = F_OLD(q, jc);
+    Y_PASS(0, 0, yn, F_NEW(k, jc - xo + xn) = v);
+    for (int q = 0; q < n3; q++) ys[q] = F_OLD(yb + q, jc);
+    Y_PASS(yb, yn + Ny, yn, F_NEW(yn + Ny + k, jc - xo + xn) = v);
+  }
+  /* ---- right (:435-688) ---- */
+  const int cr = Nx + xn; /* first new column of the right padding */
+  for (int j = 0; j < yo; j++) X_PASS(j, xb, n2, cr, xrn, n2 - 1, SEMI(k, j) = v);
+  for (int j = yo; j < yo + Ny; j++) X_PASS(j, xb, n2, cr, xrn, n2 - 1, F_NEW(j - yo + yn, cr + k) = v);
+  for (int j = yo; j < yo + 2; j++)
+    for (int i = 0; i < xrn; i++) SEMI(i, j) = F_NEW(j - yo + yn, i + cr - 2);
+  for (int i = 0; i < xrn; i++) {
+    for (int q = 0; q < n3; q++) ys[q] = SEMI(i, q);
+    Y_PASS(0, 0, yn, F_NEW(k, i + cr) = v);
+  }
+  for (int j = yo + Ny - 2; j < yo + Ny; j++)
+    for (int i = 0; i < xrn; i++) SEMI(i, j + 2 - yo - Ny) = F_NEW(j - yo + yn, i + cr - 2);
+  for (int j = yo + Ny; j < Ny + 2 * yo; j++) X_PASS(j, xb, n2, cr, xrn, n1 - 1, SEMI(k, j + 2 - yo - Ny) = v);
+  for (int i = 0; i < xrn; i++) {
+    for (int q = 0; q < n3; q++) ys[q] = SEMI(i, q);
+    Y_PASS(yb, yn + Ny, yn, F_NEW(yn + Ny + k, i + cr) = v);
+  }
+#undef X_PASS
+#undef Y_PASS
+#undef F_OLD
+#undef F_NEW
+#undef SEMI
+  free(a); free(b); free(c); free(w); free(ys); free(semi);
+}
+
+/* The filter of UpContinuation.cpp:70-97 on the half spectrum: physical wavenumbers
+ * u = 2 pi ix / (L1 dxmin), v = +-2 pi iy / (M1 dymin), upcc = exp(-dzmin/2 |k|) / |k|,
+ * FBx = i u upcc FBz, FBy = i v upcc FBz, DC FBx = FBy = i FBz; then the C2R of the FFTW plans
+ * (Hermitian completion, real part), unnormalised.  Identical to the GPU filter when dxmin =
+ * dymin; otherwise the GPU form carries the extra factors dx/dy and dy/dx
+ * (UpContinuation_gpu.cu:159-160). */
+void orc_upcont_filter_cpu(orc_model *m, const double *bzair, double *bxr, double *byr) {
+  int L = m->L1, M = m->M1, H = L / 2 + 1;
+  size_t np = (size_t)L * M;
+  double *Fr = m->Fr, *Fi = m->Fi, *Gr = m->Gr, *Gi = m->Gi;
+  for (size_t p = 0; p < np; p++) { Fr[p] = bzair[p]; Fi[p] = 0.0; }
+  orc_fft2(M, L, Fr, Fi, -1);
+  const double dz = m->s.dzmin, dxm = m->s.dxmin, dym = m->s.dymin;
+  for (int pass = 0; pass < 2; pass++) {
+    double *out = pass == 0 ? bxr : byr;
+    for (int j = 0; j < M; j++)
+      for (int i = 0; i < H; i++) {
+        double v = (j < M / 2 + 1) ? 2 * ORC_PI * j / M / dym : -2 * ORC_PI * (M - j) / M / dym;
+        double u = 2 * ORC_PI * i / L / dxm;
+        size_t p = (size_t)j * L + i;
+        double g = 1.0;
+        if (!(i == 0 && j == 0)) {
+          double k = sqrt(u * u + v * v);
+          g = (pass == 0 ? u : v) * exp(-dz / 2 * k) / k;
+        }
+        Gr[p] = -g * Fi[p];
+        Gi[p] = g * Fr[p];
+      }
+    for (int j = 0; j < M; j++)
+      for (int i = H; i < L; i++) {
+        size_t p = (size_t)j * L + i, q = (size_t)((M - j) % M) * L + (L - i);
+        Gr[p] = Gr[q]; Gi[p] = -Gi[q];
+      }
+    /* self-conjugate bins: the C2R keeps their real part */
+    for (int j = 0; j < M; j += (M % 2 == 0 ? M / 2 : M))
+      for (int i = 0; i < L; i += (L % 2 == 0 ? L / 2 : L)) Gi[(size_t)j * L + i] = 0.0;
+    orc_fft2(M, L, Gr, Gi, +1);
+    for (size_t p = 0; p < np; p++) out[p] = Gr[p];
+  }
+}
+
+/* BilinearGridConvR2IR (BilinearGridConvR2IR.cpp:8-352): linear interpolation between the
+ * regular nodes around each Bx / By point, linear extrapolation from the first / last two nodes
+ * where the search index is 0 or L1 (M1); divided by L1 M1 as UpContinuation.cpp:142-150 does.
+ * Where the reference reads Bnew[JJ][L1] (one past the row, :99 and :257, the II == L1, JJ == 0
+ * corner), the last-two-nodes formula of the neighbouring branches is used. */
+static double cpu_bilin_point(const double *B, int L1, int M1, const double *Xn, const double *Yn,
+                              int II, int JJ, double x, double y) {
+  int i0, j0; /* the two nodes used in x: i0, i0 + 1; in y: j0, j0 + 1 */
+  if (II == 0) i0 = 0;
+  else if (II == L1) i0 = L1 - 2;
+  else i0 = II - 1;
+  if (JJ == 0) j0 = 0;
+  else if (JJ == M1) j0 = M1 - 2;
+  else j0 = JJ - 1;
+  double hx = Xn[i0 + 1] - Xn[i0];
+  double k1 = (B[(size_t)j0 * L1 + i0 + 1] - B[(size_t)j0 * L1 + i0]) / hx;
+  double k2 = (B[(size_t)(j0 + 1) * L1 + i0 + 1] - B[(size_t)(j0 + 1) * L1 + i0]) / hx;
+  double a1 = B[(size_t)j0 * L1 + i0] + k1 * (x - Xn[i0]);
+  double a2 = B[(size_t)(j0 + 1) * L1 + i0] + k2 * (x - Xn[i0]);
+  return a1 + (a2 - a1) / (Yn[j0 + 1] - Yn[j0]) * (y - Yn[j0]);
+}
+
+void orc_bilinear_cpu(orc_model *m, const double *bxr, const double *byr, double *bxair, double *byair) {
+  int L0 = m->L, M0 = m->M, L1 = m->L1, M1 = m->M1;
+  double norm = (double)L1 * M1;
+  for (int j = 0; j < M0; j++)
+    for (int i = 0; i < L0 + 1; i++) {
+      double x = (i == L0) ? m->X[L0 - 1] + 0.5 * m->dx[L0 - 1] : m->X[i] - 0.5 * m->dx[i];
+      bxair[(size_t)j * (L0 + 1) + i] =
+          cpu_bilin_point(bxr, L1, M1, m->Xn, m->Yn, m->xBx2N[i], m->yo2N[j], x, m->Y[j]) / norm;
+    }
+  for (int j = 0; j < M0 + 1; j++)
+    for (int i = 0; i < L0; i++) {
+      double y = (j == M0) ? m->Y[M0 - 1] + 0.5 * m->dy[M0 - 1] : m->Y[j] - 0.5 * m->dy[j];
+      byair[(size_t)j * L0 + i] =
+          cpu_bilin_point(byr, L1, M1, m->Xn, m->Yn, m->xo2N[i], m->yBy2N[j], m->X[i], y) / norm;
+    }
+}
+
+/* UpContinuation.cpp:8-211: the CPU-twin chain on the plane bz0 (M x L). */
+void orc_upcont_plane_cpu(orc_model *m, const double *bz0, double *bxair, double *byair) {
+  size_t np = (size_t)m->L1 * m->M1;
+  double *bxr = malloc(np * sizeof(double)), *byr = malloc(np * sizeof(double));
+  orc_spline_ir2r_cpu(m, bz0, m->BzAir);
+  orc_upcont_filter_cpu(m, m->BzAir, bxr, byr);
+  orc_bilinear_cpu(m, bxr, byr, bxair, byair);
+  free(bxr); free(byr);
+}
+
+void orc_set_cpu_twins(orc_model *m, int on) { m->cpu_twins = on != 0; }
+
+void orc_upcont(orc_model *m) {
+  if (m->cpu_twins) orc_upcont_plane_cpu(m, m->Bz, m->BxAir, m->ByAir);
+  else orc_upcont_plane(m, m->Bz, m->BxAir, m->ByAir);
+}
 
 /* ============================================================================================
  * E update (ElectricalFiled.cu:89-345), race-free semantics.
