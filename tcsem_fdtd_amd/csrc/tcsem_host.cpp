@@ -673,14 +673,15 @@ UpcontArgs<T> upcont_args(tcsem_ctx *c) {
   u.tiled = c->up_tiled;
   if (c->up_tiled) {
     const long long ns = c->nslots;
-    u.A = (double2 *)w;
+    T *wt = (T *)w;  // tiled planes in the field type; complex entries are T pairs
+    u.A = wt;
     u.sA = c->sA;
     u.MA = c->MA;
-    u.G = (double2 *)(w + ns * c->sA);
+    u.G = wt + 2 * ns * c->sA;
     u.sG = c->sG;
     u.MG = c->MG;
-    u.bxr = w + ns * (c->sA + c->sG);
-    u.byr = u.bxr + ns * c->sX;
+    u.tbx = wt + 2 * ns * (c->sA + c->sG);
+    u.tby = u.tbx + ns * c->sX;
     u.sX = c->sX;
     u.XR = c->XR;
   } else {
@@ -741,8 +742,11 @@ int alloc_slots(tcsem_ctx *c, int nslots) {
   c->nslots = 0;
   const size_t fb = (size_t)2 * 5 * nslots * c->fstride * c->esz;
   const size_t pb = (size_t)5 * nslots * c->pstride * c->esz;
-  const long long per_slot = c->up_tiled ? c->sA + c->sG + 2 * c->sX : 5 * c->wstride;
-  const size_t wb = (size_t)nslots * per_slot * sizeof(double);
+  // bytes per slot: tiled chain A and G (complex of the field type) and Bx, By (field type);
+  // line chain cplx (complex FP64), tx, bxr (also the y-padding rows), byr (FP64)
+  const size_t per_slot = c->up_tiled ? (size_t)(2 * c->sA + 2 * c->sG + 2 * c->sX) * c->esz
+                                      : (size_t)5 * c->wstride * sizeof(double);
+  const size_t wb = (size_t)nslots * per_slot;
   if (hipMalloc(&c->d_fields, fb) != hipSuccess) return fail(TCSEM_ENOMEM, "field allocation of %zu bytes failed", fb);
   if (hipMalloc(&c->d_planes, pb) != hipSuccess) return fail(TCSEM_ENOMEM, "plane allocation failed");
   if (hipMalloc((void **)&c->d_work, wb) != hipSuccess) return fail(TCSEM_ENOMEM, "work allocation failed");
@@ -1608,12 +1612,14 @@ int tcsem_create(const tcsem_setup *s, const double *cond, int precision, tcsem_
         if (n % 2048 == 0) n += 32;
         return n;
       };
-      c->MA = pitch(M, 8);    // complex
-      c->MG = pitch(M1, 4);   // (Gx, Gy) pairs
-      c->XR = pitch(L1, 16);  // doubles
-      c->sA = slot_stride(2LL * H * c->MA);
-      c->sG = slot_stride(4LL * H * c->MG);
-      c->sX = slot_stride((long long)M1 * c->XR);
+      // in entries of the field type: 128 bytes hold 128/esz reals, half as many complex
+      const int per = kLineBytes / (int)c->esz;
+      c->MA = pitch(M, per / 2);    // complex
+      c->MG = pitch(M1, per / 4);   // (Gx, Gy) pairs of complex
+      c->XR = pitch(L1, per);       // reals
+      c->sA = slot_stride(H * (long long)c->MA);       // complex entries
+      c->sG = slot_stride(2LL * H * c->MG);            // complex entries
+      c->sX = slot_stride((long long)M1 * c->XR);      // reals
       c->gxT.assign((size_t)H * M1, 0.0);
       c->gyT.assign((size_t)H * M1, 0.0);
       for (int ky = 0; ky < M1; ++ky)

@@ -124,15 +124,17 @@ struct UpcontArgs {
   double *ry;              // per slot M1 x L1: the y-padding rows of the regular plane
   double2 *cplx;           // per slot M1 x L1 complex work (column spectra / row results)
   long long wstride;       // elements between slots of the line-chain work planes
-  // tiled chain work planes (slot strides in doubles)
-  double2 *A;              // per slot H x MA: x half-spectra of the irregular rows, kx-major
-  double2 *G;              // per slot H x MG pairs (Gx, Gy): y-inverse columns, kx-major
+  // tiled chain work planes, in the field precision T (complex entries: T pairs); slot strides
+  // in entries of each plane
+  void *A;                 // per slot H x MA complex: x half-spectra of the irregular rows, kx-major
+  void *G;                 // per slot H x MG pairs (Gx, Gy) of complex: y-inverse columns, kx-major
+  T *tbx, *tby;            // per slot M1 rows of pitch XR: Bx, By on the regular plane
   int MA, MG;
   long long sA, sG;
-  // both chains: real results, rows of pitch XR, slot stride sX (doubles)
+  // line chain results (FP64), rows of pitch XR
   double *bxr, *byr;
   int XR;
-  long long sX;
+  long long sX;            // slot stride of the results of either chain (entries)
   FftPlan px, py;          // x (length L1) and y (length M1) plans
   const double *gx, *gy;   // filter gains on the half spectrum: M1 x (L1/2+1)
   const double *gxT, *gyT; // the same, kx-major (H x M1), with the non-Hermitian bins zeroed

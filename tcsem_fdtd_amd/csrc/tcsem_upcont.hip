@@ -49,75 +49,120 @@
 
 namespace tcsem {
 
-__device__ inline double2 cmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+// Complex arithmetic on float2 / double2 (V), scalar S.
+template <typename S>
+struct CxOf;
+template <>
+struct CxOf<float> {
+  using V = float2;
+};
+template <>
+struct CxOf<double> {
+  using V = double2;
+};
+template <typename V>
+__device__ inline V mkc(decltype(V::x) x, decltype(V::x) y) {
+  V v;
+  v.x = x;
+  v.y = y;
+  return v;
 }
-__device__ inline double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ inline double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ inline double2 conj2(double2 a) { return make_double2(a.x, -a.y); }
+template <typename V>
+__device__ inline V cmul(V a, V b) {
+  return mkc<V>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename V>
+__device__ inline V cadd(V a, V b) {
+  return mkc<V>(a.x + b.x, a.y + b.y);
+}
+template <typename V>
+__device__ inline V csub(V a, V b) {
+  return mkc<V>(a.x - b.x, a.y - b.y);
+}
+template <typename V>
+__device__ inline V conj2(V a) {
+  return mkc<V>(a.x, -a.y);
+}
 // a * (s i) for real s: (x + i y) s i = -s y + i s x
-__device__ inline double2 muli(double2 a, double s) { return make_double2(-s * a.y, s * a.x); }
+template <typename V>
+__device__ inline V muli(V a, decltype(V::x) s) {
+  return mkc<V>(-s * a.y, s * a.x);
+}
+// entry k of an FP64 complex table, in the precision of V
+template <typename V>
+__device__ inline V from_table(const double *t, int k) {
+  const double2 w = ((const double2 *)t)[k];
+  return mkc<V>((decltype(V::x))w.x, (decltype(V::x))w.y);
+}
 
 // exp(sign 2 pi i q / m) from the m-entry table of exp(-2 pi i q / m), 0 <= q < m
-__device__ inline double2 twid(const double2 *tw, int q, int sign) {
-  double2 w = tw[q];
+template <typename V>
+__device__ inline V twid(const V *tw, int q, int sign) {
+  V w = tw[q];
   if (sign > 0) w.y = -w.y;
   return w;
 }
 
 // in-register DFTs of size R with exp(sign 2 pi i / R) as the root
-template <int R>
-__device__ inline void dft(double2 *v, int sign);
-template <>
-__device__ inline void dft<2>(double2 *v, int) {
-  const double2 a = v[0], b = v[1];
-  v[0] = cadd(a, b);
-  v[1] = csub(a, b);
-}
-template <>
-__device__ inline void dft<4>(double2 *v, int sign) {
-  const double s = (double)sign;
-  const double2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
-  const double2 t2 = cadd(v[1], v[3]), t3 = muli(csub(v[1], v[3]), s);
-  v[0] = cadd(t0, t2);
-  v[2] = csub(t0, t2);
-  v[1] = cadd(t1, t3);
-  v[3] = csub(t1, t3);
-}
-template <>
-__device__ inline void dft<8>(double2 *v, int sign) {
-  const double s = (double)sign, h = 0.70710678118654752440;  // cos(pi/4)
-  double2 e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
-  dft<4>(e, sign);
-  dft<4>(o, sign);
-  // w^k, w = exp(sign i pi / 4): w = (h, s h), w^2 = s i, w^3 = (-h, s h)
-  o[1] = cmul(o[1], make_double2(h, s * h));
-  o[2] = muli(o[2], s);
-  o[3] = cmul(o[3], make_double2(-h, s * h));
-#pragma unroll
-  for (int k = 0; k < 4; ++k) {
-    v[k] = cadd(e[k], o[k]);
-    v[k + 4] = csub(e[k], o[k]);
+template <int R, typename V>
+struct Dft;
+template <typename V>
+struct Dft<2, V> {
+  __device__ static inline void run(V *v, int) {
+    const V a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
   }
-}
+};
+template <typename V>
+struct Dft<4, V> {
+  __device__ static inline void run(V *v, int sign) {
+    using S = decltype(V::x);
+    const S s = (S)sign;
+    const V t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    const V t2 = cadd(v[1], v[3]), t3 = muli(csub(v[1], v[3]), s);
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+  }
+};
+template <typename V>
+struct Dft<8, V> {
+  __device__ static inline void run(V *v, int sign) {
+    using S = decltype(V::x);
+    const S s = (S)sign, h = (S)0.70710678118654752440;  // cos(pi/4)
+    V e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+    Dft<4, V>::run(e, sign);
+    Dft<4, V>::run(o, sign);
+    // w^k, w = exp(sign i pi / 4): w = (h, s h), w^2 = s i, w^3 = (-h, s h)
+    o[1] = cmul(o[1], mkc<V>(h, s * h));
+    o[2] = muli(o[2], s);
+    o[3] = cmul(o[3], mkc<V>(-h, s * h));
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      v[k] = cadd(e[k], o[k]);
+      v[k + 4] = csub(e[k], o[k]);
+    }
+  }
+};
 
 // One Stockham autosort pass of radix R over length m, reading a and writing b; Ns is the
 // length of the sub-transforms already done.  Butterfly j reads a[j + r m/R], applies the
 // twiddles exp(sign 2 pi i r (j mod Ns) / (Ns R)), and writes b[(j - j mod Ns) R + j mod Ns + r Ns].
-template <int R>
-__device__ inline void stockham_pass(const double2 *a, double2 *b, int m, int Ns, const double2 *tw, int sign,
-                                     int tid, int nthr) {
+template <int R, typename V>
+__device__ inline void stockham_pass(const V *a, V *b, int m, int Ns, const V *tw, int sign, int tid, int nthr) {
   const int nb = m / R, span = m / (Ns * R);
   for (int j = tid; j < nb; j += nthr) {
     const int k = j & (Ns - 1);
-    double2 v[R];
+    V v[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) v[r] = a[j + r * nb];
     if (Ns > 1) {
 #pragma unroll
       for (int r = 1; r < R; ++r) v[r] = cmul(v[r], twid(tw, r * k * span, sign));
     }
-    dft<R>(v, sign);
+    Dft<R, V>::run(v, sign);
     const int d = (j - k) * R + k;
 #pragma unroll
     for (int r = 0; r < R; ++r) b[d + r * Ns] = v[r];
@@ -127,9 +172,8 @@ __device__ inline void stockham_pass(const double2 *a, double2 *b, int m, int Ns
 // Unnormalised DFT of length m = 2^log2m of the line in a; with PP the passes alternate between
 // a and b (m entries each), otherwise they run in place in a.  Returns the buffer holding the
 // result.  Every thread of the workgroup calls it with the same m (it synchronises the workgroup).
-template <bool PP>
-__device__ double2 *fft_pow2(double2 *a, double2 *b, int m, int log2m, const double2 *tw, int sign, int tid,
-                             int nthr) {
+template <bool PP, typename V>
+__device__ V *fft_pow2(V *a, V *b, int m, int log2m, const V *tw, int sign, int tid, int nthr) {
   if constexpr (PP) {
     int Ns = 1, left = log2m;
     while (left > 0) {
@@ -147,7 +191,7 @@ __device__ double2 *fft_pow2(double2 *a, double2 *b, int m, int log2m, const dou
         left -= 1;
       }
       __syncthreads();
-      double2 *t = a;
+      V *t = a;
       a = b;
       b = t;
     }
@@ -157,7 +201,7 @@ __device__ double2 *fft_pow2(double2 *a, double2 *b, int m, int log2m, const dou
     for (int idx = tid; idx < m; idx += nthr) {
       const int r = (int)(__brev((unsigned)idx) >> (32 - log2m));
       if (idx < r) {
-        const double2 t = a[idx];
+        const V t = a[idx];
         a[idx] = a[r];
         a[r] = t;
       }
@@ -168,7 +212,7 @@ __device__ double2 *fft_pow2(double2 *a, double2 *b, int m, int log2m, const dou
       for (int bf = tid; bf < (m >> 1); bf += nthr) {
         const int grp = bf >> (s - 1), pos = bf & (half - 1);
         const int i0 = (grp << s) + pos, i1 = i0 + half;
-        const double2 u = a[i0], v = cmul(a[i1], twid(tw, pos * stride, sign));
+        const V u = a[i0], v = cmul(a[i1], twid(tw, pos * stride, sign));
         a[i0] = cadd(u, v);
         a[i1] = csub(u, v);
       }
@@ -179,30 +223,29 @@ __device__ double2 *fft_pow2(double2 *a, double2 *b, int m, int log2m, const dou
 }
 
 // Unnormalised DFT of length P.n of a[0..n) (a and, with PP, b hold P.m entries; tw is the
-// m-entry twiddle table, staged in LDS with PP).  The caller has written a[0..n) and
-// synchronised; returns the buffer holding the result in [0, n).
-template <bool PP>
-__device__ double2 *fft_line(double2 *a, double2 *b, const FftPlan &P, const double2 *tw, int sign, int tid,
-                             int nthr) {
+// m-entry twiddle table in the precision of V, staged in LDS with PP; the Bluestein chirp tables
+// are FP64 and rounded on use).  The caller has written a[0..n) and synchronised; returns the
+// buffer holding the result in [0, n).
+template <bool PP, typename V>
+__device__ V *fft_line(V *a, V *b, const FftPlan &P, const V *tw, int sign, int tid, int nthr) {
   if (P.m == P.n) return fft_pow2<PP>(a, b, P.m, P.log2m, tw, sign, tid, nthr);
-  const double2 *w = (const double2 *)P.chirp, *bh = (const double2 *)P.bhat;
   for (int k = tid; k < P.m; k += nthr) {
     if (k < P.n) {
-      double2 x = a[k];
+      V x = a[k];
       if (sign > 0) x = conj2(x);  // inverse = conj(forward(conj))
-      a[k] = cmul(x, w[k]);
+      a[k] = cmul(x, from_table<V>(P.chirp, k));
     } else {
-      a[k] = make_double2(0.0, 0.0);
+      a[k] = mkc<V>(0, 0);
     }
   }
   __syncthreads();
-  double2 *r = fft_pow2<PP>(a, b, P.m, P.log2m, tw, -1, tid, nthr);
-  double2 *o = (r == a) ? b : a;
-  for (int k = tid; k < P.m; k += nthr) r[k] = cmul(r[k], bh[k]);
+  V *r = fft_pow2<PP>(a, b, P.m, P.log2m, tw, -1, tid, nthr);
+  V *o = (r == a) ? b : a;
+  for (int k = tid; k < P.m; k += nthr) r[k] = cmul(r[k], from_table<V>(P.bhat, k));
   __syncthreads();
   r = fft_pow2<PP>(r, o, P.m, P.log2m, tw, +1, tid, nthr);
   for (int k = tid; k < P.n; k += nthr) {
-    double2 y = cmul(r[k], w[k]);
+    V y = cmul(r[k], from_table<V>(P.chirp, k));
     if (sign > 0) y = conj2(y);
     r[k] = y;
   }
@@ -371,35 +414,38 @@ __global__ void __launch_bounds__(256) k_up_ycols2(const UpcontArgs<T> u) {
   }
 }
 
-// ---- tiled chain ---------------------------------------------------------------------------
-// LDS of a tiled kernel: `lines` line regions of two buffers of m + 2 entries each (the two extra
-// entries hold the Nyquist bin of an unpacked half spectrum and shift the lines across LDS
-// banks), then the m-entry twiddle table.
+// ---- tiled chain (computes in the field precision T) -------------------------------------
+// LDS of a tiled kernel: `lines` line regions of two buffers of m + 2 complex entries each (the
+// two extra entries hold the Nyquist bin of an unpacked half spectrum and shift the lines across
+// LDS banks), then the m-entry twiddle table.
 __host__ __device__ inline int tile_span(int m) { return m + 2; }
+template <typename T>
 __host__ __device__ inline size_t tile_lds(int m, int lines) {
-  return ((size_t)lines * 2 * tile_span(m) + m) * sizeof(double2);
+  return ((size_t)lines * 2 * tile_span(m) + m) * 2 * sizeof(T);
 }
 
-__device__ inline const double2 *stage_twiddles(double2 *dst, const FftPlan &P) {
-  const double2 *g = (const double2 *)P.tw;
+// The twiddle table (FP64 on the device, exp(-2 pi i q / m) from the host) staged in LDS in the
+// precision of V: each entry rounded once from its FP64 value.
+template <typename V>
+__device__ inline const V *stage_twiddles(V *dst, const FftPlan &P) {
   const int flat = threadIdx.y * blockDim.x + threadIdx.x, nt = blockDim.x * blockDim.y;
-  for (int q = flat; q < P.m; q += nt) dst[q] = g[q];
+  for (int q = flat; q < P.m; q += nt) dst[q] = from_table<V>(P.tw, q);
   return dst;
 }
 
 // Regular column c of one irregular row staged in LDS (the x pass of the IR->R resampling).
 template <typename T>
-__device__ inline double resample_x(const UpcontArgs<T> &u, const double *srow, int c) {
+__device__ inline T resample_x(const UpcontArgs<T> &u, const T *srow, int c) {
   const int aL = u.AnLx;
   if (c >= aL && c < aL + u.Nx) return srow[u.AL + c - aL];
-  double v = 0.0;
+  T v = 0;
   if (c < aL) {
 #pragma unroll 8
-    for (int q = 0; q < u.n1; ++q) v += u.wl[(long long)q * aL + c] * srow[q];
+    for (int q = 0; q < u.n1; ++q) v += (T)u.wl[(long long)q * aL + c] * srow[q];
   } else {
     const int cr = c - aL - u.Nx, ox = u.Nx + u.AL - kNPad;
 #pragma unroll 8
-    for (int q = 0; q < u.n2; ++q) v += u.wr[(long long)q * u.AnRx + cr] * srow[ox + q];
+    for (int q = 0; q < u.n2; ++q) v += (T)u.wr[(long long)q * u.AnRx + cr] * srow[ox + q];
   }
   return v;
 }
@@ -408,33 +454,36 @@ __device__ inline double resample_x(const UpcontArgs<T> &u, const double *srow, 
 // j0 = 2 (blockIdx.x P + p) and j0 + 1.
 template <typename T>
 __global__ void __launch_bounds__(256) k_up_rows_fwd(const UpcontArgs<T> u) {
-  extern __shared__ double2 lds[];
+  using V = typename CxOf<T>::V;
+  extern __shared__ char lds_raw[];
+  V *lds = (V *)lds_raw;
   const int lane = threadIdx.x, W = blockDim.x, p = threadIdx.y, np = blockDim.y, slot = blockIdx.y;
   const int m = u.px.m, span = tile_span(m), L1 = u.L1, L = u.L, M = u.M;
-  double2 *a = lds + (long long)p * 2 * span, *b = a + span;
-  const double2 *tw = stage_twiddles(lds + (long long)np * 2 * span, u.px);
+  V *a = lds + (long long)p * 2 * span, *b = a + span;
+  const V *tw = stage_twiddles(lds + (long long)np * 2 * span, u.px);
   const int jb = blockIdx.x * 2 * np, j0 = jb + 2 * p;
-  // the two rows are staged in the second buffer (L <= m + 2 doubles each, checked on the host)
-  double *s0 = (double *)b, *s1 = s0 + L;
+  // the two rows are staged in the second buffer (L <= m + 2 entries each, checked on the host)
+  T *s0 = (T *)b, *s1 = s0 + L;
   const T *plane = u.bz0 + (long long)slot * u.pstride;
   for (int i = lane; i < L; i += W) {
-    s0[i] = j0 < M ? (double)plane[(long long)j0 * u.PX + i] : 0.0;
-    s1[i] = j0 + 1 < M ? (double)plane[(long long)(j0 + 1) * u.PX + i] : 0.0;
+    s0[i] = j0 < M ? plane[(long long)j0 * u.PX + i] : (T)0;
+    s1[i] = j0 + 1 < M ? plane[(long long)(j0 + 1) * u.PX + i] : (T)0;
   }
   __syncthreads();
-  for (int c = lane; c < L1; c += W) a[c] = make_double2(resample_x(u, s0, c), resample_x(u, s1, c));
+  for (int c = lane; c < L1; c += W) a[c] = mkc<V>(resample_x(u, s0, c), resample_x(u, s1, c));
   __syncthreads();
-  double2 *f = fft_line<true>(a, b, u.px, tw, -1, lane, W);
-  double2 *o = (f == a) ? b : a;
+  V *f = fft_line<true>(a, b, u.px, tw, -1, lane, W);
+  V *o = (f == a) ? b : a;
+  const T h = (T)0.5;
   for (int k = lane; k < u.H; k += W) {
-    const double2 z = f[k], w = f[k == 0 ? 0 : L1 - k];
-    o[2 * k] = make_double2(0.5 * (z.x + w.x), 0.5 * (z.y - w.y));       // (Z(k) + conj Z(-k)) / 2
-    o[2 * k + 1] = make_double2(0.5 * (z.y + w.y), -0.5 * (z.x - w.x));  // (Z(k) - conj Z(-k)) / 2i
+    const V z = f[k], w = f[k == 0 ? 0 : L1 - k];
+    o[2 * k] = mkc<V>(h * (z.x + w.x), h * (z.y - w.y));       // (Z(k) + conj Z(-k)) / 2
+    o[2 * k + 1] = mkc<V>(h * (z.y + w.y), -h * (z.x - w.x));  // (Z(k) - conj Z(-k)) / 2i
   }
   __syncthreads();
   // out through LDS: for each kx the workgroup's 2 np rows are one contiguous run
   const int R = 2 * np, ooff = (f == a) ? span : 0;
-  double2 *A = u.A + (long long)slot * (u.sA / 2);
+  V *A = (V *)u.A + (long long)slot * u.sA;
   for (int idx = p * W + lane; idx < u.H * R; idx += np * W) {
     const int kx = idx / R, r = idx - kx * R, j = jb + r;
     if (j < M) A[(long long)kx * u.MA + j] = lds[(long long)(r >> 1) * 2 * span + ooff + 2 * kx + (r & 1)];
@@ -445,63 +494,65 @@ __global__ void __launch_bounds__(256) k_up_rows_fwd(const UpcontArgs<T> u) {
 // kx = blockIdx.x Q + q.  E >= M1 / W: column entries kept in registers per thread.
 template <typename T, int E>
 __global__ void __launch_bounds__(256) k_up_cols(const UpcontArgs<T> u) {
-  extern __shared__ double2 lds[];
+  using V = typename CxOf<T>::V;
+  extern __shared__ char lds_raw[];
+  V *lds = (V *)lds_raw;
   const int lane = threadIdx.x, W = blockDim.x, q = threadIdx.y, nq = blockDim.y, slot = blockIdx.y;
   const int m = u.py.m, span = tile_span(m), M1 = u.M1, M = u.M;
-  double2 *a = lds + (long long)q * 2 * span, *b = a + span;
-  const double2 *tw = stage_twiddles(lds + (long long)nq * 2 * span, u.py);
+  V *a = lds + (long long)q * 2 * span, *b = a + span;
+  const V *tw = stage_twiddles(lds + (long long)nq * 2 * span, u.py);
   const int kx = blockIdx.x * nq + q;
   const bool live = kx < u.H;
   const int kc = live ? kx : 0;
-  const double2 *Acol = u.A + (long long)slot * (u.sA / 2) + (long long)kc * u.MA;
-  for (int j = lane; j < M; j += W) b[j] = live ? Acol[j] : make_double2(0.0, 0.0);
+  const V *Acol = (const V *)u.A + (long long)slot * u.sA + (long long)kc * u.MA;
+  for (int j = lane; j < M; j += W) b[j] = live ? Acol[j] : mkc<V>(0, 0);
   __syncthreads();
   // the regular column: core rows are irregular rows, padding rows the y-window sums
   // (the y pass of SplineGridConvIR2R.cu:539-752)
   const int y0 = u.Any, y1 = u.Any + u.Ny, sh = u.AY - u.Any, ob = u.Ny + u.AY - kNPad;
   for (int r = lane; r < M1; r += W) {
-    double2 v;
+    V v;
     if (r >= y0 && r < y1) {
       v = b[r + sh];
     } else {
       const double *w = r < y0 ? u.wf + (long long)r * u.n3 : u.wb + (long long)(r - y1) * u.n3;
-      const double2 *s = r < y0 ? b : b + ob;
-      double vx = 0.0, vy = 0.0;
+      const V *s = r < y0 ? b : b + ob;
+      T vx = 0, vy = 0;
 #pragma unroll 8
       for (int t = 0; t < u.n3; ++t) {
-        const double wt = w[t];
+        const T wt = (T)w[t];
         vx += wt * s[t].x;
         vy += wt * s[t].y;
       }
-      v = make_double2(vx, vy);
+      v = mkc<V>(vx, vy);
     }
     a[r] = v;
   }
   __syncthreads();
-  double2 *f = fft_line<true>(a, b, u.py, tw, -1, lane, W);
-  double2 *o = (f == a) ? b : a;
+  V *f = fft_line<true>(a, b, u.py, tw, -1, lane, W);
+  V *o = (f == a) ? b : a;
   const double *gx = u.gxT + (long long)kc * M1, *gy = u.gyT + (long long)kc * M1;
-  double2 F[E], Gx[E];
-  // F stays in registers; each lane rewrites only its own entries
+  V F[E], Gx[E];
+  // F stays in registers; each thread rewrites only its own entries
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int ky = lane + W * e;
-    F[e] = ky < M1 ? f[ky] : make_double2(0.0, 0.0);
-    if (ky < M1) f[ky] = muli(F[e], gx[ky]);  // FBx = i gx F
+    F[e] = ky < M1 ? f[ky] : mkc<V>(0, 0);
+    if (ky < M1) f[ky] = muli(F[e], (T)gx[ky]);  // FBx = i gx F
   }
   __syncthreads();
-  double2 *rx = fft_line<true>(f, o, u.py, tw, +1, lane, W);
-  double2 *o2 = (rx == f) ? o : f;
+  V *rx = fft_line<true>(f, o, u.py, tw, +1, lane, W);
+  V *o2 = (rx == f) ? o : f;
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int ky = lane + W * e;
-    Gx[e] = ky < M1 ? rx[ky] : make_double2(0.0, 0.0);
-    if (ky < M1) rx[ky] = muli(F[e], gy[ky]);  // FBy = i gy F
+    Gx[e] = ky < M1 ? rx[ky] : mkc<V>(0, 0);
+    if (ky < M1) rx[ky] = muli(F[e], (T)gy[ky]);  // FBy = i gy F
   }
   __syncthreads();
-  double2 *ry = fft_line<true>(rx, o2, u.py, tw, +1, lane, W);
+  V *ry = fft_line<true>(rx, o2, u.py, tw, +1, lane, W);
   if (live) {
-    double2 *Gc = u.G + (long long)slot * (u.sG / 2) + (long long)kx * u.MG * 2;
+    V *Gc = (V *)u.G + (long long)slot * u.sG + (long long)kx * u.MG * 2;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
       const int y = lane + W * e;
@@ -517,46 +568,51 @@ __global__ void __launch_bounds__(256) k_up_cols(const UpcontArgs<T> u) {
 // y = blockIdx.x Q + q along x.
 template <typename T>
 __global__ void __launch_bounds__(256) k_up_rows_inv(const UpcontArgs<T> u) {
-  extern __shared__ double2 lds[];
+  using V = typename CxOf<T>::V;
+  extern __shared__ char lds_raw[];
+  V *lds = (V *)lds_raw;
   const int lane = threadIdx.x, W = blockDim.x, q = threadIdx.y, nq = blockDim.y, slot = blockIdx.y;
   const int m = u.px.m, span = tile_span(m), L1 = u.L1, H = u.H, M1 = u.M1;
-  const double2 *tw = stage_twiddles(lds + (long long)nq * 2 * span, u.px);
+  const V *tw = stage_twiddles(lds + (long long)nq * 2 * span, u.px);
   const int yb = blockIdx.x * nq;
-  const double2 *G = u.G + (long long)slot * (u.sG / 2);
+  const V *G = (const V *)u.G + (long long)slot * u.sG;
   const bool evenL = (L1 & 1) == 0;
   // tile in: for each kx the nq rows are one contiguous run of (Gx, Gy) pairs
   for (int idx = q * W + lane; idx < H * nq; idx += nq * W) {
     const int kx = idx / nq, r = idx - kx * nq, y = yb + r;
-    double2 gx = make_double2(0.0, 0.0), gy = gx;
+    V gx = mkc<V>(0, 0), gy = gx;
     if (y < M1) {
       gx = G[((long long)kx * u.MG + y) * 2];
       gy = G[((long long)kx * u.MG + y) * 2 + 1];
     }
-    double2 *row = lds + (long long)r * 2 * span;
+    V *row = lds + (long long)r * 2 * span;
     if (kx == 0 || (evenL && 2 * kx == L1)) {
-      row[kx] = make_double2(gx.x, gy.x);  // self-conjugate bins: real parts
+      row[kx] = mkc<V>(gx.x, gy.x);  // self-conjugate bins: real parts
     } else {
-      row[kx] = make_double2(gx.x - gy.y, gx.y + gy.x);       // Gx + i Gy
-      row[L1 - kx] = make_double2(gx.x + gy.y, gy.x - gx.y);  // conj Gx + i conj Gy
+      row[kx] = mkc<V>(gx.x - gy.y, gx.y + gy.x);       // Gx + i Gy
+      row[L1 - kx] = mkc<V>(gx.x + gy.y, gy.x - gx.y);  // conj Gx + i conj Gy
     }
   }
   __syncthreads();
-  double2 *a = lds + (long long)q * 2 * span, *b = a + span;
-  double2 *z = fft_line<true>(a, b, u.px, tw, +1, lane, W);
+  V *a = lds + (long long)q * 2 * span, *b = a + span;
+  V *z = fft_line<true>(a, b, u.px, tw, +1, lane, W);
   const int y = yb + q;
   if (y < M1) {
-    double *bx = u.bxr + (long long)slot * u.sX + (long long)y * u.XR;
-    double *by = u.byr + (long long)slot * u.sX + (long long)y * u.XR;
+    T *bx = u.tbx + (long long)slot * u.sX + (long long)y * u.XR;
+    T *by = u.tby + (long long)slot * u.sX + (long long)y * u.XR;
     for (int x = lane; x < L1; x += W) {
-      const double2 v = z[x];
+      const V v = z[x];
       bx[x] = v.x;
       by[x] = v.y;
     }
   }
 }
 
-template <typename T>
-__global__ void k_up_bilin(const UpcontArgs<T> u) {
+// Bilinear R -> IR (UpContinuation_gpu.cu:42-110) from the regular results fx, fy (rows of
+// pitch XR, slot stride sX; FP64 from the line chain, the field type from the tiled chain),
+// arithmetic in FP64.
+template <typename T, typename B>
+__global__ void k_up_bilin(const UpcontArgs<T> u, const B *fx, const B *fy) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x, slot = blockIdx.y;
   const int L0 = u.L, M0 = u.M, L1 = u.L1, M1 = u.M1, XR = u.XR;
   const double norm = (double)u.L1 * (double)M1;
@@ -566,10 +622,10 @@ __global__ void k_up_bilin(const UpcontArgs<T> u) {
     const int iy = t / (L0 + 1), ix = t % (L0 + 1);
     const int II = min(max(u.xBx2N[ix], 1), L1 - 1), JJ = min(max(u.yo2N[iy], 1), M1 - 1);
     const double Xp = (ix == L0) ? u.X[L0 - 1] + 0.5 * u.dx[L0 - 1] : u.X[ix] - 0.5 * u.dx[ix];
-    const double *F = u.bxr + wo;
+    const B *F = fx + wo;
     double l1 = (Xp - u.Xn[II - 1]) / (u.Xn[II] - u.Xn[II - 1]);
-    const double a0 = (1.0 - l1) * F[(long long)(JJ - 1) * XR + II - 1] + l1 * F[(long long)(JJ - 1) * XR + II];
-    const double a1 = (1.0 - l1) * F[(long long)JJ * XR + II - 1] + l1 * F[(long long)JJ * XR + II];
+    const double a0 = (1.0 - l1) * (double)F[(long long)(JJ - 1) * XR + II - 1] + l1 * (double)F[(long long)(JJ - 1) * XR + II];
+    const double a1 = (1.0 - l1) * (double)F[(long long)JJ * XR + II - 1] + l1 * (double)F[(long long)JJ * XR + II];
     l1 = (u.Y[iy] - u.Yn[JJ - 1]) / (u.Yn[JJ] - u.Yn[JJ - 1]);
     u.bxair[po + (long long)iy * u.PX + ix] = (T)(((1.0 - l1) * a0 + l1 * a1) / norm);
   } else if (t < nbx + (M0 + 1) * L0) {
@@ -577,10 +633,10 @@ __global__ void k_up_bilin(const UpcontArgs<T> u) {
     const int iy = tt / L0, ix = tt % L0;
     const int II = min(max(u.xo2N[ix], 1), L1 - 1), JJ = min(max(u.yBy2N[iy], 1), M1 - 1);
     const double Yp = (iy == M0) ? u.Y[M0 - 1] + 0.5 * u.dy[M0 - 1] : u.Y[iy] - 0.5 * u.dy[iy];
-    const double *F = u.byr + wo;
+    const B *F = fy + wo;
     double l1 = (u.X[ix] - u.Xn[II - 1]) / (u.Xn[II] - u.Xn[II - 1]);
-    const double a0 = (1.0 - l1) * F[(long long)(JJ - 1) * XR + II - 1] + l1 * F[(long long)(JJ - 1) * XR + II];
-    const double a1 = (1.0 - l1) * F[(long long)JJ * XR + II - 1] + l1 * F[(long long)JJ * XR + II];
+    const double a0 = (1.0 - l1) * (double)F[(long long)(JJ - 1) * XR + II - 1] + l1 * (double)F[(long long)(JJ - 1) * XR + II];
+    const double a1 = (1.0 - l1) * (double)F[(long long)JJ * XR + II - 1] + l1 * (double)F[(long long)JJ * XR + II];
     l1 = (Yp - u.Yn[JJ - 1]) / (u.Yn[JJ] - u.Yn[JJ - 1]);
     u.byair[po + (long long)iy * u.PX + ix] = (T)(((1.0 - l1) * a0 + l1 * a1) / norm);
   }
@@ -622,14 +678,15 @@ struct Tile {
   int w, q;
   size_t lds;
 };
+template <typename T>
 Tile tile_shape(int m, long long nlines) {
   constexpr size_t kHalfCu = 80 * 1024;
   if (nlines >= 1024) {
-    if (tile_lds(m, 4) <= kHalfCu) return {64, 4, tile_lds(m, 4)};
-    if (tile_lds(m, 2) <= kHalfCu) return {64, 2, tile_lds(m, 2)};
-    return {64, 1, tile_lds(m, 1)};
+    if (tile_lds<T>(m, 4) <= kHalfCu) return {64, 4, tile_lds<T>(m, 4)};
+    if (tile_lds<T>(m, 2) <= kHalfCu) return {64, 2, tile_lds<T>(m, 2)};
+    return {64, 1, tile_lds<T>(m, 1)};
   }
-  return {256, 1, tile_lds(m, 1)};
+  return {256, 1, tile_lds<T>(m, 1)};
 }
 
 template <typename T, int E>
@@ -642,7 +699,7 @@ hipError_t launch_cols_e(const UpcontArgs<T> &u, const Tile &t, int nslot, hipSt
 
 template <typename T>
 hipError_t launch_cols(const UpcontArgs<T> &u, int nslot, hipStream_t s) {
-  const Tile t = tile_shape(u.py.m, (long long)u.H * nslot);
+  const Tile t = tile_shape<T>(u.py.m, (long long)u.H * nslot);
   const int need = (u.M1 + t.w - 1) / t.w;  // column entries per thread
   if (need <= 1) return launch_cols_e<T, 1>(u, t, nslot, s);
   if (need <= 2) return launch_cols_e<T, 2>(u, t, nslot, s);
@@ -655,14 +712,14 @@ hipError_t launch_cols(const UpcontArgs<T> &u, int nslot, hipStream_t s) {
 template <typename T>
 hipError_t launch_tiled(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEvent_t e0) {
   const int npair = (u.M + 1) / 2;
-  Tile t = tile_shape(u.px.m, (long long)npair * nslot);
+  Tile t = tile_shape<T>(u.px.m, (long long)npair * nslot);
   hipError_t e = allow_lds(k_up_rows_fwd<T>, t.lds);
   if (e != hipSuccess) return e;
   hipExtLaunchKernelGGL((k_up_rows_fwd<T>), dim3((npair + t.q - 1) / t.q, nslot), dim3(t.w, t.q),
                         (std::uint32_t)t.lds, s, e0, nullptr, 0u, u);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = launch_cols<T>(u, nslot, s)) != hipSuccess) return e;
-  t = tile_shape(u.px.m, (long long)u.M1 * nslot);
+  t = tile_shape<T>(u.px.m, (long long)u.M1 * nslot);
   if ((e = allow_lds(k_up_rows_inv<T>, t.lds)) != hipSuccess) return e;
   hipLaunchKernelGGL((k_up_rows_inv<T>), dim3((u.M1 + t.q - 1) / t.q, nslot), dim3(t.w, t.q), t.lds, s, u);
   return hipGetLastError();
@@ -699,7 +756,12 @@ hipError_t launch_upcont(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEv
   hipError_t e = u.tiled ? launch_tiled<T>(u, nslot, s, e0) : launch_lined<T>(u, nslot, s, e0);
   if (e != hipSuccess) return e;
   const int n = u.M * (u.L + 1) + (u.M + 1) * u.L;
-  hipExtLaunchKernelGGL((k_up_bilin<T>), dim3((n + 255) / 256, nslot), dim3(256), 0u, s, nullptr, e1, 0u, u);
+  if (u.tiled)
+    hipExtLaunchKernelGGL((k_up_bilin<T, T>), dim3((n + 255) / 256, nslot), dim3(256), 0u, s, nullptr, e1, 0u, u,
+                          (const T *)u.tbx, (const T *)u.tby);
+  else
+    hipExtLaunchKernelGGL((k_up_bilin<T, double>), dim3((n + 255) / 256, nslot), dim3(256), 0u, s, nullptr, e1, 0u,
+                          u, (const double *)u.bxr, (const double *)u.byr);
   return hipGetLastError();
 }
 

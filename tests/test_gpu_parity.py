@@ -118,6 +118,21 @@ def test_upcont_plane_long_x_fp64():
     assert m.L1 == 1156
 
 
+@pytest.mark.parametrize("kw", [{}, {"nx": 23, "ny": 17}, POW2_DECKS["128x64"]],
+                         ids=["stretched", "bluestein", "128x64"])
+def test_upcont_plane_fp32(kw):
+    """FP32 contexts run the tiled chain in FP32 (twiddles rounded once from FP64): within
+    single-precision FFT error of the FP64 oracle."""
+    so, st, m, cond = _pair(kw)
+    bz0 = np.random.default_rng(8).standard_normal((m.M, m.L))
+    bxo, byo = m.upcont_plane(bz0)
+    with T.Context(st, cond, T.FP32) as c:
+        bx, by = c.upcont_plane(bz0)
+    ex, ey = decks.rel_l2(bx, bxo), decks.rel_l2(by, byo)
+    print("FP32 air-plane chain vs FP64 oracle: Bx %.2e By %.2e" % (ex, ey))
+    assert ex < 2e-5 and ey < 2e-5
+
+
 def test_upcont_plane_marine_1d_fp64():
     """The 256 x 256 air plane of the 64 x 64 x 48 marine model (three radix-8/4 passes)."""
     from tcsem_fdtd_amd import workloads as W

@@ -34,3 +34,22 @@ def test_marine_3d_full_run_fp32_vs_fp64():
     print("steps %d, per-receiver FP32 trace error: max %.3e median %.3e" % (n32, err.max(), np.median(err)),
           flush=True)
     assert np.all(np.isfinite(got)) and np.all(err < 1e-4), err
+
+
+def test_marine_1d_full_run_vs_oracle():
+    """The 64 x 64 x 48 marine model (1000 m of seawater over sediment, 21 seafloor receivers) run
+    to its last sample (3755 steps) on the HIP path in FP64 and FP32 and in the FP64 oracle."""
+    from oracle import oracle as O
+    deck = W.marine_1d(T)
+    so = O.make_setup(**deck.setup)
+    r = deck.rxtime
+    m = O.Model(so, deck.cond)
+    ref, nref = m.run(O.make_rxtime(r["T0"], r["Tn"], r["Nt"], r["is_log"], r["comps"]), deck.rx,
+                      O.make_tx(**deck.txs[0]))
+    for precision, tol in ((T.FP64, 1e-9), (T.FP32, 1e-4)):
+        got, n = _run(deck, precision)
+        assert n == nref
+        err = decks.trace_error(got, ref)
+        print("marine_1d FP%d vs oracle over %d steps: max %.3e median %.3e" % (precision, n, err.max(),
+                                                                             np.median(err)), flush=True)
+        assert np.all(np.isfinite(got)) and np.all(err < tol), err
