@@ -49,8 +49,9 @@ extern "C" {
 
 /* ---- precision of the field state --------------------------------------------------------- */
 #define TCSEM_FP64 64         /* fields, coefficients and sums in FP64 (reference precision)  */
-#define TCSEM_FP32 32         /* FP32 fields; FP64 coefficients, schedule, air-plane FFT;
-                                 compensated FP32 Bz integral                                  */
+#define TCSEM_FP32 32         /* FP32 fields and air-plane FFT (tables rounded from FP64); FP64
+                                 coefficients, schedule, receiver splines; compensated FP32 Bz
+                                 integral                                                      */
 
 /* ---- field component ids (tcsem_get_field / tcsem_set_field) ------------------------------ */
 #define TCSEM_EX 0            /* (N+1, M+1, L)   x fastest */

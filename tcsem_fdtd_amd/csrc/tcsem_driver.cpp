@@ -9,7 +9,7 @@
 //     uninitialised traces);
 //   * transmitters are advanced --batch at a time in one set of launches (independent field sets
 //     sharing the conductivity);
-//   * --precision 32 runs FP32 fields (FP64 schedule, coefficients, air-plane FFT);
+//   * --precision 32 runs FP32 fields and air-plane FFT (FP64 schedule, coefficients, receivers);
 //   * --autotune times every march tiling for the batch size first and keeps the fastest.
 // All device work goes through the C ABI of libtcsem.so; errors print the library message and
 // exit with the (positive) error code.

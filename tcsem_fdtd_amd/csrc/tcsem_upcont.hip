@@ -1,8 +1,9 @@
 // tcsem_upcont.hip -- gfx950 kernels of the air boundary: upward continuation of Bz(0) to
 // BxAir/ByAir at z = -dz0/2 (UpContinuation_gpu.cu:166-274).
 //
-// The operator is linear in Bz(0) and runs in FP64 whatever the field precision.  Two chains
-// compute it; both end in k_up_bilin.
+// The operator is linear in Bz(0).  Two chains compute it; both end in k_up_bilin (FP64
+// arithmetic).  The tiled chain computes in the field precision T (its FFT twiddles, Bluestein
+// chirps, window weights and gains are FP64 tables, rounded once on use); the line chain in FP64.
 //
 // Tiled chain (both FFT working lengths <= 2048), four kernels, every global access coalesced:
 //   k_up_rows_fwd  one wavefront per pair of irregular rows: the natural-spline IR->R resampling
