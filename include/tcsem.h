@@ -194,6 +194,27 @@ int tcsem_timing(const tcsem_ctx *ctx, double t[4], long n[4]);
 int tcsem_enable_timing(tcsem_ctx *ctx, int on);
 int tcsem_sync(tcsem_ctx *ctx);
 
+/* ---- model batch (no counterpart in the reference, which runs one model per process) --------
+ * Conductivity models of uneven grids, one transmitter each, advanced together: every step's
+ * volume march, air-plane chain and surface level run as one launch over all members.  Each
+ * member is a context of its own (tcsem_create, tcsem_set_receivers, tcsem_set_transmitters),
+ * begun with one transmitter (tcsem_begin(ctx, tx, 1)); all share the precision, have N >= 6
+ * levels and air planes whose FFT lengths are at most 2048.  Members stay owned by the caller
+ * and must outlive the batch; their traces are read with tcsem_get_traces(ctx, 0, ...) as
+ * usual.  A member whose receivers have all been sampled is left out of later calls.
+ *   tcsem_batch_autotune: times nsteps batch steps for every rows x prefetch tiling (before the
+ *     first step; the members' fields are zeroed again) and keeps the fastest.
+ *   tcsem_batch_step: nsteps steps of every member still sampling (per member as tcsem_step).
+ *   tcsem_batch_run: steps until every member has taken all samples (max_steps > 0 caps the
+ *     member step count; members finish at the end of a 256-step chunk). */
+typedef struct tcsem_batch tcsem_batch;
+int tcsem_batch_create(tcsem_ctx *const *models, int nmodel, tcsem_batch **out);
+int tcsem_batch_autotune(tcsem_batch *batch, int nsteps, int *rows, int *pf, double *ms);
+int tcsem_batch_step(tcsem_batch *batch, int nsteps);
+int tcsem_batch_run(tcsem_batch *batch, int max_steps, int *steps_done);
+int tcsem_batch_sync(tcsem_batch *batch);
+void tcsem_batch_destroy(tcsem_batch *batch);
+
 #ifdef __cplusplus
 }
 #endif

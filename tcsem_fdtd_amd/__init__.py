@@ -111,6 +111,12 @@ def lib():
         "tcsem_timing": (C.c_int, [P, dp, C.POINTER(C.c_long)]),
         "tcsem_enable_timing": (C.c_int, [P, C.c_int]),
         "tcsem_sync": (C.c_int, [P]),
+        "tcsem_batch_create": (C.c_int, [C.POINTER(P), C.c_int, C.POINTER(P)]),
+        "tcsem_batch_autotune": (C.c_int, [P, C.c_int, ip, ip, dp]),
+        "tcsem_batch_step": (C.c_int, [P, C.c_int]),
+        "tcsem_batch_run": (C.c_int, [P, C.c_int, ip]),
+        "tcsem_batch_sync": (C.c_int, [P]),
+        "tcsem_batch_destroy": (None, [P]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -356,3 +362,52 @@ class Context:
 
     def sync(self):
         _check(self.L_.tcsem_sync(self.h))
+
+
+# ---- model batch -----------------------------------------------------------------------------
+class Batch:
+    """Models of uneven grids advanced together, one launch per kernel for all of them.  Each
+    member is a Context with receivers and transmitters set; begin(tx) starts every member's run
+    with its transmitter tx[q] (default 0) and creates the batch."""
+
+    def __init__(self, contexts, tx=None):
+        self.L_ = lib()
+        self.members = list(contexts)
+        for q, c in enumerate(self.members):
+            c.begin(0 if tx is None else tx[q], 1)
+        hs = (C.c_void_p * len(self.members))(*[c.h.value for c in self.members])
+        h = C.c_void_p()
+        _check(self.L_.tcsem_batch_create(hs, len(self.members), C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L_.tcsem_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def autotune(self, nsteps=10):
+        r, p, ms = C.c_int(), C.c_int(), C.c_double()
+        _check(self.L_.tcsem_batch_autotune(self.h, nsteps, C.byref(r), C.byref(p), C.byref(ms)))
+        return {"rows": r.value, "pf": p.value, "step_ms": ms.value}
+
+    def step(self, n=1):
+        _check(self.L_.tcsem_batch_step(self.h, n))
+
+    def run(self, max_steps=0):
+        """Steps until every member has all samples; returns each member's [nr][NComp][Nt]."""
+        steps = C.c_int()
+        _check(self.L_.tcsem_batch_run(self.h, max_steps, C.byref(steps)))
+        self.steps_done = steps.value
+        return [c.traces(0) for c in self.members]
+
+    def sync(self):
+        _check(self.L_.tcsem_batch_sync(self.h))

@@ -1068,16 +1068,110 @@ hipError_t enqueue_march(tcsem_ctx *c, const MarchArgs<T> &a, int nb, hipStream_
   return launch_march<T>(hi, nb, st, te[2], te[3]);
 }
 
+// Step scalars of the context's current step (main.cu:223-260).
 template <typename T>
-int do_step(tcsem_ctx *c) {
+void step_scalars(const tcsem_ctx *c, MarchArgs<T> &a) {
   const tcsem_setup &s = c->setup;
   const int ns = s.n_src / 2;
-  const double miu = kMu0, minsita = 0.1, mindx = c->g.mindx;
-  MarchArgs<T> a = march_args<T>(c);
-  a.lam = 7.5 * s.sfa2 * c->dt2 * c->dt1 / miu / mindx / mindx;  // main.cu:223
+  const double mindx = c->g.mindx;
+  a.lam = 7.5 * s.sfa2 * c->dt2 * c->dt1 / kMu0 / mindx / mindx;  // main.cu:223
   a.dt = c->dt1;
   a.dtb = 0.5 * (c->dt1 + c->dt2);
   a.amp = c->n < 2 * ns ? c->wwave[c->n] : 0.0;
+}
+
+// Time advance after a step (main.cu:253-260).
+void advance_time(tcsem_ctx *c) {
+  const tcsem_setup &s = c->setup;
+  const int ns = s.n_src / 2;
+  const double minsita = 0.1;
+  c->toT += c->dt1;
+  c->dt1 = c->dt2;
+  c->n++;
+  if (c->n < 2 * ns) c->dt2 = c->wdt[c->n + 1];
+  else c->dt2 = std::min(0.15 * s.sfa1 * std::sqrt(kMu0 * c->toT * minsita / 8.) * c->g.mindx, 1.2 * c->dt1);
+  c->steps++;
+}
+
+// Receiver sampling after a step (EFInterp2Rx(t0 = toT + dt1, dt = dt2)): the gathers of the
+// events that fire, on stream st2.  timed: record their intervals.
+template <typename T>
+int sample_rx(tcsem_ctx *c, hipStream_t st2, bool timed) {
+  if (!c->have_rx) return TCSEM_OK;
+  const int nb = c->nbatch;
+  const double t0 = c->toT + c->dt1;
+  GatherArgs gA{};
+  const long long org = c->PX + c->xm;
+  const long long fs = c->fstride * c->nslots;
+  const T *f = (const T *)c->d_fields + (long long)c->cur * 5 * fs;
+  for (int q = 0; q < 5; ++q) gA.f[q] = f + q * fs + org;
+  gA.f[5] = nullptr;
+  gA.fstride = c->fstride;
+  gA.PS = c->PS;
+  gA.L = c->g.info.L;
+  gA.M = c->g.info.M;
+  gA.N = c->g.info.N;
+  gA.PX = c->PX;
+  const tcsem_grid_info &I = c->g.info;
+  gA.X = c->d_grid;
+  gA.Y = c->d_grid + I.L;
+  gA.Z = c->d_grid + I.L + I.M;
+  gA.dx = c->d_grid + I.L + I.M + I.N;
+  gA.dy = gA.dx + I.L;
+  gA.dz = gA.dy + I.M;
+  gA.rx_first = c->d_rxl + 0;
+  gA.rx_count = c->d_rxl + nb;
+  gA.rx_local = c->d_rxl + 2 * nb;
+  gA.rxx = c->d_rx;
+  gA.rxy = c->d_rx + c->nrx;
+  gA.rxz = c->d_rx + 2 * c->nrx;
+  gA.rix = c->d_rxi;
+  gA.riy = c->d_rxi + c->nrx;
+  gA.riz = c->d_rxi + 2 * c->nrx;
+  gA.ncomp = c->ncomp;
+  gA.nt = c->rt.Nt;
+  gA.ef0 = c->d_ef0;
+  gA.efs = c->d_efs;
+  gA.nrmax = c->nrmax;
+  const bool has_b = c->rt.is_comp[3] || c->rt.is_comp[4] || c->rt.is_comp[5];
+  for (int fld = 0; fld < 2; ++fld) {
+    if (fld == 0 && c->ne >= c->rt.Nt) continue;
+    if (fld == 1 && (c->nb >= c->rt.Nt || !has_b)) continue;
+    Event ev{};
+    const size_t before = c->op_it.size();
+    if (!sample_group(c, fld, fld == 0 ? t0 : t0 + 0.5 * c->dt2, c->op_it, c->op_l1, ev)) continue;
+    // upload the new ops (few per event)
+    if (ev.op_count > 0) {
+      HIPCHK(hipMemcpyAsync(c->d_ops + before, c->op_it.data() + before, ev.op_count * sizeof(int),
+                            hipMemcpyHostToDevice, st2));
+      HIPCHK(hipMemcpyAsync(c->d_opl1 + before, c->op_l1.data() + before, ev.op_count * sizeof(double),
+                            hipMemcpyHostToDevice, st2));
+    }
+    const int c0 = fld == 0 ? 0 : c->nefcmp, c1 = fld == 0 ? c->nefcmp : c->ncomp;
+    if (c1 <= c0) continue;
+    gA.ncomp_group = c1 - c0;
+    gA.comp_base = c0;
+    for (int q = 0; q < gA.ncomp_group; ++q) gA.comps[q] = c->comps[c0 + q];
+    gA.slot_store = ev.slot_store;
+    gA.op_it = c->d_ops + ev.op_first;
+    gA.op_l1 = c->d_opl1 + ev.op_first;
+    gA.nops = ev.op_count;
+    hipEvent_t g0 = nullptr, g1 = nullptr;
+    if (timed) {
+      g0 = timing_event(c);
+      g1 = timing_event(c);
+      if (!g0 || !g1) return fail(TCSEM_EDEVICE, "timing event creation failed");
+      c->spans.push_back({3, 1, g0, g1});
+    }
+    HIPCHK(launch_gather<T>(gA, fld, nb, march_args<T>(c), st2, g0, g1));
+  }
+  return TCSEM_OK;
+}
+
+template <typename T>
+int do_step(tcsem_ctx *c) {
+  MarchArgs<T> a = march_args<T>(c);
+  step_scalars<T>(c, a);
   const UpcontArgs<T> &u = up_of<T>(c);
   const int nb = c->nbatch, N = c->g.info.N;
   hipStream_t st = c->stream, st2 = c->stream2;
@@ -1103,82 +1197,8 @@ int do_step(tcsem_ctx *c) {
   HIPCHK(launch_upcont<T>(u, nb, st2, te[4], te[5]));
   HIPCHK(launch_surface<T>(a, nb, st2, te[6], te[7]));
   c->cur ^= 1;  // the new state is current
-  // time advance (main.cu:253-260)
-  c->toT += c->dt1;
-  c->dt1 = c->dt2;
-  c->n++;
-  if (c->n < 2 * ns) c->dt2 = c->wdt[c->n + 1];
-  else c->dt2 = std::min(0.15 * s.sfa1 * std::sqrt(miu * c->toT * minsita / 8.) * mindx, 1.2 * c->dt1);
-  c->steps++;
-  // receiver sampling (EFInterp2Rx(t0 = toT + dt1, dt = dt2))
-  if (c->have_rx) {
-    const double t0 = c->toT + c->dt1;
-    GatherArgs gA{};
-    const long long org = c->PX + c->xm;
-    const long long fs = c->fstride * c->nslots;
-    const T *f = (const T *)c->d_fields + (long long)c->cur * 5 * fs;
-    for (int q = 0; q < 5; ++q) gA.f[q] = f + q * fs + org;
-    gA.f[5] = nullptr;
-    gA.fstride = c->fstride;
-    gA.PS = c->PS;
-    gA.L = c->g.info.L;
-    gA.M = c->g.info.M;
-    gA.N = c->g.info.N;
-    gA.PX = c->PX;
-    const tcsem_grid_info &I = c->g.info;
-    gA.X = c->d_grid;
-    gA.Y = c->d_grid + I.L;
-    gA.Z = c->d_grid + I.L + I.M;
-    gA.dx = c->d_grid + I.L + I.M + I.N;
-    gA.dy = gA.dx + I.L;
-    gA.dz = gA.dy + I.M;
-    gA.rx_first = c->d_rxl + 0;
-    gA.rx_count = c->d_rxl + nb;
-    gA.rx_local = c->d_rxl + 2 * nb;
-    gA.rxx = c->d_rx;
-    gA.rxy = c->d_rx + c->nrx;
-    gA.rxz = c->d_rx + 2 * c->nrx;
-    gA.rix = c->d_rxi;
-    gA.riy = c->d_rxi + c->nrx;
-    gA.riz = c->d_rxi + 2 * c->nrx;
-    gA.ncomp = c->ncomp;
-    gA.nt = c->rt.Nt;
-    gA.ef0 = c->d_ef0;
-    gA.efs = c->d_efs;
-    gA.nrmax = c->nrmax;
-    const bool has_b = c->rt.is_comp[3] || c->rt.is_comp[4] || c->rt.is_comp[5];
-    for (int fld = 0; fld < 2; ++fld) {
-      if (fld == 0 && c->ne >= c->rt.Nt) continue;
-      if (fld == 1 && (c->nb >= c->rt.Nt || !has_b)) continue;
-      Event ev{};
-      const size_t before = c->op_it.size();
-      if (!sample_group(c, fld, fld == 0 ? t0 : t0 + 0.5 * c->dt2, c->op_it, c->op_l1, ev)) continue;
-      // upload the new ops (few per event)
-      if (ev.op_count > 0) {
-        HIPCHK(hipMemcpyAsync(c->d_ops + before, c->op_it.data() + before, ev.op_count * sizeof(int),
-                              hipMemcpyHostToDevice, st2));
-        HIPCHK(hipMemcpyAsync(c->d_opl1 + before, c->op_l1.data() + before, ev.op_count * sizeof(double),
-                              hipMemcpyHostToDevice, st2));
-      }
-      const int c0 = fld == 0 ? 0 : c->nefcmp, c1 = fld == 0 ? c->nefcmp : c->ncomp;
-      if (c1 <= c0) continue;
-      gA.ncomp_group = c1 - c0;
-      gA.comp_base = c0;
-      for (int q = 0; q < gA.ncomp_group; ++q) gA.comps[q] = c->comps[c0 + q];
-      gA.slot_store = ev.slot_store;
-      gA.op_it = c->d_ops + ev.op_first;
-      gA.op_l1 = c->d_opl1 + ev.op_first;
-      gA.nops = ev.op_count;
-      hipEvent_t g0 = nullptr, g1 = nullptr;
-      if (timed) {
-        g0 = timing_event(c);
-        g1 = timing_event(c);
-        if (!g0 || !g1) return fail(TCSEM_EDEVICE, "timing event creation failed");
-        c->spans.push_back({3, 1, g0, g1});
-      }
-      HIPCHK(launch_gather<T>(gA, fld, nb, march_args<T>(c), st2, g0, g1));
-    }
-  }
+  advance_time(c);
+  if (int rc = sample_rx<T>(c, st2, timed)) return rc;
   HIPCHK(hipEventRecord(c->xev[1], st2));
   if (timed) {
     // the march of a step is its lower and top launches (not the wait between them for the
@@ -1294,6 +1314,291 @@ int autotune_march(tcsem_ctx *c, int nbatch, int nsteps, int *best_by, int *best
   *best_ms = pick->ms;
   return TCSEM_OK;
 }
+
+// ---- model batch ----------------------------------------------------------------------------------
+// Models of uneven grids, one transmitter each (every member a context begun with one slot), stepped
+// together: each step's volume march, air-plane chain and surface level run as one launch over all
+// members, from per-member argument records (2 parities of MarchArgs: the member's own fields,
+// tables, sizes and tiling; one UpcontArgs each), with the step scalars of every member in a
+// device table (field-independent, precomputed per call).  Source passes and receiver gathers
+// stay per member.
+struct tcsem_batch {
+  std::vector<tcsem_ctx *> m;
+  int prec = 32;
+  int by = 0, pf = 0;  // autotuned march tiling (0: heuristic)
+  hipStream_t stream = nullptr, stream2 = nullptr;
+  hipEvent_t xev[2] = {}, done = nullptr;
+  char *d_geo = nullptr;     // 2 x nmodel MarchArgs (parity-major)
+  char *d_ugeo = nullptr;    // nmodel UpcontArgs
+  double *d_scal = nullptr;  // steps x nmodel x (lam, dt, dtb, amp)
+  int scal_cap = 0;
+  std::vector<char> active;  // per member: still sampling
+  int steps = 0;
+};
+
+namespace {
+
+int batch_fail_if(bool bad, const char *msg) { return bad ? fail(TCSEM_ESTATE, "%s", msg) : TCSEM_OK; }
+
+// Common tiling of a batch step: the autotuned one, else 16-row tiles once the batch's tiles give
+// two workgroups per CU, else 4 (FP64 at most 8); TCSEM_MARCH_ROWS / _PF override either.
+void batch_tiling(const tcsem_batch *b, int &by, int &pf) {
+  if (b->by) {
+    by = b->by;
+    pf = b->pf;
+  } else {
+    long long tiles = 0;
+    for (const tcsem_ctx *c : b->m)
+      tiles += (long long)((c->g.info.L + kMarchLanes - 1) / kMarchLanes) * ((c->g.info.M + 13) / 14);
+    by = tiles >= 2LL * device_cus() ? 16 : 4;
+    if (b->prec == 64 && by > 8) by = 8;
+    pf = (by == 16 || b->prec == 64) ? 1 : 2;
+  }
+  if (const char *e = std::getenv("TCSEM_MARCH_ROWS")) {
+    const int v = std::atoi(e);
+    if (v == 4 || v == 8 || v == 16) by = v;
+  }
+  if (const char *e = std::getenv("TCSEM_MARCH_PF")) {
+    const int v = std::atoi(e);
+    if (v >= 1 && v <= 4) pf = v;
+  }
+  if (b->prec == 64 && by > 8) by = 8;
+}
+
+// Builds and uploads the member records for tiling (by, pf) and returns the launch records: the
+// largest sizes over the members (grid and LDS extents), the common tiling and level range.
+// The batch's streams must be idle (the records are overwritten).
+template <typename T>
+int batch_records(tcsem_batch *b, int by, int pf, MarchArgs<T> &la, UpcontArgs<T> &ul) {
+  const int nm = (int)b->m.size();
+  std::vector<MarchArgs<T>> rec(2 * (size_t)nm);
+  std::vector<UpcontArgs<T>> urec(nm);
+  la = MarchArgs<T>{};
+  ul = UpcontArgs<T>{};
+  for (int q = 0; q < nm; ++q) {
+    tcsem_ctx *c = b->m[q];
+    apply_tiling(c, by, pf, 1);
+    by = c->by;  // FP64 tiles clamped to 8 rows
+    const int cur0 = c->cur;
+    for (int p = 0; p < 2; ++p) {
+      c->cur = cur0 ^ p;
+      MarchArgs<T> a = march_args<T>(c);
+      if (!b->active[q]) a.nper = a.sper = 0;  // every workgroup of the member returns at once
+      rec[(size_t)p * nm + q] = a;
+    }
+    c->cur = cur0;
+    UpcontArgs<T> u = up_of<T>(c);
+    if (!b->active[q]) u.L = u.M = u.M1 = u.H = 0;
+    urec[q] = u;
+    const tcsem_grid_info &I = c->g.info;
+    la.L = std::max(la.L, I.L);
+    la.M = std::max(la.M, I.M);
+    la.N = std::max(la.N, I.N);
+    la.nper = std::max(la.nper, c->nper);
+    la.sper = std::max(la.sper, c->sper);
+    ul.L = std::max(ul.L, u.L);
+    ul.M = std::max(ul.M, u.M);
+    ul.M1 = std::max(ul.M1, u.M1);
+    ul.H = std::max(ul.H, u.H);
+    ul.px.m = std::max(ul.px.m, u.px.m);
+    ul.py.m = std::max(ul.py.m, u.py.m);
+  }
+  la.rows = by;
+  la.pf = pf;
+  la.nchunk = 0;
+  la.k_hi = la.N - 1;
+  la.k_lo = 1;
+  ul.tiled = 1;
+  HIPCHK(hipMemcpy(b->d_geo, rec.data(), rec.size() * sizeof(MarchArgs<T>), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(b->d_ugeo, urec.data(), urec.size() * sizeof(UpcontArgs<T>), hipMemcpyHostToDevice));
+  return TCSEM_OK;
+}
+
+int batch_scal_reserve(tcsem_batch *b, int nsteps) {
+  if (nsteps <= b->scal_cap) return TCSEM_OK;
+  (void)hipFree(b->d_scal);
+  b->d_scal = nullptr;
+  b->scal_cap = 0;
+  const size_t bytes = (size_t)nsteps * b->m.size() * 4 * sizeof(double);
+  if (hipMalloc((void **)&b->d_scal, bytes) != hipSuccess) return fail(TCSEM_ENOMEM, "step-scalar table allocation");
+  b->scal_cap = nsteps;
+  return TCSEM_OK;
+}
+
+// Waits for the members' own streams and the batch's, so records and tables can be rewritten.
+int batch_quiesce(tcsem_batch *b) {
+  for (tcsem_ctx *c : b->m) {
+    HIPCHK(hipStreamSynchronize(c->stream2));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(b->stream2));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return TCSEM_OK;
+}
+
+// One batch step on the batch's streams, as do_step for one context: lower march of every member,
+// (after the previous step's stream2 work) the top march, the members' source passes; on stream2
+// the air-plane chain, the surface level and the members' receiver gathers.  par: record parity.
+template <typename T>
+int batch_enqueue(tcsem_batch *b, const MarchArgs<T> &la, const UpcontArgs<T> &ul, int par, int row,
+                  bool live) {
+  const int nm = (int)b->m.size();
+  const MarchArgs<T> *geo = (const MarchArgs<T> *)b->d_geo + (size_t)par * nm;
+  const UpcontArgs<T> *ugeo = (const UpcontArgs<T> *)b->d_ugeo;
+  hipStream_t st = b->stream, st2 = b->stream2;
+  MarchArgs<T> lo = la, hi = la;
+  lo.gscal = hi.gscal = b->d_scal + (size_t)row * nm * 4;
+  lo.k_lo = 2;
+  lo.carry = 2;
+  hi.k_hi = 1;
+  hi.k_lo = 1;
+  hi.carry = 1;
+  HIPCHK(launch_march<T>(lo, nm, st, nullptr, nullptr, geo));
+  HIPCHK(hipStreamWaitEvent(st, b->xev[1], 0));
+  HIPCHK(launch_march<T>(hi, nm, st, nullptr, nullptr, geo));
+  if (live)
+    for (int q = 0; q < nm; ++q) {
+      tcsem_ctx *c = b->m[q];
+      if (!b->active[q] || !c->has_fix) continue;
+      MarchArgs<T> a = march_args<T>(c);
+      step_scalars<T>(c, a);
+      if (a.amp != 0.0) HIPCHK(launch_src_fix<T>(a, c->d_fixe, c->d_fixidx, c->d_fixb, c->d_fixidx + 2, 1, st));
+    }
+  HIPCHK(hipEventRecord(b->xev[0], st));
+  HIPCHK(hipStreamWaitEvent(st2, b->xev[0], 0));
+  HIPCHK(launch_upcont<T>(ul, nm, st2, nullptr, nullptr, ugeo));
+  HIPCHK(launch_surface<T>(hi, nm, st2, nullptr, nullptr, geo));
+  if (live)
+    for (int q = 0; q < nm; ++q) {
+      tcsem_ctx *c = b->m[q];
+      c->cur ^= 1;
+      advance_time(c);
+      if (b->active[q])
+        if (int rc = sample_rx<T>(c, st2, false)) return rc;
+    }
+  HIPCHK(hipEventRecord(b->xev[1], st2));
+  return TCSEM_OK;
+}
+
+// Orders the batch's work before what is issued next on the batch's and the members' streams.
+int batch_join(tcsem_batch *b) {
+  HIPCHK(hipEventRecord(b->xev[1], b->stream2));
+  HIPCHK(hipStreamWaitEvent(b->stream, b->xev[1], 0));
+  HIPCHK(hipEventRecord(b->done, b->stream));
+  for (tcsem_ctx *c : b->m) {
+    HIPCHK(hipStreamWaitEvent(c->stream, b->done, 0));
+    HIPCHK(hipStreamWaitEvent(c->stream2, b->done, 0));
+  }
+  return TCSEM_OK;
+}
+
+template <typename T>
+int batch_step_t(tcsem_batch *b, int nsteps) {
+  const int nm = (int)b->m.size();
+  if (int rc = batch_quiesce(b)) return rc;
+  for (int q = 0; q < nm; ++q) b->active[q] = !sampling_done(b->m[q]);
+  // every member's step scalars for the nsteps steps (the schedule is field-independent)
+  if (int rc = batch_scal_reserve(b, nsteps)) return rc;
+  std::vector<double> scal((size_t)nsteps * nm * 4, 0.0);
+  for (int q = 0; q < nm; ++q) {
+    tcsem_ctx *c = b->m[q];
+    const int n0 = c->n, st0 = c->steps;
+    const double t0 = c->toT, d1 = c->dt1, d2 = c->dt2;
+    for (int k = 0; k < nsteps; ++k) {
+      MarchArgs<T> a{};
+      step_scalars<T>(c, a);
+      double *r = &scal[((size_t)k * nm + q) * 4];
+      r[0] = a.lam;
+      r[1] = a.dt;
+      r[2] = a.dtb;
+      r[3] = a.amp;
+      advance_time(c);
+    }
+    c->n = n0;
+    c->steps = st0;
+    c->toT = t0;
+    c->dt1 = d1;
+    c->dt2 = d2;
+  }
+  HIPCHK(hipMemcpy(b->d_scal, scal.data(), scal.size() * sizeof(double), hipMemcpyHostToDevice));
+  int by = 0, pf = 0;
+  batch_tiling(b, by, pf);
+  MarchArgs<T> la;
+  UpcontArgs<T> ul;
+  if (int rc = batch_records<T>(b, by, pf, la, ul)) return rc;
+  for (int k = 0; k < nsteps; ++k)
+    if (int rc = batch_enqueue<T>(b, la, ul, k & 1, k, true)) return rc;
+  b->steps += nsteps;
+  return batch_join(b);
+}
+
+// Times batch steps (zeroed fields, no source, no sampling) for each rows x prefetch tiling and
+// keeps the fastest; the members' fields are zeroed again afterwards.
+template <typename T>
+int batch_autotune_t(tcsem_batch *b, int nsteps, int *best_by, int *best_pf, double *best_ms) {
+  const int nm = (int)b->m.size();
+  if (int rc = batch_quiesce(b)) return rc;
+  for (int q = 0; q < nm; ++q) b->active[q] = 1;
+  if (int rc = batch_scal_reserve(b, 1)) return rc;
+  std::vector<double> scal((size_t)nm * 4);
+  for (int q = 0; q < nm; ++q) {
+    scal[4 * q + 0] = 1.0;
+    scal[4 * q + 1] = 1e-6;
+    scal[4 * q + 2] = 1e-6;
+    scal[4 * q + 3] = 0.0;
+  }
+  HIPCHK(hipMemcpy(b->d_scal, scal.data(), scal.size() * sizeof(double), hipMemcpyHostToDevice));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(TCSEM_EDEVICE, "event creation failed");
+  }
+  std::vector<int> rows_c = b->prec == 64 ? std::vector<int>{4, 8} : std::vector<int>{4, 8, 16};
+  double best = 1e300;
+  int rc = TCSEM_OK;
+  for (int by : rows_c) {
+    for (int pf = 1; pf <= 4 && rc == TCSEM_OK; ++pf) {
+      MarchArgs<T> la;
+      UpcontArgs<T> ul;
+      if ((rc = batch_records<T>(b, by, pf, la, ul))) break;
+      int par = 0;
+      if ((rc = batch_enqueue<T>(b, la, ul, par, 0, false))) break;  // first launches of this tiling
+      par ^= 1;
+      hipError_t e = hipStreamWaitEvent(b->stream, b->xev[1], 0);
+      if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
+      for (int k = 0; k < nsteps && e == hipSuccess && rc == TCSEM_OK; ++k, par ^= 1)
+        rc = batch_enqueue<T>(b, la, ul, par, 0, false);
+      if (rc) break;
+      if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->xev[1], 0);
+      if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) {
+        rc = fail(TCSEM_EDEVICE, "batch autotune: %s", hipGetErrorString(e));
+        break;
+      }
+      const double per = (double)ms / nsteps;
+      if (per < best) {
+        best = per;
+        *best_by = b->prec == 64 ? std::min(by, 8) : by;
+        *best_pf = pf;
+      }
+    }
+    if (rc) break;
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(b->stream2));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  for (tcsem_ctx *c : b->m)
+    if ((rc = tcsem_reset_fields(c, 0, 1))) return rc;
+  *best_ms = best;
+  return TCSEM_OK;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -2256,6 +2561,111 @@ int tcsem_timing(const tcsem_ctx *cc, double t[4], long n[4]) {
 int tcsem_sync(tcsem_ctx *c) {
   if (check_ctx(c)) return TCSEM_EINVAL;
   HIPCHK(hipStreamSynchronize(c->stream));
+  return TCSEM_OK;
+}
+
+// ---- model batch -------------------------------------------------------------------------------
+int tcsem_batch_create(tcsem_ctx *const *models, int nmodel, tcsem_batch **out) {
+  if (!models || !out || nmodel < 1) return fail(TCSEM_EINVAL, "null argument or empty batch");
+  *out = nullptr;
+  for (int q = 0; q < nmodel; ++q) {
+    const tcsem_ctx *c = models[q];
+    if (!c) return fail(TCSEM_EINVAL, "null member %d", q);
+    if (!c->begun || c->nbatch != 1)
+      return fail(TCSEM_ESTATE, "member %d: begin it with one transmitter (tcsem_begin(ctx, tx, 1)) first", q);
+    if (c->prec != models[0]->prec) return fail(TCSEM_EINVAL, "members differ in precision");
+    if (c->g.info.N < 6) return fail(TCSEM_ESIZE, "member %d: a batch needs at least 6 levels", q);
+    if (!c->up_tiled) return fail(TCSEM_ESIZE, "member %d: air plane beyond the tiled chain (FFT length > 2048)", q);
+    for (int r = 0; r < q; ++r)
+      if (models[r] == c) return fail(TCSEM_EINVAL, "member %d appears twice", q);
+  }
+  tcsem_batch *b = new (std::nothrow) tcsem_batch();
+  if (!b) return fail(TCSEM_ENOMEM, "batch allocation");
+  b->m.assign(models, models + nmodel);
+  b->prec = models[0]->prec;
+  b->active.assign(nmodel, 1);
+  const size_t rsz = b->prec == 64 ? sizeof(MarchArgs<double>) : sizeof(MarchArgs<float>);
+  const size_t usz = b->prec == 64 ? sizeof(UpcontArgs<double>) : sizeof(UpcontArgs<float>);
+  bool ok = hipStreamCreate(&b->stream) == hipSuccess && hipStreamCreate(&b->stream2) == hipSuccess &&
+            hipEventCreateWithFlags(&b->xev[0], hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&b->xev[1], hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&b->done, hipEventDisableTiming) == hipSuccess &&
+            hipMalloc((void **)&b->d_geo, 2 * nmodel * rsz) == hipSuccess &&
+            hipMalloc((void **)&b->d_ugeo, nmodel * usz) == hipSuccess;
+  if (!ok) {
+    tcsem_batch_destroy(b);
+    return fail(TCSEM_EDEVICE, "batch stream, event or record allocation failed");
+  }
+  *out = b;
+  return TCSEM_OK;
+}
+
+void tcsem_batch_destroy(tcsem_batch *b) {
+  if (!b) return;
+  if (b->stream2) (void)hipStreamSynchronize(b->stream2);
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  (void)hipFree(b->d_geo);
+  (void)hipFree(b->d_ugeo);
+  (void)hipFree(b->d_scal);
+  for (auto &e : b->xev)
+    if (e) (void)hipEventDestroy(e);
+  if (b->done) (void)hipEventDestroy(b->done);
+  if (b->stream2) (void)hipStreamDestroy(b->stream2);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+int tcsem_batch_autotune(tcsem_batch *b, int nsteps, int *rows, int *pf, double *ms) {
+  if (!b) return fail(TCSEM_EINVAL, "null batch");
+  if (nsteps < 1 || nsteps > 1000) return fail(TCSEM_EINVAL, "1 <= steps <= 1000");
+  for (const tcsem_ctx *c : b->m)
+    if (int rc = batch_fail_if(c->steps != 0, "autotune a batch before its first step")) return rc;
+  int by = 0, p = 0;
+  double t = 0.0;
+  const int rc = b->prec == 64 ? batch_autotune_t<double>(b, nsteps, &by, &p, &t)
+                               : batch_autotune_t<float>(b, nsteps, &by, &p, &t);
+  if (rc) return rc;
+  b->by = by;
+  b->pf = p;
+  if (rows) *rows = by;
+  if (pf) *pf = p;
+  if (ms) *ms = t;
+  return TCSEM_OK;
+}
+
+int tcsem_batch_step(tcsem_batch *b, int nsteps) {
+  if (!b) return fail(TCSEM_EINVAL, "null batch");
+  if (nsteps < 0) return fail(TCSEM_EINVAL, "negative step count");
+  if (nsteps == 0) return TCSEM_OK;
+  for (const tcsem_ctx *c : b->m)
+    if (int rc = batch_fail_if(!c->begun || c->nbatch != 1, "a member's run was restarted or ended")) return rc;
+  return b->prec == 64 ? batch_step_t<double>(b, nsteps) : batch_step_t<float>(b, nsteps);
+}
+
+int tcsem_batch_run(tcsem_batch *b, int max_steps, int *steps_done) {
+  if (!b) return fail(TCSEM_EINVAL, "null batch");
+  constexpr int kChunk = 256;  // steps per call of the step loop (members finish at a chunk's end)
+  for (;;) {
+    bool any = false;
+    int most = 0;
+    for (const tcsem_ctx *c : b->m) {
+      if (!c->have_rx) return fail(TCSEM_ESTATE, "set receivers of every member first");
+      if (!sampling_done(c)) any = true;
+      most = std::max(most, c->steps);
+    }
+    if (!any || (max_steps > 0 && most >= max_steps)) break;
+    const int n = max_steps > 0 ? std::min(kChunk, max_steps - most) : kChunk;
+    if (int rc = tcsem_batch_step(b, n)) return rc;
+  }
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (steps_done) *steps_done = b->steps;
+  return TCSEM_OK;
+}
+
+int tcsem_batch_sync(tcsem_batch *b) {
+  if (!b) return fail(TCSEM_EINVAL, "null batch");
+  HIPCHK(hipStreamSynchronize(b->stream2));
+  HIPCHK(hipStreamSynchronize(b->stream));
   return TCSEM_OK;
 }
 

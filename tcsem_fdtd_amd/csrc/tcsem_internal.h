@@ -73,6 +73,8 @@ struct MarchArgs {
   int nchunk, chunk_len;
   T *zs;
   long long zstride;
+  // model batch: lam, dt, dtb, amp of each slot for this step (4 FP64 per slot), launch record only
+  const double *gscal;
 };
 
 // Receiver gather descriptor (EFInterp2Rx.cu:60-470) for one field group (E or B) of one event.
@@ -147,12 +149,15 @@ struct UpcontArgs {
 // ---- launchers (tcsem_kernels.hip, tcsem_upcont.hip) ------------------------------------------
 // Optional events e0/e1 are recorded by the dispatch packets themselves (start of the first and
 // end of the last kernel of the launcher), so timing adds no packets to the stream.
+// geo (device array of nslot MarchArgs records, or null): a model batch of uneven grids, each
+// slot with its own fields, tables, sizes and tiling; a then carries the step scalars, the level
+// range and the largest N, nper and sper over the records.
 template <typename T>
 hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEvent_t e0 = nullptr,
-                        hipEvent_t e1 = nullptr);
+                        hipEvent_t e1 = nullptr, const MarchArgs<T> *geo = nullptr);
 template <typename T>
 hipError_t launch_surface(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEvent_t e0 = nullptr,
-                          hipEvent_t e1 = nullptr);
+                          hipEvent_t e1 = nullptr, const MarchArgs<T> *geo = nullptr);
 template <typename T>
 hipError_t launch_zsum(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEvent_t e0 = nullptr);
 template <typename T>
@@ -163,9 +168,11 @@ hipError_t launch_bz_volume(const MarchArgs<T> &a, T *bz, int slot, hipStream_t 
 template <typename T>
 hipError_t launch_gather(const GatherArgs &g, int is_b, int nslot, const MarchArgs<T> &a,
                          hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+// geo (device array of nslot UpcontArgs records, or null): the air planes of a model batch; u then
+// carries the largest L, M, L1, M1, H and FFT working lengths over the records (tiled chain only).
 template <typename T>
 hipError_t launch_upcont(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEvent_t e0 = nullptr,
-                         hipEvent_t e1 = nullptr);
+                         hipEvent_t e1 = nullptr, const UpcontArgs<T> *geo = nullptr);
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);
 
 }  // namespace tcsem

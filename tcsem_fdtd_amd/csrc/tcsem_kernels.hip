@@ -135,11 +135,11 @@ __device__ inline bool tile_of(int ntx, int nty, int nper, Tile &t) {
 // loads/stores at a byte offset from a uniform base (global_load ... vOff, s[base])
 template <typename T>
 __device__ __forceinline__ T ldo(const T *__restrict__ base, unsigned off) {
-  return *(const T *)((const char *)base + off);
+  return *(const __attribute__((address_space(1))) T *)((const char *)base + off);
 }
 template <typename T>
 __device__ __forceinline__ void sto(T *__restrict__ base, unsigned off, T v) {
-  *(T *)((char *)base + off) = v;
+  *(__attribute__((address_space(1))) T *)((char *)base + off) = v;
 }
 
 // Branch-free masked stores: a raw buffer resource over one slot's field drops any store whose
@@ -212,15 +212,39 @@ struct Lv {
 // needs come from two edge lanes: lane 0 keeps a second running Bz sum for column i0-1 and
 // fetches its sigma and By; lane 63 fetches column i0+64 and computes its new Ez.  They go to the
 // halo slots 0 and 65 of the LDS rows.
-template <typename T, int BY, int PF>
-__global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> a) {
+// A model batch (uneven grids, one transmitter each) passes one MarchArgs record per slot: the
+// slot's own fields, tables, sizes and tiling, with the launch's level range and the slot's step
+// scalars (lam, dt, dtb, amp) from the launch's table row la.gscal.
+template <typename T>
+__device__ inline MarchArgs<T> slot_args(const MarchArgs<T> &la, const MarchArgs<T> &g) {
+  MarchArgs<T> a = g;
+  const double *q = la.gscal + 4 * blockIdx.y;
+  a.lam = q[0];
+  a.dt = q[1];
+  a.dtb = q[2];
+  a.amp = q[3];
+  a.rows = la.rows;
+  a.pf = la.pf;
+  a.k_lo = la.k_lo;
+  a.carry = la.carry;
+  if (la.carry == 1) a.k_hi = la.k_hi;  // the top launch: level 1; the lower launch: N-1 of the slot
+  a.nchunk = 0;
+  a.chunk_len = 0;
+  return a;
+}
+
+template <typename T, int BY, int PF, bool GEO>
+__global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> la, const MarchArgs<T> *geo) {
+  MarchArgs<T> a;
+  if constexpr (GEO) a = slot_args(la, geo[blockIdx.y]);
+  else a = la;
   extern __shared__ __align__(16) unsigned char zdyn[];  // 3 N entries of T
   constexpr int LW = kMarchLanes + 2;                   // LDS row: halo, 64 lanes, halo
   __shared__ T sBx[2][BY][LW], sBy[2][BY][LW], sS[2][BY][LW];
   __shared__ T sBz[BY][LW], sEz[BY][LW];
   const int b = blockIdx.x;
   const int tile = (b & 7) * a.nper + (b >> 3);  // XCD-contiguous bands of tiles
-  if (tile >= a.ntx * a.nty) return;
+  if ((GEO && (b >> 3) >= a.nper) || tile >= a.ntx * a.nty) return;
   const int tx = tile % a.ntx, ty = tile / a.ntx;
   const int lx = threadIdx.x, ly = threadIdx.y;
   const int L = a.L, M = a.M, N = a.N;
@@ -249,7 +273,7 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> a)
   // keeps neighbouring tiles (halo rows and columns) close in time; the conductivity that the
   // slots share stays in the 256 MB last-level cache between them (measured: putting the slots of
   // a tile together in grid x made the 32-slot march 16 % slower)
-  const int slot = blockIdx.y;
+  const int slot = GEO ? 0 : (int)blockIdx.y;  // a model batch's records point at slot 0 of each model
   const long long fo = (long long)slot * a.fstride - (a.PX + a.xm);  // allocation base of the slot
   const T *__restrict__ exi = a.exi + fo;
   const T *__restrict__ eyi = a.eyi + fo;
@@ -424,17 +448,20 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> a)
 
 // Level 0: the air cells count as sigma = 0 and the level above is BxAir/ByAir at -dz0/2
 // (ElectricalFiled.cu:107-139, 204-234).
-template <typename T, int BY>
-__global__ void __launch_bounds__(kMarchLanes *BY) k_surface(const MarchArgs<T> a) {
+template <typename T, int BY, bool GEO>
+__global__ void __launch_bounds__(kMarchLanes *BY) k_surface(const MarchArgs<T> la, const MarchArgs<T> *geo) {
+  MarchArgs<T> a;
+  if constexpr (GEO) a = slot_args(la, geo[blockIdx.y]);
+  else a = la;
   __shared__ T sBx[BY][kMarchLanes], sBy[BY][kMarchLanes], sS[BY][kMarchLanes];
   __shared__ T sBz[BY][kMarchLanes], sEz[BY][kMarchLanes];
   Tile t;
-  if (!tile_of<BY>(a.stx, a.sty, a.sper, t)) return;
+  if ((GEO && (int)(blockIdx.x >> 3) >= a.sper) || !tile_of<BY>(a.stx, a.sty, a.sper, t)) return;
   Lane<T> ln;
   lane_setup<T, BY>(a, t, ln);
   const int lx = t.lx, ly = t.ly, i = ln.i, j = ln.j;
   const int L = a.L, M = a.M;
-  const int slot = blockIdx.y;
+  const int slot = GEO ? 0 : (int)blockIdx.y;
   const long long fo = (long long)slot * a.fstride - (a.PX + a.xm);
   const long long po = (long long)slot * a.pstride - (a.PX + a.xm);
   const unsigned off = ln.colb, PSb = (unsigned)a.PS * (unsigned)sizeof(T);
@@ -681,40 +708,50 @@ __global__ void k_fill_zero(uint32_t *p, size_t n) {
 
 // ---- launchers -----------------------------------------------------------------------------------
 // Raises a march kernel's dynamic-LDS limit once per size (static + dynamic may pass 64 KiB).
-template <typename T, int BY, int PF>
+template <typename T, int BY, int PF, bool GEO>
 hipError_t march_lds(size_t bytes) {
   static size_t allowed = 0;  // per instantiation; only grows
   if (bytes <= allowed) return hipSuccess;
-  const hipError_t e =
-      hipFuncSetAttribute((const void *)k_march<T, BY, PF>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+  const hipError_t e = hipFuncSetAttribute((const void *)k_march<T, BY, PF, GEO>,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   if (e == hipSuccess) allowed = bytes;
   return e;
 }
+// geo == nullptr: one model (slots in grid y share a's geometry); else a model batch (a.N, a.nper:
+// the largest over the slots' records, which carry the rest).
 template <typename T, int BY, int PF>
-hipError_t launch_march_pf(const MarchArgs<T> &a, dim3 grd, dim3 blk, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+hipError_t launch_march_pf(const MarchArgs<T> &a, const MarchArgs<T> *geo, dim3 grd, dim3 blk, hipStream_t s,
+                           hipEvent_t e0, hipEvent_t e1) {
   const size_t zl = (size_t)3 * a.N * sizeof(T);
-  const hipError_t e = march_lds<T, BY, PF>(zl);
-  if (e != hipSuccess) return e;
-  hipExtLaunchKernelGGL((k_march<T, BY, PF>), grd, blk, (std::uint32_t)zl, s, e0, e1, 0u, a);
+  hipError_t e;
+  if (geo) {
+    if ((e = march_lds<T, BY, PF, true>(zl)) != hipSuccess) return e;
+    hipExtLaunchKernelGGL((k_march<T, BY, PF, true>), grd, blk, (std::uint32_t)zl, s, e0, e1, 0u, a, geo);
+  } else {
+    if ((e = march_lds<T, BY, PF, false>(zl)) != hipSuccess) return e;
+    hipExtLaunchKernelGGL((k_march<T, BY, PF, false>), grd, blk, (std::uint32_t)zl, s, e0, e1, 0u, a, geo);
+  }
   return hipGetLastError();
 }
 template <typename T, int BY>
-hipError_t launch_march_by(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+hipError_t launch_march_by(const MarchArgs<T> &a, const MarchArgs<T> *geo, int nslot, hipStream_t s, hipEvent_t e0,
+                           hipEvent_t e1) {
   dim3 blk(kMarchLanes, BY), grd(8 * a.nper, nslot, max(a.nchunk, 1));
   switch (a.pf) {
-    case 1: return launch_march_pf<T, BY, 1>(a, grd, blk, s, e0, e1);
-    case 2: return launch_march_pf<T, BY, 2>(a, grd, blk, s, e0, e1);
-    case 3: return launch_march_pf<T, BY, 3>(a, grd, blk, s, e0, e1);
-    case 4: return launch_march_pf<T, BY, 4>(a, grd, blk, s, e0, e1);
+    case 1: return launch_march_pf<T, BY, 1>(a, geo, grd, blk, s, e0, e1);
+    case 2: return launch_march_pf<T, BY, 2>(a, geo, grd, blk, s, e0, e1);
+    case 3: return launch_march_pf<T, BY, 3>(a, geo, grd, blk, s, e0, e1);
+    case 4: return launch_march_pf<T, BY, 4>(a, geo, grd, blk, s, e0, e1);
   }
   return hipErrorInvalidValue;
 }
 template <typename T>
-hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                        const MarchArgs<T> *geo) {
   switch (a.rows) {
-    case 4: return launch_march_by<T, 4>(a, nslot, s, e0, e1);
-    case 8: return launch_march_by<T, 8>(a, nslot, s, e0, e1);
-    case 16: return launch_march_by<T, 16>(a, nslot, s, e0, e1);
+    case 4: return launch_march_by<T, 4>(a, geo, nslot, s, e0, e1);
+    case 8: return launch_march_by<T, 8>(a, geo, nslot, s, e0, e1);
+    case 16: return launch_march_by<T, 16>(a, geo, nslot, s, e0, e1);
   }
   return hipErrorInvalidValue;
 }
@@ -725,9 +762,13 @@ hipError_t launch_zsum(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEvent
   return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_surface(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+hipError_t launch_surface(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                          const MarchArgs<T> *geo) {
   dim3 blk(kMarchLanes, kSurfaceRows), grd(8 * a.sper, nslot);
-  hipExtLaunchKernelGGL((k_surface<T, kSurfaceRows>), grd, blk, 0u, s, e0, e1, 0u, a);
+  if (geo)
+    hipExtLaunchKernelGGL((k_surface<T, kSurfaceRows, true>), grd, blk, 0u, s, e0, e1, 0u, a, geo);
+  else
+    hipExtLaunchKernelGGL((k_surface<T, kSurfaceRows, false>), grd, blk, 0u, s, e0, e1, 0u, a, geo);
   return hipGetLastError();
 }
 template <typename T>
@@ -762,10 +803,14 @@ hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s) {
   return hipGetLastError();
 }
 
-template hipError_t launch_march<float>(const MarchArgs<float> &, int, hipStream_t, hipEvent_t, hipEvent_t);
-template hipError_t launch_march<double>(const MarchArgs<double> &, int, hipStream_t, hipEvent_t, hipEvent_t);
-template hipError_t launch_surface<float>(const MarchArgs<float> &, int, hipStream_t, hipEvent_t, hipEvent_t);
-template hipError_t launch_surface<double>(const MarchArgs<double> &, int, hipStream_t, hipEvent_t, hipEvent_t);
+template hipError_t launch_march<float>(const MarchArgs<float> &, int, hipStream_t, hipEvent_t, hipEvent_t,
+                                        const MarchArgs<float> *);
+template hipError_t launch_march<double>(const MarchArgs<double> &, int, hipStream_t, hipEvent_t, hipEvent_t,
+                                         const MarchArgs<double> *);
+template hipError_t launch_surface<float>(const MarchArgs<float> &, int, hipStream_t, hipEvent_t, hipEvent_t,
+                                          const MarchArgs<float> *);
+template hipError_t launch_surface<double>(const MarchArgs<double> &, int, hipStream_t, hipEvent_t, hipEvent_t,
+                                           const MarchArgs<double> *);
 template hipError_t launch_src_fix<float>(const MarchArgs<float> &, const SrcCell *, const int *, const SrcCell *,
                                           const int *, int, hipStream_t);
 template hipError_t launch_src_fix<double>(const MarchArgs<double> &, const SrcCell *, const int *, const SrcCell *,

@@ -416,6 +416,13 @@ __global__ void __launch_bounds__(256) k_up_ycols2(const UpcontArgs<T> u) {
 }
 
 // ---- tiled chain (computes in the field precision T) -------------------------------------
+// The argument record of a workgroup: the launch's own, or (model batch, GEO) the record of the
+// model in grid row blockIdx.y, copied by value so that its fields stay scalar loads.
+template <bool GEO, typename U>
+__device__ inline U slot_record(const U &u0, const U *geo) {
+  if constexpr (GEO) return geo[blockIdx.y];
+  else return u0;
+}
 // LDS of a tiled kernel: `lines` line regions of two buffers of m + 2 complex entries each (the
 // two extra entries hold the Nyquist bin of an unpacked half spectrum and shift the lines across
 // LDS banks), then the m-entry twiddle table.
@@ -453,13 +460,15 @@ __device__ inline T resample_x(const UpcontArgs<T> &u, const T *srow, int c) {
 
 // blockDim (W, P), W = 64, 128 or 256 threads per line: line p transforms irregular rows
 // j0 = 2 (blockIdx.x P + p) and j0 + 1.
-template <typename T>
-__global__ void __launch_bounds__(256) k_up_rows_fwd(const UpcontArgs<T> u) {
+template <typename T, bool GEO>
+__global__ void __launch_bounds__(256) k_up_rows_fwd(const UpcontArgs<T> u0, const UpcontArgs<T> *geo) {
   using V = typename CxOf<T>::V;
   extern __shared__ char lds_raw[];
   V *lds = (V *)lds_raw;
-  const int lane = threadIdx.x, W = blockDim.x, p = threadIdx.y, np = blockDim.y, slot = blockIdx.y;
+  const UpcontArgs<T> u = slot_record<GEO>(u0, geo);
+  const int lane = threadIdx.x, W = blockDim.x, p = threadIdx.y, np = blockDim.y, slot = GEO ? 0 : blockIdx.y;
   const int m = u.px.m, span = tile_span(m), L1 = u.L1, L = u.L, M = u.M;
+  if (GEO && (int)blockIdx.x * 2 * np >= M) return;  // the slot's rows end before this workgroup
   V *a = lds + (long long)p * 2 * span, *b = a + span;
   const V *tw = stage_twiddles(lds + (long long)np * 2 * span, u.px);
   const int jb = blockIdx.x * 2 * np, j0 = jb + 2 * p;
@@ -493,13 +502,15 @@ __global__ void __launch_bounds__(256) k_up_rows_fwd(const UpcontArgs<T> u) {
 
 // blockDim (W, Q), W = 64, 128 or 256 threads per line: line q owns half-spectrum column
 // kx = blockIdx.x Q + q.  E >= M1 / W: column entries kept in registers per thread.
-template <typename T, int E>
-__global__ void __launch_bounds__(256) k_up_cols(const UpcontArgs<T> u) {
+template <typename T, int E, bool GEO>
+__global__ void __launch_bounds__(256) k_up_cols(const UpcontArgs<T> u0, const UpcontArgs<T> *geo) {
   using V = typename CxOf<T>::V;
   extern __shared__ char lds_raw[];
   V *lds = (V *)lds_raw;
-  const int lane = threadIdx.x, W = blockDim.x, q = threadIdx.y, nq = blockDim.y, slot = blockIdx.y;
+  const UpcontArgs<T> u = slot_record<GEO>(u0, geo);
+  const int lane = threadIdx.x, W = blockDim.x, q = threadIdx.y, nq = blockDim.y, slot = GEO ? 0 : blockIdx.y;
   const int m = u.py.m, span = tile_span(m), M1 = u.M1, M = u.M;
+  if (GEO && (int)blockIdx.x * nq >= u.H) return;
   V *a = lds + (long long)q * 2 * span, *b = a + span;
   const V *tw = stage_twiddles(lds + (long long)nq * 2 * span, u.py);
   const int kx = blockIdx.x * nq + q;
@@ -567,13 +578,15 @@ __global__ void __launch_bounds__(256) k_up_cols(const UpcontArgs<T> u) {
 
 // blockDim (W, Q), W = 64, 128 or 256 threads per line: line q inverts regular row
 // y = blockIdx.x Q + q along x.
-template <typename T>
-__global__ void __launch_bounds__(256) k_up_rows_inv(const UpcontArgs<T> u) {
+template <typename T, bool GEO>
+__global__ void __launch_bounds__(256) k_up_rows_inv(const UpcontArgs<T> u0, const UpcontArgs<T> *geo) {
   using V = typename CxOf<T>::V;
   extern __shared__ char lds_raw[];
   V *lds = (V *)lds_raw;
-  const int lane = threadIdx.x, W = blockDim.x, q = threadIdx.y, nq = blockDim.y, slot = blockIdx.y;
+  const UpcontArgs<T> u = slot_record<GEO>(u0, geo);
+  const int lane = threadIdx.x, W = blockDim.x, q = threadIdx.y, nq = blockDim.y, slot = GEO ? 0 : blockIdx.y;
   const int m = u.px.m, span = tile_span(m), L1 = u.L1, H = u.H, M1 = u.M1;
+  if (GEO && (int)blockIdx.x * nq >= M1) return;
   const V *tw = stage_twiddles(lds + (long long)nq * 2 * span, u.px);
   const int yb = blockIdx.x * nq;
   const V *G = (const V *)u.G + (long long)slot * u.sG;
@@ -612,9 +625,11 @@ __global__ void __launch_bounds__(256) k_up_rows_inv(const UpcontArgs<T> u) {
 // Bilinear R -> IR (UpContinuation_gpu.cu:42-110) from the regular results fx, fy (rows of
 // pitch XR, slot stride sX; FP64 from the line chain, the field type from the tiled chain),
 // arithmetic in FP64.
-template <typename T, typename B>
-__global__ void k_up_bilin(const UpcontArgs<T> u, const B *fx, const B *fy) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x, slot = blockIdx.y;
+template <typename T, typename B, bool GEO>
+__global__ void k_up_bilin(const UpcontArgs<T> u0, const B *fx0, const B *fy0, const UpcontArgs<T> *geo) {
+  const UpcontArgs<T> u = slot_record<GEO>(u0, geo);
+  const B *fx = GEO ? (const B *)u.tbx : fx0, *fy = GEO ? (const B *)u.tby : fy0;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x, slot = GEO ? 0 : blockIdx.y;
   const int L0 = u.L, M0 = u.M, L1 = u.L1, M1 = u.M1, XR = u.XR;
   const double norm = (double)u.L1 * (double)M1;
   const long long wo = (long long)slot * u.sX, po = (long long)slot * u.pstride;
@@ -690,39 +705,42 @@ Tile tile_shape(int m, long long nlines) {
   return {256, 1, tile_lds<T>(m, 1)};
 }
 
-template <typename T, int E>
-hipError_t launch_cols_e(const UpcontArgs<T> &u, const Tile &t, int nslot, hipStream_t s) {
-  hipError_t e = allow_lds(k_up_cols<T, E>, t.lds);
+// geo (model batch): u carries the largest FFT lengths, M, M1 and H over the slots' records, so
+// the grid, the tile shape and the LDS cover every slot; workgroups beyond a slot's own lines
+// return at once.
+template <typename T, int E, bool GEO>
+hipError_t launch_cols_e(const UpcontArgs<T> &u, const Tile &t, int nslot, hipStream_t s, const UpcontArgs<T> *geo) {
+  hipError_t e = allow_lds(k_up_cols<T, E, GEO>, t.lds);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_up_cols<T, E>), dim3((u.H + t.q - 1) / t.q, nslot), dim3(t.w, t.q), t.lds, s, u);
+  hipLaunchKernelGGL((k_up_cols<T, E, GEO>), dim3((u.H + t.q - 1) / t.q, nslot), dim3(t.w, t.q), t.lds, s, u, geo);
   return hipGetLastError();
 }
 
-template <typename T>
-hipError_t launch_cols(const UpcontArgs<T> &u, int nslot, hipStream_t s) {
+template <typename T, bool GEO>
+hipError_t launch_cols(const UpcontArgs<T> &u, int nslot, hipStream_t s, const UpcontArgs<T> *geo) {
   const Tile t = tile_shape<T>(u.py.m, (long long)u.H * nslot);
   const int need = (u.M1 + t.w - 1) / t.w;  // column entries per thread
-  if (need <= 1) return launch_cols_e<T, 1>(u, t, nslot, s);
-  if (need <= 2) return launch_cols_e<T, 2>(u, t, nslot, s);
-  if (need <= 4) return launch_cols_e<T, 4>(u, t, nslot, s);
-  if (need <= 8) return launch_cols_e<T, 8>(u, t, nslot, s);
-  if (need <= 16) return launch_cols_e<T, 16>(u, t, nslot, s);
-  return launch_cols_e<T, 32>(u, t, nslot, s);
+  if (need <= 1) return launch_cols_e<T, 1, GEO>(u, t, nslot, s, geo);
+  if (need <= 2) return launch_cols_e<T, 2, GEO>(u, t, nslot, s, geo);
+  if (need <= 4) return launch_cols_e<T, 4, GEO>(u, t, nslot, s, geo);
+  if (need <= 8) return launch_cols_e<T, 8, GEO>(u, t, nslot, s, geo);
+  if (need <= 16) return launch_cols_e<T, 16, GEO>(u, t, nslot, s, geo);
+  return launch_cols_e<T, 32, GEO>(u, t, nslot, s, geo);
 }
 
-template <typename T>
-hipError_t launch_tiled(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEvent_t e0) {
+template <typename T, bool GEO>
+hipError_t launch_tiled(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEvent_t e0, const UpcontArgs<T> *geo) {
   const int npair = (u.M + 1) / 2;
   Tile t = tile_shape<T>(u.px.m, (long long)npair * nslot);
-  hipError_t e = allow_lds(k_up_rows_fwd<T>, t.lds);
+  hipError_t e = allow_lds(k_up_rows_fwd<T, GEO>, t.lds);
   if (e != hipSuccess) return e;
-  hipExtLaunchKernelGGL((k_up_rows_fwd<T>), dim3((npair + t.q - 1) / t.q, nslot), dim3(t.w, t.q),
-                        (std::uint32_t)t.lds, s, e0, nullptr, 0u, u);
+  hipExtLaunchKernelGGL((k_up_rows_fwd<T, GEO>), dim3((npair + t.q - 1) / t.q, nslot), dim3(t.w, t.q),
+                        (std::uint32_t)t.lds, s, e0, nullptr, 0u, u, geo);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  if ((e = launch_cols<T>(u, nslot, s)) != hipSuccess) return e;
+  if ((e = launch_cols<T, GEO>(u, nslot, s, geo)) != hipSuccess) return e;
   t = tile_shape<T>(u.px.m, (long long)u.M1 * nslot);
-  if ((e = allow_lds(k_up_rows_inv<T>, t.lds)) != hipSuccess) return e;
-  hipLaunchKernelGGL((k_up_rows_inv<T>), dim3((u.M1 + t.q - 1) / t.q, nslot), dim3(t.w, t.q), t.lds, s, u);
+  if ((e = allow_lds(k_up_rows_inv<T, GEO>, t.lds)) != hipSuccess) return e;
+  hipLaunchKernelGGL((k_up_rows_inv<T, GEO>), dim3((u.M1 + t.q - 1) / t.q, nslot), dim3(t.w, t.q), t.lds, s, u, geo);
   return hipGetLastError();
 }
 
@@ -753,20 +771,28 @@ hipError_t launch_lined(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEve
 }  // namespace
 
 template <typename T>
-hipError_t launch_upcont(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-  hipError_t e = u.tiled ? launch_tiled<T>(u, nslot, s, e0) : launch_lined<T>(u, nslot, s, e0);
+hipError_t launch_upcont(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                         const UpcontArgs<T> *geo) {
+  if (geo && !u.tiled) return hipErrorInvalidValue;  // model batches run the tiled chain only
+  hipError_t e = geo ? launch_tiled<T, true>(u, nslot, s, e0, geo)
+                     : (u.tiled ? launch_tiled<T, false>(u, nslot, s, e0, nullptr) : launch_lined<T>(u, nslot, s, e0));
   if (e != hipSuccess) return e;
   const int n = u.M * (u.L + 1) + (u.M + 1) * u.L;
-  if (u.tiled)
-    hipExtLaunchKernelGGL((k_up_bilin<T, T>), dim3((n + 255) / 256, nslot), dim3(256), 0u, s, nullptr, e1, 0u, u,
-                          (const T *)u.tbx, (const T *)u.tby);
+  if (geo)
+    hipExtLaunchKernelGGL((k_up_bilin<T, T, true>), dim3((n + 255) / 256, nslot), dim3(256), 0u, s, nullptr, e1, 0u,
+                          u, (const T *)nullptr, (const T *)nullptr, geo);
+  else if (u.tiled)
+    hipExtLaunchKernelGGL((k_up_bilin<T, T, false>), dim3((n + 255) / 256, nslot), dim3(256), 0u, s, nullptr, e1,
+                          0u, u, (const T *)u.tbx, (const T *)u.tby, (const UpcontArgs<T> *)nullptr);
   else
-    hipExtLaunchKernelGGL((k_up_bilin<T, double>), dim3((n + 255) / 256, nslot), dim3(256), 0u, s, nullptr, e1, 0u,
-                          u, (const double *)u.bxr, (const double *)u.byr);
+    hipExtLaunchKernelGGL((k_up_bilin<T, double, false>), dim3((n + 255) / 256, nslot), dim3(256), 0u, s, nullptr,
+                          e1, 0u, u, (const double *)u.bxr, (const double *)u.byr, (const UpcontArgs<T> *)nullptr);
   return hipGetLastError();
 }
 
-template hipError_t launch_upcont<float>(const UpcontArgs<float> &, int, hipStream_t, hipEvent_t, hipEvent_t);
-template hipError_t launch_upcont<double>(const UpcontArgs<double> &, int, hipStream_t, hipEvent_t, hipEvent_t);
+template hipError_t launch_upcont<float>(const UpcontArgs<float> &, int, hipStream_t, hipEvent_t, hipEvent_t,
+                                         const UpcontArgs<float> *);
+template hipError_t launch_upcont<double>(const UpcontArgs<double> &, int, hipStream_t, hipEvent_t, hipEvent_t,
+                                          const UpcontArgs<double> *);
 
 }  // namespace tcsem

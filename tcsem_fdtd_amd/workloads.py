@@ -25,6 +25,12 @@ the deck.
               150-450 m), two 100 ohm-m reservoirs (4 x 3 km x 80 m and 3 x 2 km x 60 m); three
               x-dipoles 50 m above the local seafloor, 401 receivers 10 m above it from -10 to
               10 km inline.
+  uneven_models  64 seeded models whose grids are drawn uniformly between 96 x 96 x 96 and
+              224 x 224 x 160 cells (even L, M; 20 padding cells per side as marine_3d, 40 z cells
+              growing by 1.05 under a 20 m core), 1 ohm-m sediment under 0.5-2 km of seawater
+              with 5-10 axis-aligned ellipsoids of log-uniform 0.3-300 ohm-m; one x-dipole 50 m
+              above the seafloor and 41 seafloor receivers from -1 to 1 km inline each.  All
+              share dxmin, dymin, dzmin and the source, hence one time-step schedule.
 """
 import os
 from dataclasses import dataclass, field
@@ -185,4 +191,34 @@ def large_5(setup_mod, seed=20261203, nt=100, tn=50.0):
     return Deck("large_5 512x512x256", setup, cond,
                 dict(T0=0.01, Tn=tn, Nt=nt, is_log=1, comps=(1, 1, 1, 1, 1, 1)), rx, txs, seed=seed,
                 steps=50000, grid=g)
+
+
+def uneven_models(setup_mod, n=64, seed=20261204, nt=40, tn=5.0):
+    rng = np.random.default_rng(seed)
+    out = []
+    for q in range(n):
+        L = 2 * int(rng.integers(48, 113))
+        M = 2 * int(rng.integers(48, 113))
+        N = int(rng.integers(96, 161))
+        setup = _setup(L - 40, (6, 14), N - 40, 40, 0, arg1=1.05, arg2=1.747)
+        setup["Ny"] = M - 40
+        g = _grid(setup_mod, setup)
+        depth = float(rng.uniform(500.0, 2000.0))
+        Z = g["Z"][:, None, None]
+        X, Y = g["X"][None, None, :], g["Y"][None, :, None]
+        log_rho = np.where(Z < depth, np.log10(1.0 / SEA), 0.0) * np.ones((1, len(g["Y"]), len(g["X"])))
+        for _ in range(int(rng.integers(5, 11))):
+            cx, cy = rng.uniform(-1500.0, 1500.0, 2)
+            cz = depth + rng.uniform(150.0, 1200.0)
+            ax, ay = rng.uniform(200.0, 1500.0, 2)
+            az = rng.uniform(30.0, 300.0)
+            inside = ((X - cx) / ax) ** 2 + ((Y - cy) / ay) ** 2 + ((Z - cz) / az) ** 2 <= 1.0
+            log_rho = np.where(inside, rng.uniform(np.log10(0.3), np.log10(300.0)), log_rho)
+        cond = (10.0 ** -log_rho).astype(np.float64)
+        rx = np.array([[x, 0.0, depth - 1.0] for x in np.linspace(-1000.0, 1000.0, 41)])
+        txs = [dict(ort=1, bipolen=50.0, x=0.0, y=0.0, z=depth - 50.0, rx_ids=list(range(1, len(rx) + 1)))]
+        out.append(Deck("uneven %d: %dx%dx%d" % (q, L, M, N), setup, cond,
+                        dict(T0=0.01, Tn=tn, Nt=nt, is_log=1, comps=(1, 1, 1, 1, 1, 1)), rx, txs,
+                        seed=seed, grid=g))
+    return out
 
