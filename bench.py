@@ -9,7 +9,8 @@ carries the plain-FP64 and the FP32 transmitter-batched (32 x-dipoles) throughpu
 512 x 512 x 256 model in FP32 and FP64 over a window of steps, the roofline of the volume march
 (HIP events on the library's stream), the LDS and HBM rates of the air-plane chain and of the
 receiver gathers, and the FP64 CPU oracle timed on this host as the baseline.  Every GPU run first
-lets the library's autotuner pick the march tiling for its batch size.
+lets the library's autotuner pick the march tiling for its batch size.  The line also carries
+64 conductivity models of uneven grids advanced as one model batch (FP32 and FP64).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu] [--no-tune]
 """
@@ -62,6 +63,66 @@ def gpu_run(T, deck, precision, nbatch, steps, warmup, tune=True):
         "timing": tm, "L": c.L, "M": c.M, "N": c.N, "ex_norm": float(np.linalg.norm(ex)),
         "ex": ex, "tiling": tiling,
     }
+
+
+def models_run(T, decks, precision, steps, warmup, tune=True, single_steps=0):
+    """A model batch: every deck its own context (one transmitter), all advanced together by
+    tcsem_batch_step.  single_steps > 0 first times the same contexts stepped one by one
+    (tcsem_step per context, each on its own streams, heuristic tiling) for comparison."""
+    import numpy as np
+    ctxs = []
+    for d in decks:
+        c = T.Context(T.make_setup(**d.setup), d.cond, precision)
+        r = d.rxtime
+        c.set_receivers(T.make_rxtime(r["T0"], r["Tn"], r["Nt"], r["is_log"], r["comps"]), d.rx)
+        c.set_transmitters([T.make_tx(**dict(t, txid=i + 1)) for i, t in enumerate(d.txs)])
+        ctxs.append(c)
+    cells = sum(c.L * c.M * c.N for c in ctxs)
+    out = {"models": len(ctxs), "cells": cells}
+    try:
+        if single_steps > 0:
+            for c in ctxs:
+                c.begin(0, 1)
+                c.step(warmup)
+            for c in ctxs:
+                c.sync()
+            t0 = time.perf_counter()
+            for c in ctxs:
+                c.step(single_steps)
+            for c in ctxs:
+                c.sync()
+            sec = time.perf_counter() - t0
+            out["unbatched"] = {"gcell_s": cells * single_steps / sec / 1e9,
+                                "ms_per_step": 1000.0 * sec / single_steps, "steps": single_steps}
+        with T.Batch(ctxs) as b:
+            out["tiling"] = b.autotune(5) if tune else None
+            b.step(warmup)
+            b.sync()
+            b.enable_timing(True)
+            t0 = time.perf_counter()
+            b.step(steps)
+            b.sync()
+            sec = time.perf_counter() - t0
+            out["timing"] = b.timing()
+            b.enable_timing(False)
+            out["ex"] = np.concatenate([c.get(T.EX, 0).ravel() for c in ctxs])
+        out["gcell_s"] = cells * steps / sec / 1e9
+        out["ms_per_step"] = 1000.0 * sec / steps
+        # march bytes: levels 1..N-1 of every member, 13 words per cell
+        out["march_cells"] = sum(c.L * c.M * (c.N - 1) for c in ctxs)
+    finally:
+        for c in ctxs:
+            c.close()
+    return out
+
+
+def models_roofline(res, esz):
+    algo = res["march_cells"] * WORDS_PER_CELL * esz
+    ms = res["timing"]["march_ms"]
+    gbs = algo / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None, "kernel": "k_march (model batch)",
+            "algorithmic_bytes_per_launch": int(algo), "avg_launch_ms": round(ms, 5)}
 
 
 def roofline(res, esz, nbatch, traffic):
@@ -239,6 +300,14 @@ def main():
     rl64 = gpu_run(T, big, T.FP64, 1, nl, wl, tune)
     big_err = float(np.linalg.norm(rl32["ex"] - rl64["ex"]) / max(np.linalg.norm(rl64["ex"]), 1e-300))
     rl32.pop("ex"), rl64.pop("ex")
+    # 64 conductivity models of uneven grids (96^3 .. 224x224x160), one x-dipole each, in one
+    # model batch: FP32 and FP64, FP32 also stepped model by model for comparison
+    models = W.uneven_models(T)
+    nm, wm = max(args.steps // 5, 1), args.warmup
+    m32 = models_run(T, models, T.FP32, nm, wm, tune, single_steps=max(nm // 2, 1))
+    m64 = models_run(T, models, T.FP64, nm, wm, tune)
+    models_err = float(np.linalg.norm(m32["ex"] - m64["ex"]) / max(np.linalg.norm(m64["ex"]), 1e-300))
+    m32.pop("ex"), m64.pop("ex")
     line = {
         "metric": METRIC,
         "value": round(r32["gcell_s"], 3),
@@ -277,6 +346,17 @@ def main():
                     "fp64": {"value": round(rl64["gcell_s"], 3), "ms_per_step": round(rl64["ms_per_step"], 5),
                              "roofline": roofline(rl64, 8, 1, None), "march_tiling": rl64["tiling"]},
                     "fp32_vs_fp64_ex_rel_l2_after_window": big_err},
+        "uneven_models": {"workload": "64 models, grids 96^3 .. 224x224x160 (%d cells), 1 x-dipole and 41 receivers each"
+                                      % m32["cells"], "steps": nm, "warmup": wm,
+                          "fp32": {"value": round(m32["gcell_s"], 3), "ms_per_step": round(m32["ms_per_step"], 5),
+                                   "roofline": models_roofline(m32, 4), "march_tiling": m32["tiling"],
+                                   "kernel_ms": {k: round(v, 5) for k, v in m32["timing"].items() if k.endswith("_ms")},
+                                   "unbatched": {k: round(v, 5) if isinstance(v, float) else v
+                                                 for k, v in m32["unbatched"].items()}},
+                          "fp64": {"value": round(m64["gcell_s"], 3), "ms_per_step": round(m64["ms_per_step"], 5),
+                                   "roofline": models_roofline(m64, 8), "march_tiling": m64["tiling"],
+                                   "kernel_ms": {k: round(v, 5) for k, v in m64["timing"].items() if k.endswith("_ms")}},
+                          "fp32_vs_fp64_ex_rel_l2_after_window": models_err},
         "ratio_fp32_batched_vs_fp64": round(rb["gcell_s"] / r64["gcell_s"], 3),
         "fp32_vs_fp64_ex_rel_l2_after_run": field_err,
     }

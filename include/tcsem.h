@@ -213,6 +213,10 @@ int tcsem_batch_autotune(tcsem_batch *batch, int nsteps, int *rows, int *pf, dou
 int tcsem_batch_step(tcsem_batch *batch, int nsteps);
 int tcsem_batch_run(tcsem_batch *batch, int max_steps, int *steps_done);
 int tcsem_batch_sync(tcsem_batch *batch);
+/* As tcsem_timing for the batch's launches: t[0] volume march per step (lower and top launch
+ * over all members), t[1] surface level, t[2] air-plane chain; n[i] counts. */
+int tcsem_batch_enable_timing(tcsem_batch *batch, int on);
+int tcsem_batch_timing(tcsem_batch *batch, double t[3], long n[3]);
 void tcsem_batch_destroy(tcsem_batch *batch);
 
 #ifdef __cplusplus

@@ -117,6 +117,8 @@ def lib():
         "tcsem_batch_run": (C.c_int, [P, C.c_int, ip]),
         "tcsem_batch_sync": (C.c_int, [P]),
         "tcsem_batch_destroy": (None, [P]),
+        "tcsem_batch_enable_timing": (C.c_int, [P, C.c_int]),
+        "tcsem_batch_timing": (C.c_int, [P, dp, C.POINTER(C.c_long)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -411,3 +413,11 @@ class Batch:
 
     def sync(self):
         _check(self.L_.tcsem_batch_sync(self.h))
+
+    def enable_timing(self, on=True):
+        _check(self.L_.tcsem_batch_enable_timing(self.h, int(on)))
+
+    def timing(self):
+        t, n = (C.c_double * 3)(), (C.c_long * 3)()
+        _check(self.L_.tcsem_batch_timing(self.h, t, n))
+        return {"march_ms": t[0], "surface_ms": t[1], "upcont_ms": t[2], "n": list(n)}

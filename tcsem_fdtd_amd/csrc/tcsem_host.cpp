@@ -1334,6 +1334,14 @@ struct tcsem_batch {
   int scal_cap = 0;
   std::vector<char> active;  // per member: still sampling
   int steps = 0;
+  long nenq = 0;             // batch steps enqueued (live or not)
+  // timing, as for a context: kernel-recorded events every kTimingStride-th step, folded on read
+  bool timing = false;
+  std::vector<hipEvent_t> evpool;
+  size_t evused = 0;
+  std::vector<tcsem_ctx::Span> spans;  // kinds 0 march, 1 surface, 2 air-plane chain
+  double tacc[3] = {0, 0, 0};
+  long tn[3] = {0, 0, 0};
 };
 
 namespace {
@@ -1425,6 +1433,8 @@ int batch_scal_reserve(tcsem_batch *b, int nsteps) {
   return TCSEM_OK;
 }
 
+int batch_collect(tcsem_batch *b);
+
 // Waits for the members' own streams and the batch's, so records and tables can be rewritten.
 int batch_quiesce(tcsem_batch *b) {
   for (tcsem_ctx *c : b->m) {
@@ -1433,7 +1443,7 @@ int batch_quiesce(tcsem_batch *b) {
   }
   HIPCHK(hipStreamSynchronize(b->stream2));
   HIPCHK(hipStreamSynchronize(b->stream));
-  return TCSEM_OK;
+  return batch_collect(b);
 }
 
 // One batch step on the batch's streams, as do_step for one context: lower march of every member,
@@ -1453,9 +1463,21 @@ int batch_enqueue(tcsem_batch *b, const MarchArgs<T> &la, const UpcontArgs<T> &u
   hi.k_hi = 1;
   hi.k_lo = 1;
   hi.carry = 1;
-  HIPCHK(launch_march<T>(lo, nm, st, nullptr, nullptr, geo));
+  const bool timed = b->timing && b->nenq % kTimingStride == 0;
+  hipEvent_t te[8] = {};
+  if (timed)
+    for (auto &e : te) {
+      if (b->evused == b->evpool.size()) {
+        hipEvent_t x = nullptr;
+        if (hipEventCreate(&x) != hipSuccess) return fail(TCSEM_EDEVICE, "timing event creation failed");
+        b->evpool.push_back(x);
+      }
+      e = b->evpool[b->evused++];
+    }
+  ++b->nenq;
+  HIPCHK(launch_march<T>(lo, nm, st, te[0], te[1], geo));
   HIPCHK(hipStreamWaitEvent(st, b->xev[1], 0));
-  HIPCHK(launch_march<T>(hi, nm, st, nullptr, nullptr, geo));
+  HIPCHK(launch_march<T>(hi, nm, st, te[2], te[3], geo));
   if (live)
     for (int q = 0; q < nm; ++q) {
       tcsem_ctx *c = b->m[q];
@@ -1466,8 +1488,14 @@ int batch_enqueue(tcsem_batch *b, const MarchArgs<T> &la, const UpcontArgs<T> &u
     }
   HIPCHK(hipEventRecord(b->xev[0], st));
   HIPCHK(hipStreamWaitEvent(st2, b->xev[0], 0));
-  HIPCHK(launch_upcont<T>(ul, nm, st2, nullptr, nullptr, ugeo));
-  HIPCHK(launch_surface<T>(hi, nm, st2, nullptr, nullptr, geo));
+  HIPCHK(launch_upcont<T>(ul, nm, st2, te[4], te[5], ugeo));
+  HIPCHK(launch_surface<T>(hi, nm, st2, te[6], te[7], geo));
+  if (timed) {
+    b->spans.push_back({0, 0, te[0], te[1]});
+    b->spans.push_back({0, 1, te[2], te[3]});
+    b->spans.push_back({2, 1, te[4], te[5]});
+    b->spans.push_back({1, 1, te[6], te[7]});
+  }
   if (live)
     for (int q = 0; q < nm; ++q) {
       tcsem_ctx *c = b->m[q];
@@ -1489,6 +1517,22 @@ int batch_join(tcsem_batch *b) {
     HIPCHK(hipStreamWaitEvent(c->stream, b->done, 0));
     HIPCHK(hipStreamWaitEvent(c->stream2, b->done, 0));
   }
+  return TCSEM_OK;
+}
+
+// Waits for the recorded intervals of the batch and adds them to the per-kind totals.
+int batch_collect(tcsem_batch *b) {
+  if (b->spans.empty()) return TCSEM_OK;
+  HIPCHK(hipStreamSynchronize(b->stream2));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  for (const auto &sp : b->spans) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, sp.a, sp.b));
+    b->tacc[sp.kind] += ms;
+    b->tn[sp.kind] += sp.count;
+  }
+  b->spans.clear();
+  b->evused = 0;
   return TCSEM_OK;
 }
 
@@ -1540,6 +1584,8 @@ int batch_autotune_t(tcsem_batch *b, int nsteps, int *best_by, int *best_pf, dou
   if (int rc = batch_quiesce(b)) return rc;
   for (int q = 0; q < nm; ++q) b->active[q] = 1;
   if (int rc = batch_scal_reserve(b, 1)) return rc;
+  const bool timing0 = b->timing;
+  b->timing = false;
   std::vector<double> scal((size_t)nm * 4);
   for (int q = 0; q < nm; ++q) {
     scal[4 * q + 0] = 1.0;
@@ -1589,6 +1635,7 @@ int batch_autotune_t(tcsem_batch *b, int nsteps, int *best_by, int *best_pf, dou
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  b->timing = timing0;
   if (rc) return rc;
   HIPCHK(hipStreamSynchronize(b->stream2));
   HIPCHK(hipStreamSynchronize(b->stream));
@@ -2610,6 +2657,8 @@ void tcsem_batch_destroy(tcsem_batch *b) {
   for (auto &e : b->xev)
     if (e) (void)hipEventDestroy(e);
   if (b->done) (void)hipEventDestroy(b->done);
+  for (auto &e : b->evpool)
+    if (e) (void)hipEventDestroy(e);
   if (b->stream2) (void)hipStreamDestroy(b->stream2);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -2659,6 +2708,27 @@ int tcsem_batch_run(tcsem_batch *b, int max_steps, int *steps_done) {
   }
   HIPCHK(hipStreamSynchronize(b->stream));
   if (steps_done) *steps_done = b->steps;
+  return TCSEM_OK;
+}
+
+int tcsem_batch_enable_timing(tcsem_batch *b, int on) {
+  if (!b) return fail(TCSEM_EINVAL, "null batch");
+  if (int rc = batch_collect(b)) return rc;
+  b->timing = on != 0;
+  for (int k = 0; k < 3; ++k) {
+    b->tacc[k] = 0;
+    b->tn[k] = 0;
+  }
+  return TCSEM_OK;
+}
+
+int tcsem_batch_timing(tcsem_batch *b, double t[3], long n[3]) {
+  if (!b || !t || !n) return fail(TCSEM_EINVAL, "null argument");
+  if (int rc = batch_collect(b)) return rc;
+  for (int k = 0; k < 3; ++k) {
+    t[k] = b->tn[k] ? b->tacc[k] / b->tn[k] : 0.0;
+    n[k] = b->tn[k];
+  }
   return TCSEM_OK;
 }
 
