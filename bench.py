@@ -37,7 +37,7 @@ def gpu_run(T, deck, precision, nbatch, steps, warmup, tune=True):
     with T.Context(st, deck.cond, precision) as c:
         # the library's autotuner times every march tiling for this batch size and keeps the
         # fastest (outside the timed region, on zeroed fields)
-        tiling = c.autotune(nbatch, 5) if tune else None
+        tiling = c.autotune(nbatch, 20) if tune else None
         r = deck.rxtime
         c.set_receivers(T.make_rxtime(r["T0"], r["Tn"], r["Nt"], r["is_log"], r["comps"]), deck.rx)
         txs = [T.make_tx(**dict(t, txid=i + 1)) for i, t in enumerate(deck.txs)]

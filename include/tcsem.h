@@ -156,10 +156,12 @@ int tcsem_reset_fields(tcsem_ctx *ctx, int first, int n);
  * fixed): times nsteps nbatch-transmitter steps as they run in production (the march beside the
  * upward continuation and surface level on the second stream; no source, no gathers) for every
  * tiling the library runs -- rows per workgroup 4/8/16 (FP64: 4/8), prefetch depth 1-4, z-chunks
- * of the lower launch 1/2/4/8 as the depth allows -- and keeps the fastest for later batches of
- * that size (TCSEM_MARCH_ROWS / _PF / _ZCHUNKS still override it).  It runs on the fields of
- * slots [0, nbatch): a begun run is discarded and the fields are left zeroed.  Outputs
- * (nullable): the chosen rows, prefetch depth, z-chunks and the time per step [ms]. */
+ * of the lower launch 1/2/3/4/6/8 as the depth allows, and for chunked tilings the source of the
+ * chunks' Bz partial sums (a pass over Bx, By, or the chunk sums the previous step's march
+ * accumulated) -- and keeps the fastest for later batches of that size (TCSEM_MARCH_ROWS / _PF /
+ * _ZCHUNKS / _ZSUM still override it).  It runs on the fields of slots [0, nbatch): a begun run
+ * is discarded and the fields are left zeroed.  Outputs (nullable): the chosen rows, prefetch
+ * depth, z-chunks and the time per step [ms]. */
 int tcsem_autotune(tcsem_ctx *ctx, int nbatch, int nsteps, int *rows, int *pf, int *zchunks, double *ms);
 
 /* One DuFort-Frankel step for field sets [0, nbatch), each driven by transmitter

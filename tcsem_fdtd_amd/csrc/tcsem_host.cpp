@@ -394,9 +394,10 @@ struct tcsem_ctx {
   int PX = 0, PY = 0;
   long long PS = 0, fstride = 0, pstride = 0, wstride = 0;
   int by = 8, pf = 2, ntx = 0, nty = 0, nper = 0;  // march tiling
-  int tune_nbatch = 0, tune_by = 0, tune_pf = 0, tune_nch = 0;  // autotuned tiling and its batch size
+  int tune_nbatch = 0, tune_by = 0, tune_pf = 0, tune_nch = 0, tune_fuse = 1;  // autotuned tiling, its batch size
   int stx = 0, sty = 0, sper = 0;                   // surface tiling
   int nchunk = 0, chunk_len = 0;                    // z-chunks of the lower march (0: one pass)
+  int fuse = 1;  // chunked march: start chunks from the chunk sums of B (1) or a k_zsum pass (0)
   int xm = 1;                                       // x margin of the layout in elements
   int nslots = 0;
   // stream: the volume march and every host-ordered operation; stream2: upward continuation,
@@ -409,6 +410,12 @@ struct tcsem_ctx {
   char *d_sig = nullptr;
   char *d_planes = nullptr;  // bz0, bxair, byair, carry sum, carry compensation: 5 x nslots x pstride
   char *d_zs = nullptr;      // z-chunk Bz partial sums: (kMaxChunks - 1) x nslots x pstride
+  // chunk sums of B written by the chunked lower march for the next step (FP64): 2 parities x
+  // (ib_cap chunks) x (x, y) x nslots x pstride; valid while the state is only advanced by steps
+  // of the same chunking (set, reset, begin, source passes and model-batch steps invalidate them)
+  double *d_ib = nullptr;
+  int ib_cap = 0, ib_par = 0, ib_nchunk = 0, ib_chunk_len = 0;
+  bool ib_valid = false;
   char *d_rd = nullptr;      // rdx, rdx2, rdy, rdy2, dzt, rdz, rdz2 (field type)
   double *d_grid = nullptr;  // X Y Z dx dy dz Xn Yn
   int *d_maps = nullptr;     // xBx2N yo2N xo2N yBy2N
@@ -526,11 +533,18 @@ void choose_tiling(tcsem_ctx *c, int nbatch) {
   int nch = 1;
   const long long wg = tiles(by) * nbatch;
   if (N >= 6 && wg < 2LL * cu) nch = (int)std::min<long long>((2LL * cu + wg - 1) / wg, std::max(1, nlev / 8));
+  int fuse = 1;
   if (c->tune_nbatch == nbatch) {
     by = c->tune_by;
     pf = c->tune_pf;
     nch = c->tune_nch;
+    fuse = c->tune_fuse;
   }
+  if (const char *e = std::getenv("TCSEM_MARCH_ZSUM")) {  // "pass": k_zsum pass, "fused": chunk sums
+    if (!std::strcmp(e, "pass")) fuse = 0;
+    if (!std::strcmp(e, "fused")) fuse = 1;
+  }
+  c->fuse = fuse;
   if (const char *e = std::getenv("TCSEM_MARCH_ROWS")) {
     const int v = std::atoi(e);
     if (v == 4 || v == 8 || v == 16) by = v;
@@ -737,6 +751,10 @@ int alloc_slots(tcsem_ctx *c, int nslots) {
   (void)hipFree(c->d_planes);
   (void)hipFree(c->d_work);
   (void)hipFree(c->d_zs);
+  (void)hipFree(c->d_ib);
+  c->d_ib = nullptr;
+  c->ib_cap = 0;
+  c->ib_valid = false;
   c->d_fields = c->d_planes = c->d_zs = nullptr;
   c->d_work = nullptr;
   c->nslots = 0;
@@ -1040,6 +1058,30 @@ int rc_collect(tcsem_ctx *c) {
 // when the tiling says so) and, after `wait` (the previous step's stream2 work; may be null), the
 // top launch (level 1 and Bz_old(0)); with fewer levels one launch.  te (null, or 4 events):
 // start/end of the lower and of the top launch.
+// Chunk-sum planes for nchunk chunks (allocated on first use, grown when the chunking grows; a
+// new allocation is zeroed, so the plane margins and wall columns read as zero).
+hipError_t ensure_ib(tcsem_ctx *c, int nchunk) {
+  if (nchunk - 1 <= c->ib_cap) return hipSuccess;
+  hipError_t e;
+  if ((e = hipStreamSynchronize(c->stream2)) != hipSuccess || (e = hipStreamSynchronize(c->stream)) != hipSuccess)
+    return e;
+  (void)hipFree(c->d_ib);
+  c->d_ib = nullptr;
+  c->ib_cap = 0;
+  c->ib_valid = false;
+  const size_t bytes = (size_t)2 * (nchunk - 1) * 2 * c->nslots * c->pstride * sizeof(double);
+  if ((e = hipMalloc((void **)&c->d_ib, bytes)) != hipSuccess) return e;
+  if ((e = launch_fill_zero(c->d_ib, bytes, c->stream)) != hipSuccess) return e;
+  c->ib_cap = nchunk - 1;
+  return hipSuccess;
+}
+
+// Origin of the chunk-sum planes of parity par (chunk 0, x sums, slot 0).
+double *ib_planes(tcsem_ctx *c, int par) {
+  const long long ibstride = (long long)c->nslots * c->pstride;
+  return c->d_ib + (long long)par * c->ib_cap * 2 * ibstride + c->PX + c->xm;
+}
+
 template <typename T>
 hipError_t enqueue_march(tcsem_ctx *c, const MarchArgs<T> &a, int nb, hipStream_t st, hipEvent_t wait,
                          const hipEvent_t *te) {
@@ -1052,12 +1094,28 @@ hipError_t enqueue_march(tcsem_ctx *c, const MarchArgs<T> &a, int nb, hipStream_
     lo.k_lo = 2;
     lo.carry = 2;
     const bool chunked = c->nchunk > 1;
+    bool fused = false;
     if (chunked) {
       lo.nchunk = c->nchunk;
       lo.chunk_len = c->chunk_len;
-      if ((e = launch_zsum<T>(lo, nb, st, te[0])) != hipSuccess) return e;
+      if (c->fuse && (e = ensure_ib(c, c->nchunk)) != hipSuccess) return e;
+      // this step writes the chunk sums of its result; it reads the previous step's when they
+      // describe the current state with this chunking, else the k_zsum pass supplies the partials
+      fused = c->fuse && c->ib_valid && c->ib_nchunk == c->nchunk && c->ib_chunk_len == c->chunk_len;
+      if (c->fuse) {
+        lo.ibstride = (long long)c->nslots * c->pstride;
+        lo.ibw = ib_planes(c, c->ib_par ^ 1);
+        lo.ibr = fused ? ib_planes(c, c->ib_par) : nullptr;
+      }
+      if (!fused && (e = launch_zsum<T>(lo, nb, st, te[0])) != hipSuccess) return e;
+      c->ib_par ^= 1;
+      c->ib_valid = c->fuse != 0;
+      c->ib_nchunk = c->nchunk;
+      c->ib_chunk_len = c->chunk_len;
+    } else {
+      c->ib_valid = false;
     }
-    if ((e = launch_march<T>(lo, nb, st, chunked ? nullptr : te[0], te[1])) != hipSuccess) return e;
+    if ((e = launch_march<T>(lo, nb, st, (chunked && !fused) ? nullptr : te[0], te[1])) != hipSuccess) return e;
   }
   if (wait && (e = hipStreamWaitEvent(st, wait, 0)) != hipSuccess) return e;
   MarchArgs<T> hi = a;
@@ -1190,8 +1248,10 @@ int do_step(tcsem_ctx *c) {
       if (!(e = timing_event(c))) return fail(TCSEM_EDEVICE, "timing event creation failed");
   }
   HIPCHK(enqueue_march<T>(c, a, nb, st, c->xev[1], te));
-  if (c->has_fix && a.amp != 0.0)
+  if (c->has_fix && a.amp != 0.0) {
     HIPCHK(launch_src_fix<T>(a, c->d_fixe, c->d_fixidx, c->d_fixb, c->d_fixidx + 2 * nb, nb, st));
+    c->ib_valid = false;  // the pass rewrites B cells after the march summed them
+  }
   HIPCHK(hipEventRecord(c->xev[0], st));
   HIPCHK(hipStreamWaitEvent(st2, c->xev[0], 0));
   HIPCHK(launch_upcont<T>(u, nb, st2, te[4], te[5]));
@@ -1240,12 +1300,13 @@ int check_ctx(const tcsem_ctx *c) {
 // (its time per step).  Timing the march alone would favour tilings that lose more bandwidth
 // to the concurrent air-plane chain.
 template <typename T>
-int autotune_march(tcsem_ctx *c, int nbatch, int nsteps, int *best_by, int *best_pf, int *best_nch, double *best_ms) {
+int autotune_march(tcsem_ctx *c, int nbatch, int nsteps, int *best_by, int *best_pf, int *best_nch, int *best_fuse,
+                   double *best_ms) {
   const int N = c->g.info.N, nlev = N - 2;
   std::vector<int> rows_c = c->prec == 64 ? std::vector<int>{4, 8} : std::vector<int>{4, 8, 16};
   std::vector<int> nch_c = {1};
   if (N >= 6)
-    for (int z : {2, 4, 8})
+    for (int z : {2, 3, 4, 6, 8})
       if (z <= std::max(1, nlev / 8)) nch_c.push_back(z);
   hipEvent_t e0 = nullptr, e1 = nullptr;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
@@ -1272,45 +1333,53 @@ int autotune_march(tcsem_ctx *c, int nbatch, int nsteps, int *best_by, int *best
   };
   struct Trial {
     double ms;
-    int by, pf, nch;
+    int by, pf, nch, fuse;
   };
   std::vector<Trial> trials;
   int rc = TCSEM_OK;
+  const int fuse0 = c->fuse;
   for (int by : rows_c)
     for (int pf = 1; pf <= 4 && rc == TCSEM_OK; ++pf)
-      for (int nch : nch_c) {
-        apply_tiling(c, by, pf, nch);
-        if (c->nchunk == 0 && nch > 1) continue;  // clamped to one pass: already timed
-        hipError_t e = one_step();               // untimed: first launches of this tiling
-        if (e == hipSuccess) e = hipStreamWaitEvent(st, c->xev[1], 0);
-        if (e == hipSuccess) e = hipEventRecord(e0, st);
-        for (int q = 0; q < nsteps && e == hipSuccess; ++q) e = one_step();
-        if (e == hipSuccess) e = hipStreamWaitEvent(st, c->xev[1], 0);
-        if (e == hipSuccess) e = hipEventRecord(e1, st);
-        if (e == hipSuccess) e = hipEventSynchronize(e1);
-        float ms = 0.f;
-        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-        if (e != hipSuccess) {
-          rc = fail(TCSEM_EDEVICE, "autotune: %s", hipGetErrorString(e));
-          break;
+      for (int nch : nch_c)
+        for (int fuse = 1; fuse >= 0 && rc == TCSEM_OK; --fuse) {
+          apply_tiling(c, by, pf, nch);
+          if (c->nchunk == 0 && nch > 1) continue;  // clamped to one pass: already timed
+          if (c->nchunk == 0 && fuse == 0) continue;  // no chunks: nothing to sum
+          c->fuse = fuse;
+          hipError_t e = one_step();               // untimed: first launches of this tiling
+          if (e == hipSuccess) e = hipStreamWaitEvent(st, c->xev[1], 0);
+          if (e == hipSuccess) e = hipEventRecord(e0, st);
+          for (int q = 0; q < nsteps && e == hipSuccess; ++q) e = one_step();
+          if (e == hipSuccess) e = hipStreamWaitEvent(st, c->xev[1], 0);
+          if (e == hipSuccess) e = hipEventRecord(e1, st);
+          if (e == hipSuccess) e = hipEventSynchronize(e1);
+          float ms = 0.f;
+          if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+          if (e != hipSuccess) {
+            rc = fail(TCSEM_EDEVICE, "autotune: %s", hipGetErrorString(e));
+            break;
+          }
+          trials.push_back({(double)ms / nsteps, c->by, pf, c->nchunk > 1 ? c->nchunk : 1, fuse});
         }
-        trials.push_back({(double)ms / nsteps, c->by, pf, c->nchunk > 1 ? c->nchunk : 1});
-      }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   c->cur = cur0;
+  c->fuse = fuse0;
+  c->ib_valid = false;
   if (rc) return rc;
-  // the fastest, except that among tilings within 2% of it (timing noise) the one with the fewest
-  // z-chunks wins: chunks add the partial-sum pass over Bx and By (2 more words per cell)
+  // the fastest, except that among tilings within 1% of it (timing noise) the one with the fewest
+  // z-chunks wins: chunks add the partial sums of Bz (a k_zsum pass over Bx and By, 2 more words
+  // per cell, or the chunk sums of B accumulated in FP64 by the march)
   double best = 1e300;
   for (const Trial &t : trials) best = std::min(best, t.ms);
   const Trial *pick = nullptr;
   for (const Trial &t : trials)
-    if (t.ms <= 1.02 * best && (!pick || t.nch < pick->nch || (t.nch == pick->nch && t.ms < pick->ms))) pick = &t;
+    if (t.ms <= 1.01 * best && (!pick || t.nch < pick->nch || (t.nch == pick->nch && t.ms < pick->ms))) pick = &t;
   if (!pick) return fail(TCSEM_EDEVICE, "autotune: no tiling timed");
   *best_by = pick->by;
   *best_pf = pick->pf;
   *best_nch = pick->nch;
+  *best_fuse = pick->fuse;
   *best_ms = pick->ms;
   return TCSEM_OK;
 }
@@ -1540,7 +1609,10 @@ template <typename T>
 int batch_step_t(tcsem_batch *b, int nsteps) {
   const int nm = (int)b->m.size();
   if (int rc = batch_quiesce(b)) return rc;
-  for (int q = 0; q < nm; ++q) b->active[q] = !sampling_done(b->m[q]);
+  for (int q = 0; q < nm; ++q) {
+    b->active[q] = !sampling_done(b->m[q]);
+    b->m[q]->ib_valid = false;  // batch steps write no chunk sums
+  }
   // every member's step scalars for the nsteps steps (the schedule is field-independent)
   if (int rc = batch_scal_reserve(b, nsteps)) return rc;
   std::vector<double> scal((size_t)nsteps * nm * 4, 0.0);
@@ -2106,6 +2178,7 @@ void tcsem_destroy(tcsem_ctx *c) {
   (void)hipFree(c->d_sig);
   (void)hipFree(c->d_planes);
   (void)hipFree(c->d_zs);
+  (void)hipFree(c->d_ib);
   (void)hipFree(c->d_rd);
   (void)hipFree(c->d_grid);
   (void)hipFree(c->d_maps);
@@ -2217,6 +2290,7 @@ int tcsem_reserve_batch(tcsem_ctx *c, int nbatch) {
 
 int tcsem_reset_fields(tcsem_ctx *c, int first, int n) {
   if (check_ctx(c) || first < 0 || n < 0 || first + n > c->nslots) return fail(TCSEM_EINVAL, "bad slot range");
+  c->ib_valid = false;
   const size_t fb = (size_t)c->fstride * c->esz, pb = (size_t)c->pstride * c->esz;
   for (int f = 0; f < 10; ++f)
     HIPCHK(launch_fill_zero(c->d_fields + ((size_t)f * c->nslots + first) * fb, fb * n, c->stream));
@@ -2413,16 +2487,17 @@ int tcsem_autotune(tcsem_ctx *c, int nbatch, int nsteps, int *rows, int *pf, int
   if ((rc = tcsem_reserve_batch(c, nbatch))) return rc;
   free_run(c);  // a begun run is discarded; tcsem_begin / tcsem_run start a new one
   if ((rc = tcsem_reset_fields(c, 0, nbatch))) return rc;
-  int by = 0, p = 0, nch = 0;
+  int by = 0, p = 0, nch = 0, fuse = 1;
   double t = 0.0;
-  rc = c->prec == 64 ? autotune_march<double>(c, nbatch, nsteps, &by, &p, &nch, &t)
-                     : autotune_march<float>(c, nbatch, nsteps, &by, &p, &nch, &t);
+  rc = c->prec == 64 ? autotune_march<double>(c, nbatch, nsteps, &by, &p, &nch, &fuse, &t)
+                     : autotune_march<float>(c, nbatch, nsteps, &by, &p, &nch, &fuse, &t);
   if (rc) return rc;
   if ((rc = tcsem_reset_fields(c, 0, nbatch))) return rc;
   c->tune_nbatch = nbatch;
   c->tune_by = by;
   c->tune_pf = p;
   c->tune_nch = nch;
+  c->tune_fuse = fuse;
   choose_tiling(c, nbatch);
   if (rows) *rows = by;
   if (pf) *pf = p;
@@ -2520,6 +2595,7 @@ int tcsem_get_field(tcsem_ctx *c, int slot, int comp, double *out) {
 }
 
 int tcsem_set_field(tcsem_ctx *c, int slot, int comp, const double *in) {
+  if (c) c->ib_valid = false;
   if (check_ctx(c) || !in) return fail(TCSEM_EINVAL, "null argument");
   if (slot < 0 || slot >= c->nslots) return fail(TCSEM_EINVAL, "slot out of range");
   const int L = c->g.info.L, M = c->g.info.M, N = c->g.info.N;

@@ -75,6 +75,15 @@ struct MarchArgs {
   long long zstride;
   // model batch: lam, dt, dtb, amp of each slot for this step (4 FP64 per slot), launch record only
   const double *gscal;
+  // Chunk sums of B (chunked lower launches): with ibw set, chunk c < nchunk-1 stores, per owned
+  // column, the FP64 sums over its stored levels of dz(k) Bx(k) and dz(k) By(k) as the step leaves
+  // them in memory (planes ibw + (2c) ibstride and + (2c+1) ibstride, plane origin, slot 0).  With
+  // ibr set (the previous step's sums, same chunking), chunk c starts from the divergence of the
+  // sums of chunks 0 .. c-1 instead of the k_zsum partials: sum_k dz(k) div_h(B(k)) =
+  // div_h(sum_k dz(k) B(k)), as dz is uniform over a level.
+  const double *ibr;
+  double *ibw;
+  long long ibstride;
 };
 
 // Receiver gather descriptor (EFInterp2Rx.cu:60-470) for one field group (E or B) of one event.

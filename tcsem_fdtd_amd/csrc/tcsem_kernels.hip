@@ -233,7 +233,8 @@ __device__ inline MarchArgs<T> slot_args(const MarchArgs<T> &la, const MarchArgs
   return a;
 }
 
-template <typename T, int BY, int PF, bool GEO>
+// IB: the chunked lower launch that accumulates the chunk sums of B (a.ibw set; a.ibr may be).
+template <typename T, int BY, int PF, bool GEO, bool IB>
 __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> la, const MarchArgs<T> *geo) {
   MarchArgs<T> a;
   if constexpr (GEO) a = slot_args(la, geo[blockIdx.y]);
@@ -355,10 +356,28 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> la
     ex_n1 = vEx ? ldo((const T *)ex, off + PSb) : zero;
     ey_n1 = vEy ? ldo((const T *)ey, off + PSb) : zero;
   }
-  for (int c = 0; c < chunk; ++c) {  // Bz partial sums of the chunks below
-    bz.s += ldo((const T *)(a.zs + c * a.zstride + po), colb);
-    bzL.s += ldo((const T *)(a.zs + c * a.zstride + po), hcolb);
+  // element offsets of the lane's column and of the column i0-1 (lane 0) in a plane
+  const int ecol = (int)rowe + ic, ehal = (int)rowe + ih;
+  if (IB && a.ibr) {  // Bz partial sums of the chunks below from the previous step's chunk sums of B
+    // every interior cell of the tile (halo rows too: the first owned row's Ex reads the Bz of the
+    // row above from LDS), the cells k_zsum covers
+    if (j >= 0 && j < M && i < L) {
+      double sx = 0.0, sl = 0.0;
+      for (int c = 0; c < chunk; ++c) {
+        const double *px = a.ibr + po + (long long)(2 * c) * a.ibstride, *py = px + a.ibstride;
+        sx += (px[ecol + 1] - px[ecol]) * (double)rdx + (py[ecol + a.PX] - py[ecol]) * (double)rdy;
+        sl += (px[ehal + 1] - px[ehal]) * (double)rdxL + (py[ehal + a.PX] - py[ehal]) * (double)rdy;
+      }
+      bz.s += (T)sx;
+      bzL.s += (T)sl;
+    }
+  } else {
+    for (int c = 0; c < chunk; ++c) {  // Bz partial sums of the chunks below (k_zsum)
+      bz.s += ldo((const T *)(a.zs + c * a.zstride + po), colb);
+      bzL.s += ldo((const T *)(a.zs + c * a.zstride + po), hcolb);
+    }
   }
+  double ibx = 0.0, iby = 0.0;  // this chunk's sums of dz B (as stored) for the next step
   T bx_last = 0, by_last = 0;  // old Bx, By of level k_lo-1 after the loop
   __syncthreads();
   // Drain the set-up loads here: values loaded before the loop (the carried sums, E_new(k_hi+1))
@@ -424,6 +443,10 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> la
     const T byn = by_k + dtb * ((sEz[ly][slp] - ezn) * rdx - (ex_n1 - exn) * rdzk);
     bsto(rbx, off, vBx && st, bxn);
     bsto(rby, off, vBy && st, byn);
+    if constexpr (IB) {  // dzk is zero on a level the launch does not store
+      ibx += (double)dzk * (double)(vBx ? bxn : bx_k);
+      iby += (double)dzk * (double)(vBy ? byn : by_k);
+    }
     ex_n1 = exn;
     ey_n1 = eyn;
     off -= PSb;
@@ -439,6 +462,11 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> la
     if (k - u >= klo) level(k - u, R[u], R[(u + 1) % S], R[(u + S - 1) % S]);
   if (a.carry == 2) {  // lower launch: hand the running sums of level klo to the top launch
     if (own && chunk == max(a.nchunk, 1) - 1) bz.save(a.cs + po, a.cc + po, colb);
+    if (IB && own && chunk < a.nchunk - 1) {
+      double *px = a.ibw + po + (long long)(2 * chunk) * a.ibstride;
+      px[ecol] = ibx;
+      px[a.ibstride + ecol] = iby;
+    }
     return;
   }
   // Bz_old(0) for the upward continuation and the surface level (level 0 sits in buffer 0)
@@ -708,30 +736,33 @@ __global__ void k_fill_zero(uint32_t *p, size_t n) {
 
 // ---- launchers -----------------------------------------------------------------------------------
 // Raises a march kernel's dynamic-LDS limit once per size (static + dynamic may pass 64 KiB).
-template <typename T, int BY, int PF, bool GEO>
+template <typename T, int BY, int PF, bool GEO, bool IB>
 hipError_t march_lds(size_t bytes) {
   static size_t allowed = 0;  // per instantiation; only grows
   if (bytes <= allowed) return hipSuccess;
-  const hipError_t e = hipFuncSetAttribute((const void *)k_march<T, BY, PF, GEO>,
+  const hipError_t e = hipFuncSetAttribute((const void *)k_march<T, BY, PF, GEO, IB>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   if (e == hipSuccess) allowed = bytes;
   return e;
 }
+template <typename T, int BY, int PF, bool GEO, bool IB>
+hipError_t launch_march_v(const MarchArgs<T> &a, const MarchArgs<T> *geo, dim3 grd, dim3 blk, hipStream_t s,
+                          hipEvent_t e0, hipEvent_t e1) {
+  const size_t zl = (size_t)3 * a.N * sizeof(T);
+  hipError_t e;
+  if ((e = march_lds<T, BY, PF, GEO, IB>(zl)) != hipSuccess) return e;
+  hipExtLaunchKernelGGL((k_march<T, BY, PF, GEO, IB>), grd, blk, (std::uint32_t)zl, s, e0, e1, 0u, a, geo);
+  return hipGetLastError();
+}
 // geo == nullptr: one model (slots in grid y share a's geometry); else a model batch (a.N, a.nper:
-// the largest over the slots' records, which carry the rest).
+// the largest over the slots' records, which carry the rest).  A chunked launch with a.ibw set
+// runs the variant that accumulates the chunk sums of B.
 template <typename T, int BY, int PF>
 hipError_t launch_march_pf(const MarchArgs<T> &a, const MarchArgs<T> *geo, dim3 grd, dim3 blk, hipStream_t s,
                            hipEvent_t e0, hipEvent_t e1) {
-  const size_t zl = (size_t)3 * a.N * sizeof(T);
-  hipError_t e;
-  if (geo) {
-    if ((e = march_lds<T, BY, PF, true>(zl)) != hipSuccess) return e;
-    hipExtLaunchKernelGGL((k_march<T, BY, PF, true>), grd, blk, (std::uint32_t)zl, s, e0, e1, 0u, a, geo);
-  } else {
-    if ((e = march_lds<T, BY, PF, false>(zl)) != hipSuccess) return e;
-    hipExtLaunchKernelGGL((k_march<T, BY, PF, false>), grd, blk, (std::uint32_t)zl, s, e0, e1, 0u, a, geo);
-  }
-  return hipGetLastError();
+  if (geo) return launch_march_v<T, BY, PF, true, false>(a, geo, grd, blk, s, e0, e1);
+  if (a.ibw && a.nchunk > 1) return launch_march_v<T, BY, PF, false, true>(a, geo, grd, blk, s, e0, e1);
+  return launch_march_v<T, BY, PF, false, false>(a, geo, grd, blk, s, e0, e1);
 }
 template <typename T, int BY>
 hipError_t launch_march_by(const MarchArgs<T> &a, const MarchArgs<T> *geo, int nslot, hipStream_t s, hipEvent_t e0,

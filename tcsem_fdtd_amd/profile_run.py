@@ -33,7 +33,7 @@ def main():
     a = ap.parse_args()
     deck = W.towline(T, ntx=a.batch) if a.workload == "towline" else getattr(W, a.workload)(T)
     with T.Context(T.make_setup(**deck.setup), deck.cond, a.precision) as c:
-        tiling = None if a.no_tune else c.autotune(a.batch, 5)
+        tiling = None if a.no_tune else c.autotune(a.batch, 20)
         r = dict(deck.rxtime)
         if a.every_step_rx:
             # linear samples about four per step over the run (the source sub-steps), so that

@@ -1,5 +1,5 @@
-"""Every march tiling the library can select (rows per workgroup x prefetch depth x z-chunks, set
-through the TCSEM_MARCH_* overrides that the heuristic and the autotuner also choose from) gives
+"""Every march tiling the library can select (rows per workgroup x prefetch depth x z-chunks x the
+source of the chunks' Bz partial sums, set through the TCSEM_MARCH_* overrides that the heuristic and the autotuner also choose from) gives
 the oracle's fields: a random state advanced four full steps, FP64 at round-off and FP32 within
 single-precision accumulation error.  The deck is taller than one 16-row tile in y so that tile
 seams, halo rows and chunk seams all fall inside the model."""
@@ -56,17 +56,20 @@ def _reference(nsteps=4):
     return _REF["ref"]
 
 
-TILINGS = [(r, p, z) for r in (4, 8, 16) for p in (1, 2, 3, 4) for z in (1, 3)]
+# z-chunked tilings twice: chunks started from a k_zsum pass, and from the chunk sums of B that
+# the previous step's march accumulated (steps 2-4; step 1 follows the field upload, a pass)
+TILINGS = [(r, p, z, m) for r in (4, 8, 16) for p in (1, 2, 3, 4) for z, m in ((1, "fused"), (3, "pass"), (3, "fused"))]
 
 
 @pytest.mark.parametrize("precision", [T.FP64, T.FP32])
-@pytest.mark.parametrize("rows,pf,zc", TILINGS, ids=["r%d-pf%d-z%d" % t for t in TILINGS])
-def test_tiling_matches_oracle(rows, pf, zc, precision, monkeypatch):
+@pytest.mark.parametrize("rows,pf,zc,zsum", TILINGS, ids=["r%d-pf%d-z%d-%s" % t for t in TILINGS])
+def test_tiling_matches_oracle(rows, pf, zc, zsum, precision, monkeypatch):
     if precision == T.FP64 and rows == 16:
         pytest.skip("FP64 runs at most 8 rows per tile (the library caps it)")
     monkeypatch.setenv("TCSEM_MARCH_ROWS", str(rows))
     monkeypatch.setenv("TCSEM_MARCH_PF", str(pf))
     monkeypatch.setenv("TCSEM_MARCH_ZCHUNKS", str(zc))
+    monkeypatch.setenv("TCSEM_MARCH_ZSUM", zsum)
     st, cond, state, ref, nsteps = _reference()
     tol = 1e-10 if precision == T.FP64 else 2e-5
     with T.Context(st, cond, precision) as c:
