@@ -161,8 +161,9 @@ int tcsem_reset_fields(tcsem_ctx *ctx, int first, int n);
  * accumulated) -- and keeps the fastest for later batches of that size (TCSEM_MARCH_ROWS / _PF /
  * _ZCHUNKS / _ZSUM still override it).  It runs on the fields of slots [0, nbatch): a begun run
  * is discarded and the fields are left zeroed.  Outputs (nullable): the chosen rows, prefetch
- * depth, z-chunks and the time per step [ms]. */
-int tcsem_autotune(tcsem_ctx *ctx, int nbatch, int nsteps, int *rows, int *pf, int *zchunks, double *ms);
+ * depth, z-chunks, partial-sum source (1 chunk sums, 0 pass) and the time per step [ms]. */
+int tcsem_autotune(tcsem_ctx *ctx, int nbatch, int nsteps, int *rows, int *pf, int *zchunks, int *zsum,
+                   double *ms);
 
 /* One DuFort-Frankel step for field sets [0, nbatch), each driven by transmitter
  * tx_first+b with the schedule of the step counter.  Equivalent to main.cu:223-260:

@@ -99,7 +99,7 @@ def lib():
         "tcsem_set_transmitters": (C.c_int, [P, C.POINTER(Tx), C.c_int]),
         "tcsem_reserve_batch": (C.c_int, [P, C.c_int]),
         "tcsem_reset_fields": (C.c_int, [P, C.c_int, C.c_int]),
-        "tcsem_autotune": (C.c_int, [P, C.c_int, C.c_int, ip, ip, ip, dp]),
+        "tcsem_autotune": (C.c_int, [P, C.c_int, C.c_int, ip, ip, ip, ip, dp]),
         "tcsem_begin": (C.c_int, [P, C.c_int, C.c_int]),
         "tcsem_step": (C.c_int, [P, C.c_int]),
         "tcsem_steps_remaining": (C.c_int, [P]),
@@ -333,10 +333,11 @@ class Context:
     def autotune(self, nbatch=1, nsteps=10):
         """Times every march tiling for nbatch-transmitter steps and keeps the fastest (zeroes the
         fields of slots [0, nbatch); a begun run is discarded)."""
-        r, p, z, ms = C.c_int(), C.c_int(), C.c_int(), C.c_double()
+        r, p, z, f, ms = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_double()
         _check(self.L_.tcsem_autotune(self.h, nbatch, nsteps, C.byref(r), C.byref(p), C.byref(z),
-                                      C.byref(ms)))
-        return {"rows": r.value, "pf": p.value, "zchunks": z.value, "march_ms": ms.value}
+                                      C.byref(f), C.byref(ms)))
+        return {"rows": r.value, "pf": p.value, "zchunks": z.value,
+                "zsum": "fused" if f.value else "pass", "march_ms": ms.value}
 
     def get(self, comp, slot=0):
         a = np.zeros(self.shape(comp))

@@ -118,7 +118,7 @@ int main(int argc, char **argv) {
   if (tune && ntx > 0) {
     int rows = 0, pf = 0, zc = 0;
     double ms = 0.0;
-    if ((rc = tcsem_autotune(ctx, std::min(batch, ntx), 20, &rows, &pf, &zc, &ms))) {
+    if ((rc = tcsem_autotune(ctx, std::min(batch, ntx), 20, &rows, &pf, &zc, nullptr, &ms))) {
       tcsem_destroy(ctx);
       return die(rc, "autotune");
     }

@@ -418,7 +418,8 @@ struct tcsem_ctx {
   bool ib_valid = false;
   char *d_rd = nullptr;      // rdx, rdx2, rdy, rdy2, dzt, rdz, rdz2 (field type)
   double *d_grid = nullptr;  // X Y Z dx dy dz Xn Yn
-  int *d_maps = nullptr;     // xBx2N yo2N xo2N yBy2N
+  int *d_maps = nullptr;     // xBx2N yo2N xo2N yBy2N, then the nyr regular rows the bilinear reads
+  int nyr = 0;
   double *d_up = nullptr;    // wl wr wf wb gx gy, plan tables
   // upward-continuation work: the tiled chain's A, G, bxr, byr regions (slot strides sA, sG, sX
   // doubles), or the line chain's cplx (complex), tx, bxr, byr regions (wstride each)
@@ -720,6 +721,8 @@ UpcontArgs<T> upcont_args(tcsem_ctx *c) {
   u.yo2N = mp + (I.L + 1);
   u.xo2N = u.yo2N + I.M;
   u.yBy2N = u.xo2N + I.L;
+  u.yrows = u.yBy2N + I.M + 1;
+  u.nyr = c->nyr;
   return u;
 }
 
@@ -1401,6 +1404,9 @@ struct tcsem_batch {
   char *d_ugeo = nullptr;    // nmodel UpcontArgs
   double *d_scal = nullptr;  // steps x nmodel x (lam, dt, dtb, amp)
   int scal_cap = 0;
+  int2 *d_work = nullptr;    // march work list: (member, tile) per workgroup
+  size_t work_cap = 0;
+  int work_len = 0;          // workgroups of a march launch
   std::vector<char> active;  // per member: still sampling
   int steps = 0;
   long nenq = 0;             // batch steps enqueued (live or not)
@@ -1465,18 +1471,18 @@ int batch_records(tcsem_batch *b, int by, int pf, MarchArgs<T> &la, UpcontArgs<T
     }
     c->cur = cur0;
     UpcontArgs<T> u = up_of<T>(c);
-    if (!b->active[q]) u.L = u.M = u.M1 = u.H = 0;
+    if (!b->active[q]) u.L = u.M = u.M1 = u.H = u.nyr = 0;
     urec[q] = u;
     const tcsem_grid_info &I = c->g.info;
     la.L = std::max(la.L, I.L);
     la.M = std::max(la.M, I.M);
     la.N = std::max(la.N, I.N);
-    la.nper = std::max(la.nper, c->nper);
     la.sper = std::max(la.sper, c->sper);
     ul.L = std::max(ul.L, u.L);
     ul.M = std::max(ul.M, u.M);
     ul.M1 = std::max(ul.M1, u.M1);
     ul.H = std::max(ul.H, u.H);
+    ul.nyr = std::max(ul.nyr, u.nyr);
     ul.px.m = std::max(ul.px.m, u.px.m);
     ul.py.m = std::max(ul.py.m, u.py.m);
   }
@@ -1486,6 +1492,56 @@ int batch_records(tcsem_batch *b, int by, int pf, MarchArgs<T> &la, UpcontArgs<T
   la.k_hi = la.N - 1;
   la.k_lo = 1;
   ul.tiled = 1;
+  // March work list.  Workgroup w runs on XCD w mod 8, so list x of the 8 per-XCD lists fills
+  // workgroups x, x + 8, ...  Each active member's tiles go to one XCD where possible, which keeps
+  // the halo rows shared by its neighbouring tiles in that XCD's L2; a member with more tiles than
+  // half an XCD's share is cut into row-major bands of that size.  Pieces go, largest first, to
+  // the XCD with the least work (tiles x levels).
+  {
+    long long total = 0;
+    for (int q = 0; q < nm; ++q)
+      if (b->active[q]) total += (long long)b->m[q]->ntx * b->m[q]->nty;
+    const long long cap = std::max(1LL, (total + 15) / 16);
+    struct Piece {
+      int member, t0, n;
+      long long work;
+    };
+    std::vector<Piece> pieces;
+    for (int q = 0; q < nm; ++q) {
+      if (!b->active[q]) continue;
+      const tcsem_ctx *c = b->m[q];
+      const int nt = c->ntx * c->nty;
+      for (int t0 = 0; t0 < nt; t0 += (int)cap) {
+        const int n = (int)std::min<long long>(cap, nt - t0);
+        pieces.push_back({q, t0, n, (long long)n * c->g.info.N});
+      }
+    }
+    std::stable_sort(pieces.begin(), pieces.end(), [](const Piece &x, const Piece &y) { return x.work > y.work; });
+    std::vector<std::vector<int2>> lists(8);
+    long long load[8] = {0};
+    for (const Piece &pc : pieces) {
+      const int x = (int)(std::min_element(load, load + 8) - load);
+      for (int t = pc.t0; t < pc.t0 + pc.n; ++t) lists[x].push_back(make_int2(pc.member, t));
+      load[x] += pc.work;
+    }
+    size_t len = 1;
+    for (const auto &l : lists) len = std::max(len, l.size());
+    std::vector<int2> tbl(8 * len, make_int2(-1, -1));
+    for (int x = 0; x < 8; ++x)
+      for (size_t i = 0; i < lists[x].size(); ++i) tbl[8 * i + x] = lists[x][i];
+    if (tbl.size() > b->work_cap) {
+      (void)hipFree(b->d_work);
+      b->d_work = nullptr;
+      b->work_cap = 0;
+      if (hipMalloc((void **)&b->d_work, tbl.size() * sizeof(int2)) != hipSuccess)
+        return fail(TCSEM_ENOMEM, "work list allocation");
+      b->work_cap = tbl.size();
+    }
+    HIPCHK(hipMemcpy(b->d_work, tbl.data(), tbl.size() * sizeof(int2), hipMemcpyHostToDevice));
+    b->work_len = (int)tbl.size();
+    la.work = b->d_work;
+    la.nper = (int)len;  // the march grid: 8 x len workgroups, one row
+  }
   HIPCHK(hipMemcpy(b->d_geo, rec.data(), rec.size() * sizeof(MarchArgs<T>), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(b->d_ugeo, urec.data(), urec.size() * sizeof(UpcontArgs<T>), hipMemcpyHostToDevice));
   return TCSEM_OK;
@@ -1544,9 +1600,9 @@ int batch_enqueue(tcsem_batch *b, const MarchArgs<T> &la, const UpcontArgs<T> &u
       e = b->evpool[b->evused++];
     }
   ++b->nenq;
-  HIPCHK(launch_march<T>(lo, nm, st, te[0], te[1], geo));
+  HIPCHK(launch_march<T>(lo, 1, st, te[0], te[1], geo));  // one grid row: the work list
   HIPCHK(hipStreamWaitEvent(st, b->xev[1], 0));
-  HIPCHK(launch_march<T>(hi, nm, st, te[2], te[3], geo));
+  HIPCHK(launch_march<T>(hi, 1, st, te[2], te[3], geo));
   if (live)
     for (int q = 0; q < nm; ++q) {
       tcsem_ctx *c = b->m[q];
@@ -2130,6 +2186,24 @@ int tcsem_create(const tcsem_setup *s, const double *cond, int precision, tcsem_
       return bail(fail(TCSEM_EDEVICE, "grid upload"));
     std::vector<int> mp;
     for (auto *v : {&g.xBx2N, &g.yo2N, &g.xo2N, &g.yBy2N}) mp.insert(mp.end(), v->begin(), v->end());
+    // the regular rows the bilinear step reads (rows JJ-1 and JJ of every BxAir and ByAir row,
+    // UpContinuation_gpu.cu:42-110): the tiled chain's row inverse computes only these
+    {
+      const int M1 = g.info.M1;
+      std::vector<char> need(M1, 0);
+      auto mark = [&](int jj) {
+        jj = std::min(std::max(jj, 1), M1 - 1);
+        need[jj - 1] = need[jj] = 1;
+      };
+      for (int iy = 0; iy < g.info.M; ++iy) mark(g.yo2N[iy]);
+      for (int iy = 0; iy <= g.info.M; ++iy) mark(g.yBy2N[iy]);
+      c->nyr = 0;
+      for (int y = 0; y < M1; ++y)
+        if (need[y]) {
+          mp.push_back(y);
+          ++c->nyr;
+        }
+    }
     if (hipMalloc((void **)&c->d_maps, mp.size() * 4) != hipSuccess) return bail(fail(TCSEM_ENOMEM, "map allocation"));
     if (hipMemcpy(c->d_maps, mp.data(), mp.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
       return bail(fail(TCSEM_EDEVICE, "map upload"));
@@ -2478,7 +2552,7 @@ int tcsem_step(tcsem_ctx *c, int nsteps) {
   return join_streams(c);
 }
 
-int tcsem_autotune(tcsem_ctx *c, int nbatch, int nsteps, int *rows, int *pf, int *zchunks, double *ms) {
+int tcsem_autotune(tcsem_ctx *c, int nbatch, int nsteps, int *rows, int *pf, int *zchunks, int *zsum, double *ms) {
   if (check_ctx(c)) return TCSEM_EINVAL;
   if (nbatch < 1 || nsteps < 1 || nsteps > 1000) return fail(TCSEM_EINVAL, "batch >= 1 and 1 <= steps <= 1000");
   int rc = join_streams(c);
@@ -2502,6 +2576,7 @@ int tcsem_autotune(tcsem_ctx *c, int nbatch, int nsteps, int *rows, int *pf, int
   if (rows) *rows = by;
   if (pf) *pf = p;
   if (zchunks) *zchunks = nch;
+  if (zsum) *zsum = fuse;
   if (ms) *ms = t;
   return TCSEM_OK;
 }
@@ -2730,6 +2805,7 @@ void tcsem_batch_destroy(tcsem_batch *b) {
   (void)hipFree(b->d_geo);
   (void)hipFree(b->d_ugeo);
   (void)hipFree(b->d_scal);
+  (void)hipFree(b->d_work);
   for (auto &e : b->xev)
     if (e) (void)hipEventDestroy(e);
   if (b->done) (void)hipEventDestroy(b->done);

@@ -73,8 +73,10 @@ struct MarchArgs {
   int nchunk, chunk_len;
   T *zs;
   long long zstride;
-  // model batch: lam, dt, dtb, amp of each slot for this step (4 FP64 per slot), launch record only
+  // model batch, launch record only: lam, dt, dtb, amp of each member for this step (4 FP64 per
+  // member); the march's work list of (member, tile) per workgroup (member -1: none)
   const double *gscal;
+  const int2 *work;
   // Chunk sums of B (chunked lower launches): with ibw set, chunk c < nchunk-1 stores, per owned
   // column, the FP64 sums over its stored levels of dz(k) Bx(k) and dz(k) By(k) as the step leaves
   // them in memory (planes ibw + (2c) ibstride and + (2c+1) ibstride, plane origin, slot 0).  With
@@ -153,6 +155,8 @@ struct UpcontArgs {
   // bilinear (UpContinuation_gpu.cu:42-110)
   const double *X, *Y, *Xn, *Yn, *dx, *dy;
   const int *xBx2N, *yo2N, *xo2N, *yBy2N;
+  const int *yrows;        // the nyr regular rows (ascending) that the bilinear step reads
+  int nyr;
 };
 
 // ---- launchers (tcsem_kernels.hip, tcsem_upcont.hip) ------------------------------------------

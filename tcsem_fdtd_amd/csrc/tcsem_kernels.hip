@@ -216,9 +216,9 @@ struct Lv {
 // slot's own fields, tables, sizes and tiling, with the launch's level range and the slot's step
 // scalars (lam, dt, dtb, amp) from the launch's table row la.gscal.
 template <typename T>
-__device__ inline MarchArgs<T> slot_args(const MarchArgs<T> &la, const MarchArgs<T> &g) {
+__device__ inline MarchArgs<T> slot_args(const MarchArgs<T> &la, const MarchArgs<T> &g, int member) {
   MarchArgs<T> a = g;
-  const double *q = la.gscal + 4 * blockIdx.y;
+  const double *q = la.gscal + 4 * member;
   a.lam = q[0];
   a.dt = q[1];
   a.dtb = q[2];
@@ -237,15 +237,25 @@ __device__ inline MarchArgs<T> slot_args(const MarchArgs<T> &la, const MarchArgs
 template <typename T, int BY, int PF, bool GEO, bool IB>
 __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> la, const MarchArgs<T> *geo) {
   MarchArgs<T> a;
-  if constexpr (GEO) a = slot_args(la, geo[blockIdx.y]);
-  else a = la;
+  int tile;
+  if constexpr (GEO) {
+    // model batch: a 1-D grid over the launch's work list of (member, tile) pairs, laid out so
+    // that every member runs on one XCD (workgroup b goes to XCD b mod 8) and its halo rows stay
+    // in that XCD's L2; padding entries have member -1
+    const int2 w = la.work[blockIdx.x];
+    if (w.x < 0) return;
+    a = slot_args(la, geo[w.x], w.x);
+    tile = w.y;
+  } else {
+    a = la;
+    const int b = blockIdx.x;
+    tile = (b & 7) * a.nper + (b >> 3);  // XCD-contiguous bands of tiles
+    if (tile >= a.ntx * a.nty) return;
+  }
   extern __shared__ __align__(16) unsigned char zdyn[];  // 3 N entries of T
   constexpr int LW = kMarchLanes + 2;                   // LDS row: halo, 64 lanes, halo
   __shared__ T sBx[2][BY][LW], sBy[2][BY][LW], sS[2][BY][LW];
   __shared__ T sBz[BY][LW], sEz[BY][LW];
-  const int b = blockIdx.x;
-  const int tile = (b & 7) * a.nper + (b >> 3);  // XCD-contiguous bands of tiles
-  if ((GEO && (b >> 3) >= a.nper) || tile >= a.ntx * a.nty) return;
   const int tx = tile % a.ntx, ty = tile / a.ntx;
   const int lx = threadIdx.x, ly = threadIdx.y;
   const int L = a.L, M = a.M, N = a.N;
@@ -479,7 +489,7 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> la
 template <typename T, int BY, bool GEO>
 __global__ void __launch_bounds__(kMarchLanes *BY) k_surface(const MarchArgs<T> la, const MarchArgs<T> *geo) {
   MarchArgs<T> a;
-  if constexpr (GEO) a = slot_args(la, geo[blockIdx.y]);
+  if constexpr (GEO) a = slot_args(la, geo[blockIdx.y], (int)blockIdx.y);
   else a = la;
   __shared__ T sBx[BY][kMarchLanes], sBy[BY][kMarchLanes], sS[BY][kMarchLanes];
   __shared__ T sBz[BY][kMarchLanes], sEz[BY][kMarchLanes];

@@ -577,7 +577,7 @@ __global__ void __launch_bounds__(256) k_up_cols(const UpcontArgs<T> u0, const U
 }
 
 // blockDim (W, Q), W = 64, 128 or 256 threads per line: line q inverts regular row
-// y = blockIdx.x Q + q along x.
+// y = yrows[blockIdx.x Q + q] along x (only the rows the bilinear step reads).
 template <typename T, bool GEO>
 __global__ void __launch_bounds__(256) k_up_rows_inv(const UpcontArgs<T> u0, const UpcontArgs<T> *geo) {
   using V = typename CxOf<T>::V;
@@ -585,17 +585,17 @@ __global__ void __launch_bounds__(256) k_up_rows_inv(const UpcontArgs<T> u0, con
   V *lds = (V *)lds_raw;
   const UpcontArgs<T> u = slot_record<GEO>(u0, geo);
   const int lane = threadIdx.x, W = blockDim.x, q = threadIdx.y, nq = blockDim.y, slot = GEO ? 0 : blockIdx.y;
-  const int m = u.px.m, span = tile_span(m), L1 = u.L1, H = u.H, M1 = u.M1;
-  if (GEO && (int)blockIdx.x * nq >= M1) return;
+  const int m = u.px.m, span = tile_span(m), L1 = u.L1, H = u.H, nyr = u.nyr;
+  if (GEO && (int)blockIdx.x * nq >= nyr) return;
   const V *tw = stage_twiddles(lds + (long long)nq * 2 * span, u.px);
-  const int yb = blockIdx.x * nq;
+  const int yb = blockIdx.x * nq;  // first entry of the workgroup in the row list
   const V *G = (const V *)u.G + (long long)slot * u.sG;
   const bool evenL = (L1 & 1) == 0;
   // tile in: for each kx the nq rows are one contiguous run of (Gx, Gy) pairs
   for (int idx = q * W + lane; idx < H * nq; idx += nq * W) {
-    const int kx = idx / nq, r = idx - kx * nq, y = yb + r;
+    const int kx = idx / nq, r = idx - kx * nq, y = yb + r < nyr ? u.yrows[yb + r] : -1;
     V gx = mkc<V>(0, 0), gy = gx;
-    if (y < M1) {
+    if (y >= 0) {
       gx = G[((long long)kx * u.MG + y) * 2];
       gy = G[((long long)kx * u.MG + y) * 2 + 1];
     }
@@ -610,8 +610,8 @@ __global__ void __launch_bounds__(256) k_up_rows_inv(const UpcontArgs<T> u0, con
   __syncthreads();
   V *a = lds + (long long)q * 2 * span, *b = a + span;
   V *z = fft_line<true>(a, b, u.px, tw, +1, lane, W);
-  const int y = yb + q;
-  if (y < M1) {
+  if (yb + q < nyr) {
+    const int y = u.yrows[yb + q];
     T *bx = u.tbx + (long long)slot * u.sX + (long long)y * u.XR;
     T *by = u.tby + (long long)slot * u.sX + (long long)y * u.XR;
     for (int x = lane; x < L1; x += W) {
@@ -738,9 +738,9 @@ hipError_t launch_tiled(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEve
                         (std::uint32_t)t.lds, s, e0, nullptr, 0u, u, geo);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = launch_cols<T, GEO>(u, nslot, s, geo)) != hipSuccess) return e;
-  t = tile_shape<T>(u.px.m, (long long)u.M1 * nslot);
+  t = tile_shape<T>(u.px.m, (long long)u.nyr * nslot);
   if ((e = allow_lds(k_up_rows_inv<T, GEO>, t.lds)) != hipSuccess) return e;
-  hipLaunchKernelGGL((k_up_rows_inv<T, GEO>), dim3((u.M1 + t.q - 1) / t.q, nslot), dim3(t.w, t.q), t.lds, s, u, geo);
+  hipLaunchKernelGGL((k_up_rows_inv<T, GEO>), dim3((u.nyr + t.q - 1) / t.q, nslot), dim3(t.w, t.q), t.lds, s, u, geo);
   return hipGetLastError();
 }
 

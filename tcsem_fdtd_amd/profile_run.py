@@ -5,7 +5,9 @@
 Builds the deck, runs the library's march autotuner for the batch size (as bench.py does; its
 trial steps precede the field resets of tcsem_begin, the last k_fill_zero dispatches), advances `warmup` untimed steps, then
 `steps` steps, and prints one JSON line with the chosen tiling and the wall time per step.
---every-step-rx replaces the deck's sample times by one sample per step from the first step on,
+--workload uneven runs the uneven-model workload as one model batch (its autotuner, the field
+reset it leaves, then the same warm-up and timed steps).  --every-step-rx replaces the deck's
+sample times by one sample per step from the first step on,
 so that every step runs the receiver gathers of all receivers and components.  Used by the
 rocprof summaries under profiles/.
 """
@@ -23,14 +25,27 @@ def main():
     import tcsem_fdtd_amd as T
     from tcsem_fdtd_amd import workloads as W
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="marine_3d", choices=["marine_1d", "marine_3d", "towline"])
+    ap.add_argument("--workload", default="marine_3d", choices=["marine_1d", "marine_3d", "towline", "uneven"])
+    ap.add_argument("--models", type=int, default=64, help="uneven: number of models in the model batch")
     ap.add_argument("--precision", type=int, default=32, choices=[32, 64])
     ap.add_argument("--batch", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--every-step-rx", action="store_true")
+    ap.add_argument("--tiling", default=None,
+                    help="fixed march tiling rows,pf[,zchunks[,pass|fused]] instead of the autotuner")
     a = ap.parse_args()
+    if a.tiling:
+        f = a.tiling.split(",")
+        os.environ["TCSEM_MARCH_ROWS"], os.environ["TCSEM_MARCH_PF"] = f[0], f[1]
+        if len(f) > 2:
+            os.environ["TCSEM_MARCH_ZCHUNKS"] = f[2]
+        if len(f) > 3:
+            os.environ["TCSEM_MARCH_ZSUM"] = f[3]
+        a.no_tune = True
+    if a.workload == "uneven":
+        return models(T, W, a)
     deck = W.towline(T, ntx=a.batch) if a.workload == "towline" else getattr(W, a.workload)(T)
     with T.Context(T.make_setup(**deck.setup), deck.cond, a.precision) as c:
         tiling = None if a.no_tune else c.autotune(a.batch, 20)
@@ -56,6 +71,35 @@ def main():
         print(json.dumps({"workload": deck.name, "precision": a.precision, "batch": a.batch,
                           "steps": a.steps, "ms_per_step": 1000 * dt / a.steps,
                           "cells": c.L * c.M * c.N, "march_tiling": tiling}), flush=True)
+
+
+def models(T, W, a):
+    """The uneven-model workload as one model batch (one context and one transmitter per model)."""
+    decks = W.uneven_models(T, n=a.models)
+    ctxs = []
+    for d in decks:
+        c = T.Context(T.make_setup(**d.setup), d.cond, a.precision)
+        r = d.rxtime
+        c.set_receivers(T.make_rxtime(r["T0"], r["Tn"], r["Nt"], r["is_log"], r["comps"]), d.rx)
+        c.set_transmitters([T.make_tx(**dict(t, txid=i + 1)) for i, t in enumerate(d.txs)])
+        ctxs.append(c)
+    with T.Batch(ctxs) as b:
+        tiling = None if a.no_tune else b.autotune(10)
+        # the autotuner zeroes the members' fields; restart their runs
+        for c in ctxs:
+            c.reset(0, 1)
+        b.step(a.warmup)
+        b.sync()
+        t0 = time.perf_counter()
+        b.step(a.steps)
+        b.sync()
+        dt = time.perf_counter() - t0
+    cells = sum(c.L * c.M * c.N for c in ctxs)
+    for c in ctxs:
+        c.close()
+    print(json.dumps({"workload": "uneven_models(%d)" % a.models, "precision": a.precision,
+                      "steps": a.steps, "ms_per_step": 1000 * dt / a.steps, "cells": cells,
+                      "march_tiling": tiling}), flush=True)
 
 
 if __name__ == "__main__":

@@ -130,3 +130,19 @@ def test_create_without_gpu_fails_loudly():
     with pytest.raises(T.TcsemError) as e:
         T.Context(s, np.ones((gi.N, gi.M, gi.L)))
     assert e.value.code == -5
+
+
+def test_model_batch_argument_errors():
+    """The model-batch entry points reject bad arguments before touching the device."""
+    L = T.lib()
+    h = C.c_void_p()
+    assert L.tcsem_batch_create(None, 1, C.byref(h)) == -1
+    arr = (C.c_void_p * 2)(None, None)
+    assert L.tcsem_batch_create(arr, 0, C.byref(h)) == -1
+    assert L.tcsem_batch_create(arr, 2, C.byref(h)) == -1  # null members
+    assert b"null member" in L.tcsem_last_error()
+    assert L.tcsem_batch_step(None, 1) == -1
+    assert L.tcsem_batch_run(None, 0, None) == -1
+    assert L.tcsem_batch_autotune(None, 5, None, None, None) == -1
+    assert L.tcsem_batch_sync(None) == -1
+    L.tcsem_batch_destroy(None)
