@@ -10,7 +10,8 @@ carries the plain-FP64 and the FP32 transmitter-batched (32 x-dipoles) throughpu
 (HIP events on the library's stream), the LDS and HBM rates of the air-plane chain and of the
 receiver gathers, and the FP64 CPU oracle timed on this host as the baseline.  Every GPU run first
 lets the library's autotuner pick the march tiling for its batch size.  The line also carries
-64 conductivity models of uneven grids advanced as one model batch (FP32 and FP64).
+64 conductivity models of uneven grids advanced as one model batch (FP32 and FP64), and complete
+runs of the small 64 x 64 x 48 model.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu] [--no-tune]
 """
@@ -63,6 +64,23 @@ def gpu_run(T, deck, precision, nbatch, steps, warmup, tune=True):
         "timing": tm, "L": c.L, "M": c.M, "N": c.N, "ex_norm": float(np.linalg.norm(ex)),
         "ex": ex, "tiling": tiling,
     }
+
+
+def full_run(T, deck, precision):
+    """A complete run (tcsem_run: every step until the last sample, receivers and all), timed by the
+    wall clock around it."""
+    with T.Context(T.make_setup(**deck.setup), deck.cond, precision) as c:
+        r = deck.rxtime
+        c.set_receivers(T.make_rxtime(r["T0"], r["Tn"], r["Nt"], r["is_log"], r["comps"]), deck.rx)
+        c.set_transmitters([T.make_tx(**dict(t, txid=i + 1)) for i, t in enumerate(deck.txs)])
+        c.run(0, 1)  # first run: code objects, allocations
+        t0 = time.perf_counter()
+        c.run(0, 1)
+        sec = time.perf_counter() - t0
+        steps = c.steps_done
+        cells = c.L * c.M * c.N
+    return {"gcell_s": round(cells * steps / sec / 1e9, 3), "ms_per_step": round(1000.0 * sec / steps, 5),
+            "steps": steps, "seconds": round(sec, 4)}
 
 
 def models_run(T, decks, precision, steps, warmup, tune=True, single_steps=0):
@@ -308,6 +326,10 @@ def main():
     m64 = models_run(T, models, T.FP64, nm, wm, tune)
     models_err = float(np.linalg.norm(m32["ex"] - m64["ex"]) / max(np.linalg.norm(m64["ex"]), 1e-300))
     m32.pop("ex"), m64.pop("ex")
+    # the small 64 x 64 x 48 marine model: complete runs (host-launch-bound: a few us of GPU work
+    # per step)
+    small = W.marine_1d(T)
+    s32, s64 = full_run(T, small, T.FP32), full_run(T, small, T.FP64)
     line = {
         "metric": METRIC,
         "value": round(r32["gcell_s"], 3),
@@ -357,6 +379,8 @@ def main():
                                    "roofline": models_roofline(m64, 8), "march_tiling": m64["tiling"],
                                    "kernel_ms": {k: round(v, 5) for k, v in m64["timing"].items() if k.endswith("_ms")}},
                           "fp32_vs_fp64_ex_rel_l2_after_window": models_err},
+        "marine_1d": {"workload": "marine_1d 64x64x48, 1 x-dipole, 21 receivers, complete run (tcsem_run)",
+                      "fp32": s32, "fp64": s64},
         "ratio_fp32_batched_vs_fp64": round(rb["gcell_s"] / r64["gcell_s"], 3),
         "fp32_vs_fp64_ex_rel_l2_after_run": field_err,
     }
