@@ -66,10 +66,11 @@ def gpu_run(T, deck, precision, nbatch, steps, warmup, tune=True):
     }
 
 
-def full_run(T, deck, precision):
+def full_run(T, deck, precision, graph=0):
     """A complete run (tcsem_run: every step until the last sample, receivers and all), timed by the
-    wall clock around it."""
+    wall clock around it; graph > 0 replays sample-free stretches of that many steps as graphs."""
     with T.Context(T.make_setup(**deck.setup), deck.cond, precision) as c:
+        c.set_graph(graph)
         r = deck.rxtime
         c.set_receivers(T.make_rxtime(r["T0"], r["Tn"], r["Nt"], r["is_log"], r["comps"]), deck.rx)
         c.set_transmitters([T.make_tx(**dict(t, txid=i + 1)) for i, t in enumerate(deck.txs)])
@@ -79,8 +80,12 @@ def full_run(T, deck, precision):
         sec = time.perf_counter() - t0
         steps = c.steps_done
         cells = c.L * c.M * c.N
-    return {"gcell_s": round(cells * steps / sec / 1e9, 3), "ms_per_step": round(1000.0 * sec / steps, 5),
-            "steps": steps, "seconds": round(sec, 4)}
+        replays = c.graph_replays()
+    out = {"gcell_s": round(cells * steps / sec / 1e9, 3), "ms_per_step": round(1000.0 * sec / steps, 5),
+           "steps": steps, "seconds": round(sec, 4)}
+    if graph:
+        out.update(graph_steps=graph, graph_replays_per_run=replays // 2)
+    return out
 
 
 def models_run(T, decks, precision, steps, warmup, tune=True, single_steps=0):
@@ -330,6 +335,7 @@ def main():
     # per step)
     small = W.marine_1d(T)
     s32, s64 = full_run(T, small, T.FP32), full_run(T, small, T.FP64)
+    s32g = full_run(T, small, T.FP32, graph=16)
     line = {
         "metric": METRIC,
         "value": round(r32["gcell_s"], 3),
@@ -380,7 +386,7 @@ def main():
                                    "kernel_ms": {k: round(v, 5) for k, v in m64["timing"].items() if k.endswith("_ms")}},
                           "fp32_vs_fp64_ex_rel_l2_after_window": models_err},
         "marine_1d": {"workload": "marine_1d 64x64x48, 1 x-dipole, 21 receivers, complete run (tcsem_run)",
-                      "fp32": s32, "fp64": s64},
+                      "fp32": s32, "fp64": s64, "fp32_graph_replays": s32g},
         "ratio_fp32_batched_vs_fp64": round(rb["gcell_s"] / r64["gcell_s"], 3),
         "fp32_vs_fp64_ex_rel_l2_after_run": field_err,
     }

@@ -197,6 +197,16 @@ int tcsem_timing(const tcsem_ctx *ctx, double t[4], long n[4]);
 int tcsem_enable_timing(tcsem_ctx *ctx, int on);
 int tcsem_sync(tcsem_ctx *ctx);
 
+/* Graph replays (no counterpart in the reference, which launches every kernel of every step):
+ * with nsteps > 0 (even, at most 32), tcsem_step and tcsem_run run each stretch of nsteps
+ * steps after the source wavelet that holds no receiver sample as one replay of a HIP graph.
+ * The graph is captured on first use for its starting state; the step scalars are written to
+ * device memory before each replay.  Results equal those of the plain steps (tested).  It pays
+ * where the host's launches bound the step (small grids); 0 turns it off (the default).
+ * tcsem_graph_replays counts the replays of the context so far. */
+int tcsem_set_graph(tcsem_ctx *ctx, int nsteps);
+long tcsem_graph_replays(const tcsem_ctx *ctx);
+
 /* ---- model batch (no counterpart in the reference, which runs one model per process) --------
  * Conductivity models of uneven grids, one transmitter each, advanced together: every step's
  * volume march, air-plane chain and surface level run as one launch over all members.  Each

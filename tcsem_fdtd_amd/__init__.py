@@ -111,6 +111,8 @@ def lib():
         "tcsem_timing": (C.c_int, [P, dp, C.POINTER(C.c_long)]),
         "tcsem_enable_timing": (C.c_int, [P, C.c_int]),
         "tcsem_sync": (C.c_int, [P]),
+        "tcsem_set_graph": (C.c_int, [P, C.c_int]),
+        "tcsem_graph_replays": (C.c_long, [P]),
         "tcsem_batch_create": (C.c_int, [C.POINTER(P), C.c_int, C.POINTER(P)]),
         "tcsem_batch_autotune": (C.c_int, [P, C.c_int, ip, ip, dp]),
         "tcsem_batch_step": (C.c_int, [P, C.c_int]),
@@ -365,6 +367,13 @@ class Context:
 
     def sync(self):
         _check(self.L_.tcsem_sync(self.h))
+
+    def set_graph(self, nsteps):
+        """Replay stretches of nsteps sample-free steady-state steps as HIP graphs (0: off)."""
+        _check(self.L_.tcsem_set_graph(self.h, nsteps))
+
+    def graph_replays(self):
+        return self.L_.tcsem_graph_replays(self.h)
 
 
 # ---- model batch -----------------------------------------------------------------------------

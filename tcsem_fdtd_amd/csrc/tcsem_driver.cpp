@@ -10,7 +10,8 @@
 //   * transmitters are advanced --batch at a time in one set of launches (independent field sets
 //     sharing the conductivity);
 //   * --precision 32 runs FP32 fields and air-plane FFT (FP64 schedule, coefficients, receivers);
-//   * --autotune times every march tiling for the batch size first and keeps the fastest.
+//   * --autotune times every march tiling for the batch size first and keeps the fastest;
+//   * --graph K replays sample-free stretches of K steps as HIP graphs (same results).
 // All device work goes through the C ABI of libtcsem.so; errors print the library message and
 // exit with the (positive) error code.
 #include <algorithm>
@@ -31,18 +32,20 @@ int die(int code, const char *what) {
 
 void usage() {
   std::printf(
-      "usage: tcsem_fdtd [--dir D] [--precision 64|32] [--batch B] [--max-steps S] [--autotune] [--quiet]\n"
+      "usage: tcsem_fdtd [--dir D] [--precision 64|32] [--batch B] [--max-steps S] [--autotune] [--graph K]\n"
+      "                  [--quiet]\n"
       "  Reads FDTD_setup.in, Cond.bin, Receiver.in, Transmitter.in from D (default: .) and writes\n"
       "  Tx<id>.bout there.  --batch: transmitters advanced together (default 8).\n"
       "  --max-steps: cap on steps per transmitter (default: until the last sample).\n"
-      "  --autotune: time every march tiling for the batch size first and keep the fastest.\n");
+      "  --autotune: time every march tiling for the batch size first and keep the fastest.\n"
+      "  --graph K: replay sample-free stretches of K steps (even, <= 32) as HIP graphs.\n");
 }
 
 }  // namespace
 
 int main(int argc, char **argv) {
   std::string dir = ".";
-  int precision = TCSEM_FP64, batch = 8, max_steps = 0;
+  int precision = TCSEM_FP64, batch = 8, max_steps = 0, graph = 0;
   bool quiet = false, tune = false;
   for (int a = 1; a < argc; ++a) {
     const std::string s = argv[a];
@@ -59,6 +62,7 @@ int main(int argc, char **argv) {
     else if (s == "--max-steps") max_steps = std::atoi(next("--max-steps"));
     else if (s == "--quiet") quiet = true;
     else if (s == "--autotune") tune = true;
+    else if (s == "--graph") graph = std::atoi(next("--graph"));
     else if (s == "-h" || s == "--help") {
       usage();
       return 0;
@@ -125,6 +129,10 @@ int main(int argc, char **argv) {
     if (!quiet)
       std::printf("autotuned march: %d rows per tile, prefetch %d, %d z-chunk(s): %.4f ms per step\n", rows, pf, zc,
                   ms);
+  }
+  if (graph && (rc = tcsem_set_graph(ctx, graph))) {
+    tcsem_destroy(ctx);
+    return die(rc, "graph setup");
   }
   for (int first = 0; first < ntx; first += batch) {
     const int nb = std::min(batch, ntx - first);

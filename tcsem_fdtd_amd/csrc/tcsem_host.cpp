@@ -475,6 +475,17 @@ struct tcsem_ctx {
   std::vector<Span> spans;
   double tacc[4] = {0, 0, 0, 0};
   long tn[4] = {0, 0, 0, 0};
+  // graph replays (tcsem_set_graph): runs of graph_k steady-state steps with no receiver sample
+  // are captured once per starting state and replayed; their step scalars come from d_gs
+  int graph_k = 0;
+  double *d_gs = nullptr;
+  struct Graph {
+    int cur, ib_par, ib_valid, by, pf, nchunk, chunk_len, fuse;  // starting state and tiling
+    int cur_after, ib_par_after, ib_valid_after;
+    hipGraphExec_t exec;
+  };
+  std::vector<Graph> graphs;
+  long graph_replays = 0;
 };
 
 namespace {
@@ -726,7 +737,14 @@ UpcontArgs<T> upcont_args(tcsem_ctx *c) {
   return u;
 }
 
+// Graph executables bake device addresses: dropped whenever an allocation they use may change.
+void drop_graphs(tcsem_ctx *c) {
+  for (auto &g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+  c->graphs.clear();
+}
+
 void free_run(tcsem_ctx *c) {
+  drop_graphs(c);
   (void)hipFree(c->d_ef0);
   (void)hipFree(c->d_efs);
   (void)hipFree(c->d_src);
@@ -749,6 +767,7 @@ void free_run(tcsem_ctx *c) {
 
 int alloc_slots(tcsem_ctx *c, int nslots) {
   if (nslots <= c->nslots) return TCSEM_OK;
+  drop_graphs(c);
   HIPCHK(hipStreamSynchronize(c->stream));
   (void)hipFree(c->d_fields);
   (void)hipFree(c->d_planes);
@@ -1068,6 +1087,7 @@ hipError_t ensure_ib(tcsem_ctx *c, int nchunk) {
   hipError_t e;
   if ((e = hipStreamSynchronize(c->stream2)) != hipSuccess || (e = hipStreamSynchronize(c->stream)) != hipSuccess)
     return e;
+  drop_graphs(c);
   (void)hipFree(c->d_ib);
   c->d_ib = nullptr;
   c->ib_cap = 0;
@@ -1272,6 +1292,162 @@ int do_step(tcsem_ctx *c) {
     c->spans.push_back({1, 1, te[6], te[7]});
   }
   return TCSEM_OK;
+}
+
+// ---- graph replays ------------------------------------------------------------------------------
+// Whether a receiver sample falls in the next k steps: the sampling bookkeeping, which depends on
+// the time schedule only, run on a copy of the state.
+bool events_ahead(tcsem_ctx *c, int k) {
+  if (!c->have_rx) return false;
+  const int n0 = c->n, st0 = c->steps, ne0 = c->ne, nb0 = c->nb;
+  const double toT0 = c->toT, d10 = c->dt1, d20 = c->dt2;
+  double tp0[2][3];
+  std::memcpy(tp0, c->Tpof, sizeof tp0);
+  std::vector<int> it;
+  std::vector<double> l1;
+  const bool has_b = c->rt.is_comp[3] || c->rt.is_comp[4] || c->rt.is_comp[5];
+  bool fire = false;
+  for (int q = 0; q < k && !fire; ++q) {
+    advance_time(c);
+    const double t0 = c->toT + c->dt1;
+    for (int fld = 0; fld < 2 && !fire; ++fld) {
+      if (fld == 0 && c->ne >= c->rt.Nt) continue;
+      if (fld == 1 && (c->nb >= c->rt.Nt || !has_b)) continue;
+      Event ev{};
+      fire = sample_group(c, fld, fld == 0 ? t0 : t0 + 0.5 * c->dt2, it, l1, ev);
+    }
+  }
+  c->n = n0;
+  c->steps = st0;
+  c->ne = ne0;
+  c->nb = nb0;
+  c->toT = toT0;
+  c->dt1 = d10;
+  c->dt2 = d20;
+  std::memcpy(c->Tpof, tp0, sizeof tp0);
+  return fire;
+}
+
+// Steps that a graph may replay: past the source (no amplitude, no source pass), no timing
+// events, and no receiver sample among the next graph_k steps.
+bool graph_ok(tcsem_ctx *c) {
+  return c->graph_k > 0 && c->n >= 2 * (c->setup.n_src / 2) && !c->timing && !events_ahead(c, c->graph_k);
+}
+
+// The kernels of k steps as do_step issues them (steady state: no source pass, no gathers), with
+// the step scalars read from d_gs; the first step's wait for the previous stream-2 work is left to
+// the caller (issued before the capture).
+template <typename T>
+hipError_t enqueue_graph_steps(tcsem_ctx *c, int k) {
+  const int nb = c->nbatch;
+  hipStream_t st = c->stream, st2 = c->stream2;
+  const UpcontArgs<T> &u = up_of<T>(c);
+  hipError_t e;
+  for (int q = 0; q < k; ++q) {
+    MarchArgs<T> a = march_args<T>(c);
+    a.gs = c->d_gs + 4 * q;
+    if ((e = enqueue_march<T>(c, a, nb, st, q ? c->xev[1] : nullptr, nullptr)) != hipSuccess) return e;
+    if ((e = hipEventRecord(c->xev[0], st)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(st2, c->xev[0], 0)) != hipSuccess) return e;
+    if ((e = launch_upcont<T>(u, nb, st2)) != hipSuccess) return e;
+    if ((e = launch_surface<T>(a, nb, st2)) != hipSuccess) return e;
+    if ((e = hipEventRecord(c->xev[1], st2)) != hipSuccess) return e;
+    c->cur ^= 1;
+  }
+  return hipStreamWaitEvent(st, c->xev[1], 0);  // join stream 2 into the capture
+}
+
+// graph_k steps as one graph replay (captured on first use for this starting state).
+template <typename T>
+int graph_steps(tcsem_ctx *c) {
+  const int k = c->graph_k;
+  hipStream_t st = c->stream;
+  // the step scalars of the k steps, from a copy of the time state
+  GraphScalars gs{};
+  {
+    const int n0 = c->n, st0 = c->steps;
+    const double toT0 = c->toT, d10 = c->dt1, d20 = c->dt2;
+    for (int q = 0; q < k; ++q) {
+      MarchArgs<T> a{};
+      step_scalars<T>(c, a);
+      gs.v[4 * q] = a.lam;
+      gs.v[4 * q + 1] = a.dt;
+      gs.v[4 * q + 2] = a.dtb;
+      gs.v[4 * q + 3] = a.amp;
+      advance_time(c);
+    }
+    c->n = n0;
+    c->steps = st0;
+    c->toT = toT0;
+    c->dt1 = d10;
+    c->dt2 = d20;
+  }
+  if (c->nchunk > 1 && c->fuse) HIPCHK(ensure_ib(c, c->nchunk));  // no allocation inside a capture
+  const tcsem_ctx::Graph *g = nullptr;
+  for (const auto &x : c->graphs)
+    if (x.cur == c->cur && x.ib_par == c->ib_par && x.ib_valid == (int)c->ib_valid && x.by == c->by && x.pf == c->pf &&
+        x.nchunk == c->nchunk && x.chunk_len == c->chunk_len && x.fuse == c->fuse)
+      g = &x;
+  if (!g) {
+    tcsem_ctx::Graph ng{c->cur, c->ib_par, (int)c->ib_valid, c->by, c->pf, c->nchunk, c->chunk_len, c->fuse, 0, 0, 0,
+                        nullptr};
+    const int ibn0 = c->ib_nchunk, ibl0 = c->ib_chunk_len;
+    HIPCHK(hipStreamWaitEvent(st, c->xev[1], 0));
+    HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    const hipError_t e = enqueue_graph_steps<T>(c, k);
+    hipGraph_t graph = nullptr;
+    const hipError_t e2 = hipStreamEndCapture(st, &graph);
+    ng.cur_after = c->cur;
+    ng.ib_par_after = c->ib_par;
+    ng.ib_valid_after = c->ib_valid;
+    c->cur = ng.cur;
+    c->ib_par = ng.ib_par;
+    c->ib_valid = ng.ib_valid != 0;
+    c->ib_nchunk = ibn0;
+    c->ib_chunk_len = ibl0;
+    if (e != hipSuccess || e2 != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      c->graph_k = 0;  // run the plain steps from here on
+      return fail(TCSEM_EDEVICE, "step capture failed: %s", hipGetErrorString(e != hipSuccess ? e : e2));
+    }
+    const hipError_t e3 = hipGraphInstantiate(&ng.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e3 != hipSuccess) {
+      c->graph_k = 0;
+      return fail(TCSEM_EDEVICE, "graph instantiation failed: %s", hipGetErrorString(e3));
+    }
+    c->graphs.push_back(ng);
+    g = &c->graphs.back();
+  }
+  HIPCHK(hipStreamWaitEvent(st, c->xev[1], 0));  // the previous step's stream-2 work
+  HIPCHK(launch_put_scalars(c->d_gs, gs, k, st));
+  HIPCHK(hipGraphLaunch(g->exec, st));
+  HIPCHK(hipEventRecord(c->xev[1], st));  // the replay joined stream 2's work into stream
+  c->cur = g->cur_after;
+  c->ib_par = g->ib_par_after;
+  c->ib_valid = g->ib_valid_after != 0;
+  if (c->nchunk > 1) {
+    c->ib_nchunk = c->nchunk;
+    c->ib_chunk_len = c->chunk_len;
+  }
+  for (int q = 0; q < k; ++q) {  // host bookkeeping of the k steps (no sample fires)
+    advance_time(c);
+    if (int rc = sample_rx<T>(c, c->stream2, false)) return rc;
+  }
+  ++c->graph_replays;
+  return TCSEM_OK;
+}
+
+// One step, or graph_k steps as a graph replay when allowed and at most `room` steps may be taken.
+// Returns the steps taken (negative: error code).
+template <typename T>
+int advance(tcsem_ctx *c, long room) {
+  if (room >= c->graph_k && graph_ok(c)) {
+    const int rc = graph_steps<T>(c);
+    return rc ? rc : c->graph_k;
+  }
+  const int rc = do_step<T>(c);
+  return rc ? rc : 1;
 }
 
 // Orders everything issued on stream2 before what is issued next on stream; every entry point
@@ -2253,6 +2429,7 @@ void tcsem_destroy(tcsem_ctx *c) {
   (void)hipFree(c->d_planes);
   (void)hipFree(c->d_zs);
   (void)hipFree(c->d_ib);
+  (void)hipFree(c->d_gs);
   (void)hipFree(c->d_rd);
   (void)hipFree(c->d_grid);
   (void)hipFree(c->d_maps);
@@ -2545,9 +2722,10 @@ int tcsem_begin(tcsem_ctx *c, int tx_first, int nbatch) {
 int tcsem_step(tcsem_ctx *c, int nsteps) {
   if (check_ctx(c)) return TCSEM_EINVAL;
   if (!c->begun) return fail(TCSEM_ESTATE, "tcsem_begin first");
-  for (int q = 0; q < nsteps; ++q) {
-    const int rc = c->prec == 64 ? do_step<double>(c) : do_step<float>(c);
-    if (rc) return rc;
+  for (int q = 0; q < nsteps;) {
+    const int r = c->prec == 64 ? advance<double>(c, nsteps - q) : advance<float>(c, nsteps - q);
+    if (r < 0) return r;
+    q += r;
   }
   return join_streams(c);
 }
@@ -2605,7 +2783,9 @@ int tcsem_run(tcsem_ctx *c, int tx_first, int ntx, int max_steps, double *traces
   int rc = tcsem_begin(c, tx_first, ntx);
   if (rc) return rc;
   while (!sampling_done(c) && (max_steps <= 0 || c->steps < max_steps)) {
-    if ((rc = c->prec == 64 ? do_step<double>(c) : do_step<float>(c))) return rc;
+    const long room = max_steps > 0 ? max_steps - c->steps : (long)1 << 40;
+    const int r = c->prec == 64 ? advance<double>(c, room) : advance<float>(c, room);
+    if (r < 0) return r;
   }
   if ((rc = join_streams(c))) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -2862,6 +3042,22 @@ int tcsem_batch_run(tcsem_batch *b, int max_steps, int *steps_done) {
   if (steps_done) *steps_done = b->steps;
   return TCSEM_OK;
 }
+
+int tcsem_set_graph(tcsem_ctx *c, int nsteps) {
+  if (check_ctx(c)) return TCSEM_EINVAL;
+  if (nsteps != 0 && (nsteps < 2 || nsteps > kGraphMaxSteps || (nsteps & 1)))
+    return fail(TCSEM_EINVAL, "graph steps: 0 (off) or even, 2 .. %d", kGraphMaxSteps);
+  if (nsteps && !c->d_gs &&
+      hipMalloc((void **)&c->d_gs, (size_t)4 * kGraphMaxSteps * sizeof(double)) != hipSuccess)
+    return fail(TCSEM_ENOMEM, "graph scalar allocation");
+  if (int rc = join_streams(c)) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  drop_graphs(c);
+  c->graph_k = nsteps;
+  return TCSEM_OK;
+}
+
+long tcsem_graph_replays(const tcsem_ctx *c) { return c ? c->graph_replays : -1; }
 
 int tcsem_batch_enable_timing(tcsem_batch *b, int on) {
   if (!b) return fail(TCSEM_EINVAL, "null batch");

@@ -77,6 +77,9 @@ struct MarchArgs {
   // member); the march's work list of (member, tile) per workgroup (member -1: none)
   const double *gscal;
   const int2 *work;
+  // graph replay: lam, dt, dtb, amp of this step from device memory (4 FP64) instead of the
+  // scalars above, which a captured launch would freeze (null: use the scalars)
+  const double *gs;
   // Chunk sums of B (chunked lower launches): with ibw set, chunk c < nchunk-1 stores, per owned
   // column, the FP64 sums over its stored levels of dz(k) Bx(k) and dz(k) By(k) as the step leaves
   // them in memory (planes ibw + (2c) ibstride and + (2c+1) ibstride, plane origin, slot 0).  With
@@ -159,6 +162,17 @@ struct UpcontArgs {
   int nyr;
 };
 
+// A launch whose dispatch packet records e0 at its start and e1 at its end when either is given;
+// without events a plain launch (the form that a stream capture records).
+template <typename F, typename... A>
+inline void launch_k(F kern, dim3 grid, dim3 block, std::uint32_t lds, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                     A... args) {
+  if (e0 || e1)
+    hipExtLaunchKernelGGL(kern, grid, block, lds, s, e0, e1, 0u, args...);
+  else
+    hipLaunchKernelGGL(kern, grid, block, lds, s, args...);
+}
+
 // ---- launchers (tcsem_kernels.hip, tcsem_upcont.hip) ------------------------------------------
 // Optional events e0/e1 are recorded by the dispatch packets themselves (start of the first and
 // end of the last kernel of the launcher), so timing adds no packets to the stream.
@@ -187,5 +201,11 @@ template <typename T>
 hipError_t launch_upcont(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEvent_t e0 = nullptr,
                          hipEvent_t e1 = nullptr, const UpcontArgs<T> *geo = nullptr);
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);
+// Step scalars of a graph replay: dst[0 .. 4 nsteps) = the values (lam, dt, dtb, amp per step).
+constexpr int kGraphMaxSteps = 32;
+struct GraphScalars {
+  double v[4 * kGraphMaxSteps];
+};
+hipError_t launch_put_scalars(double *dst, const GraphScalars &s, int nsteps, hipStream_t st);
 
 }  // namespace tcsem

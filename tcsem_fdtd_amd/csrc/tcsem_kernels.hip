@@ -251,6 +251,11 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> la
     const int b = blockIdx.x;
     tile = (b & 7) * a.nper + (b >> 3);  // XCD-contiguous bands of tiles
     if (tile >= a.ntx * a.nty) return;
+    if (a.gs) {  // graph replay: this step's scalars from device memory
+      a.lam = a.gs[0];
+      a.dt = a.gs[1];
+      a.dtb = a.gs[2];
+    }
   }
   extern __shared__ __align__(16) unsigned char zdyn[];  // 3 N entries of T
   constexpr int LW = kMarchLanes + 2;                   // LDS row: halo, 64 lanes, halo
@@ -489,8 +494,17 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> la
 template <typename T, int BY, bool GEO>
 __global__ void __launch_bounds__(kMarchLanes *BY) k_surface(const MarchArgs<T> la, const MarchArgs<T> *geo) {
   MarchArgs<T> a;
-  if constexpr (GEO) a = slot_args(la, geo[blockIdx.y], (int)blockIdx.y);
-  else a = la;
+  if constexpr (GEO) {
+    a = slot_args(la, geo[blockIdx.y], (int)blockIdx.y);
+  } else {
+    a = la;
+    if (a.gs) {  // graph replay: this step's scalars from device memory
+      a.lam = a.gs[0];
+      a.dt = a.gs[1];
+      a.dtb = a.gs[2];
+      a.amp = a.gs[3];
+    }
+  }
   __shared__ T sBx[BY][kMarchLanes], sBy[BY][kMarchLanes], sS[BY][kMarchLanes];
   __shared__ T sBz[BY][kMarchLanes], sEz[BY][kMarchLanes];
   Tile t;
@@ -761,7 +775,7 @@ hipError_t launch_march_v(const MarchArgs<T> &a, const MarchArgs<T> *geo, dim3 g
   const size_t zl = (size_t)3 * a.N * sizeof(T);
   hipError_t e;
   if ((e = march_lds<T, BY, PF, GEO, IB>(zl)) != hipSuccess) return e;
-  hipExtLaunchKernelGGL((k_march<T, BY, PF, GEO, IB>), grd, blk, (std::uint32_t)zl, s, e0, e1, 0u, a, geo);
+  launch_k(k_march<T, BY, PF, GEO, IB>, grd, blk, (std::uint32_t)zl, s, e0, e1, a, geo);
   return hipGetLastError();
 }
 // geo == nullptr: one model (slots in grid y share a's geometry); else a model batch (a.N, a.nper:
@@ -798,8 +812,7 @@ hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEven
 }
 template <typename T>
 hipError_t launch_zsum(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEvent_t e0) {
-  hipExtLaunchKernelGGL((k_zsum<T>), dim3((a.L + 255) / 256, a.M, (a.nchunk - 1) * nslot), dim3(256), 0u, s, e0,
-                        nullptr, 0u, a);
+  launch_k(k_zsum<T>, dim3((a.L + 255) / 256, a.M, (a.nchunk - 1) * nslot), dim3(256), 0u, s, e0, nullptr, a);
   return hipGetLastError();
 }
 template <typename T>
@@ -807,9 +820,9 @@ hipError_t launch_surface(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEv
                           const MarchArgs<T> *geo) {
   dim3 blk(kMarchLanes, kSurfaceRows), grd(8 * a.sper, nslot);
   if (geo)
-    hipExtLaunchKernelGGL((k_surface<T, kSurfaceRows, true>), grd, blk, 0u, s, e0, e1, 0u, a, geo);
+    launch_k(k_surface<T, kSurfaceRows, true>, grd, blk, 0u, s, e0, e1, a, geo);
   else
-    hipExtLaunchKernelGGL((k_surface<T, kSurfaceRows, false>), grd, blk, 0u, s, e0, e1, 0u, a, geo);
+    launch_k(k_surface<T, kSurfaceRows, false>, grd, blk, 0u, s, e0, e1, a, geo);
   return hipGetLastError();
 }
 template <typename T>
@@ -832,7 +845,16 @@ hipError_t launch_gather(const GatherArgs &g, int is_b, int nslot, const MarchAr
                          hipEvent_t e0, hipEvent_t e1) {
   (void)is_b;
   dim3 blk(kGatherThreads), grd(g.nrmax * g.ncomp_group, nslot);
-  hipExtLaunchKernelGGL((k_gather<T>), grd, blk, 0u, s, e0, e1, 0u, g, a);
+  launch_k(k_gather<T>, grd, blk, 0u, s, e0, e1, g, a);
+  return hipGetLastError();
+}
+__global__ void k_put_scalars(double *dst, const GraphScalars s, int n) {
+  const int t = threadIdx.x;
+  if (t < n) dst[t] = s.v[t];
+}
+hipError_t launch_put_scalars(double *dst, const GraphScalars &s, int nsteps, hipStream_t st) {
+  if (nsteps < 1 || nsteps > kGraphMaxSteps) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_put_scalars, dim3(1), dim3(4 * kGraphMaxSteps), 0, st, dst, s, 4 * nsteps);
   return hipGetLastError();
 }
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s) {

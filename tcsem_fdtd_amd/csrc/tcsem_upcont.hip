@@ -734,8 +734,8 @@ hipError_t launch_tiled(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEve
   Tile t = tile_shape<T>(u.px.m, (long long)npair * nslot);
   hipError_t e = allow_lds(k_up_rows_fwd<T, GEO>, t.lds);
   if (e != hipSuccess) return e;
-  hipExtLaunchKernelGGL((k_up_rows_fwd<T, GEO>), dim3((npair + t.q - 1) / t.q, nslot), dim3(t.w, t.q),
-                        (std::uint32_t)t.lds, s, e0, nullptr, 0u, u, geo);
+  launch_k(k_up_rows_fwd<T, GEO>, dim3((npair + t.q - 1) / t.q, nslot), dim3(t.w, t.q), (std::uint32_t)t.lds, s, e0,
+           nullptr, u, geo);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if ((e = launch_cols<T, GEO>(u, nslot, s, geo)) != hipSuccess) return e;
   t = tile_shape<T>(u.px.m, (long long)u.nyr * nslot);
@@ -747,8 +747,7 @@ hipError_t launch_tiled(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEve
 template <typename T>
 hipError_t launch_lined(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEvent_t e0) {
   hipError_t e;
-  hipExtLaunchKernelGGL((k_up_xpass<T>), dim3(u.M, nslot), dim3(256), (std::uint32_t)(u.L * sizeof(double)), s, e0,
-                        nullptr, 0u, u);
+  launch_k(k_up_xpass<T>, dim3(u.M, nslot), dim3(256), (std::uint32_t)(u.L * sizeof(double)), s, e0, nullptr, u);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   const int npad = u.M1 - u.Ny;
   if (npad > 0) {
@@ -778,15 +777,15 @@ hipError_t launch_upcont(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEv
                      : (u.tiled ? launch_tiled<T, false>(u, nslot, s, e0, nullptr) : launch_lined<T>(u, nslot, s, e0));
   if (e != hipSuccess) return e;
   const int n = u.M * (u.L + 1) + (u.M + 1) * u.L;
+  const dim3 grid((n + 255) / 256, nslot), block(256);
   if (geo)
-    hipExtLaunchKernelGGL((k_up_bilin<T, T, true>), dim3((n + 255) / 256, nslot), dim3(256), 0u, s, nullptr, e1, 0u,
-                          u, (const T *)nullptr, (const T *)nullptr, geo);
+    launch_k(k_up_bilin<T, T, true>, grid, block, 0u, s, nullptr, e1, u, (const T *)nullptr, (const T *)nullptr, geo);
   else if (u.tiled)
-    hipExtLaunchKernelGGL((k_up_bilin<T, T, false>), dim3((n + 255) / 256, nslot), dim3(256), 0u, s, nullptr, e1,
-                          0u, u, (const T *)u.tbx, (const T *)u.tby, (const UpcontArgs<T> *)nullptr);
+    launch_k(k_up_bilin<T, T, false>, grid, block, 0u, s, nullptr, e1, u, (const T *)u.tbx, (const T *)u.tby,
+             (const UpcontArgs<T> *)nullptr);
   else
-    hipExtLaunchKernelGGL((k_up_bilin<T, double, false>), dim3((n + 255) / 256, nslot), dim3(256), 0u, s, nullptr,
-                          e1, 0u, u, (const double *)u.bxr, (const double *)u.byr, (const UpcontArgs<T> *)nullptr);
+    launch_k(k_up_bilin<T, double, false>, grid, block, 0u, s, nullptr, e1, u, (const double *)u.bxr,
+             (const double *)u.byr, (const UpcontArgs<T> *)nullptr);
   return hipGetLastError();
 }
 

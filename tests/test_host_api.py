@@ -146,3 +146,9 @@ def test_model_batch_argument_errors():
     assert L.tcsem_batch_autotune(None, 5, None, None, None) == -1
     assert L.tcsem_batch_sync(None) == -1
     L.tcsem_batch_destroy(None)
+
+
+def test_graph_argument_errors():
+    L = T.lib()
+    assert L.tcsem_set_graph(None, 8) == -1
+    assert L.tcsem_graph_replays(None) == -1
