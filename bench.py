@@ -139,16 +139,17 @@ def models_run(T, decks, precision, steps, warmup, tune=True, single_steps=0):
     return out
 
 
-def models_roofline(res, esz):
+def models_roofline(res, esz, traffic_entry=(None, None)):
     algo = res["march_cells"] * WORDS_PER_CELL * esz
     ms = res["timing"]["march_ms"]
     gbs = algo / (ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": None, "kernel": "k_march (model batch)",
+            "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": traffic_entry[0],
+            "traffic_tiling": traffic_entry[1], "kernel": "k_march (model batch)",
             "algorithmic_bytes_per_launch": int(algo), "avg_launch_ms": round(ms, 5)}
 
 
-def roofline(res, esz, nbatch, traffic):
+def roofline(res, esz, nbatch, traffic_entry):
     # the march advances levels 1..N-1 of every column: 13 words per cell, sigma shared by the
     # batch ((12 + 1/B) words per cell-update)
     cells = res["L"] * res["M"] * (res["N"] - 1) * nbatch
@@ -156,8 +157,10 @@ def roofline(res, esz, nbatch, traffic):
     algo_bytes = cells * words * esz
     ms = res["timing"]["march_ms"]
     achieved = algo_bytes / (ms * 1e-3) / 1e9
+    traffic, traffic_tiling = traffic_entry if isinstance(traffic_entry, tuple) else (traffic_entry, None)
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
             "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+            "traffic_tiling": traffic_tiling,
             "kernel": "k_march", "algorithmic_bytes_per_launch": int(algo_bytes),
             "avg_launch_ms": round(ms, 5)}
 
@@ -280,12 +283,17 @@ def cpu_baseline(deck, budget_s=12.0, max_steps=30):
 
 
 def load_traffic(key):
+    """HBM bytes per step of k_march from the committed counter summary (profiles/traffic.json),
+    with the march tiling they were counted for; (None, None) when absent."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(p) as f:
-            return json.load(f).get(key)
+            v = json.load(f).get(key)
     except (OSError, ValueError):
-        return None
+        return None, None
+    if isinstance(v, dict):
+        return v.get("bytes"), v.get("tiling")
+    return v, None
 
 
 def main():
@@ -377,7 +385,8 @@ def main():
         "uneven_models": {"workload": "64 models, grids 96^3 .. 224x224x160 (%d cells), 1 x-dipole and 41 receivers each"
                                       % m32["cells"], "steps": nm, "warmup": wm,
                           "fp32": {"value": round(m32["gcell_s"], 3), "ms_per_step": round(m32["ms_per_step"], 5),
-                                   "roofline": models_roofline(m32, 4), "march_tiling": m32["tiling"],
+                                   "roofline": models_roofline(m32, 4, load_traffic("uneven64_fp32")),
+                                   "march_tiling": m32["tiling"],
                                    "kernel_ms": {k: round(v, 5) for k, v in m32["timing"].items() if k.endswith("_ms")},
                                    "unbatched": {k: round(v, 5) if isinstance(v, float) else v
                                                  for k, v in m32["unbatched"].items()}},

@@ -1865,10 +1865,13 @@ template <typename T>
 int batch_step_t(tcsem_batch *b, int nsteps) {
   const int nm = (int)b->m.size();
   if (int rc = batch_quiesce(b)) return rc;
+  bool any = false;
   for (int q = 0; q < nm; ++q) {
     b->active[q] = !sampling_done(b->m[q]);
+    any = any || b->active[q];
     b->m[q]->ib_valid = false;  // batch steps write no chunk sums
   }
+  if (!any) return TCSEM_OK;  // every member has taken all its samples
   // every member's step scalars for the nsteps steps (the schedule is field-independent)
   if (int rc = batch_scal_reserve(b, nsteps)) return rc;
   std::vector<double> scal((size_t)nsteps * nm * 4, 0.0);

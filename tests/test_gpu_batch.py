@@ -112,3 +112,39 @@ def test_batch_autotune_keeps_results():
     for q in range(len(MODELS)):
         err = decks.trace_error(got[q], ref[q])
         assert np.all(err < 1e-5), (q, err)
+
+
+def test_batch_refuses_inconsistent_members():
+    """Members of different precisions, a member listed twice and a member that was not begun
+    are refused; the members stay usable."""
+    import ctypes as C
+    a = _ctx(MODELS[0], CONDS[0], T.FP64)
+    b = _ctx(MODELS[1], CONDS[1], T.FP32)
+    c = _ctx(MODELS[2], CONDS[2], T.FP64)
+    try:
+        with pytest.raises(T.TcsemError, match="precision"):
+            T.Batch([a, b])
+        h = C.c_void_p()
+        assert T.lib().tcsem_batch_create((C.c_void_p * 2)(a.h.value, a.h.value), 2, C.byref(h)) == -1
+        assert b"twice" in T.lib().tcsem_last_error()
+        assert T.lib().tcsem_batch_create((C.c_void_p * 2)(a.h.value, c.h.value), 2, C.byref(h)) == -7
+        with T.Batch([a, c]) as bt:
+            bt.run(max_steps=20)
+            assert bt.steps_done == 20
+    finally:
+        for x in (a, b, c):
+            x.close()
+
+
+def test_batch_step_after_every_member_finished():
+    ctxs = [_ctx(kw, cond, T.FP32, tn=0.03) for kw, cond in zip(MODELS[:2], CONDS[:2])]
+    try:
+        with T.Batch(ctxs) as bt:
+            got = bt.run()
+            steps = bt.steps_done
+            bt.step(10)  # nothing left to step: a no-op
+            assert all(np.array_equal(c.traces(0), g) for c, g in zip(ctxs, got))
+            assert steps > 0
+    finally:
+        for c in ctxs:
+            c.close()
