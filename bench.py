@@ -14,6 +14,7 @@ lets the library's autotuner pick the march tiling for its batch size.  The line
 runs of the small 64 x 64 x 48 model.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu] [--no-tune]
+    python bench.py --accuracy   # complete batched runs, FP32 vs FP64 trace errors
 """
 import argparse
 import json
@@ -282,6 +283,65 @@ def cpu_baseline(deck, budget_s=12.0, max_steps=30):
                       % (n, m.L, m.M, m.N)}
 
 
+def trace_error(got, ref, floor=1e-6):
+    """Per receiver: ||a-b|| / ||b|| over all times and components, components below floor x the
+    receiver's largest component norm excluded (symmetry nulls)."""
+    import numpy as np
+    out = []
+    for r in range(ref.shape[0]):
+        norms = np.linalg.norm(ref[r], axis=1)
+        keep = norms >= floor * norms.max()
+        nb = np.linalg.norm(ref[r][keep])
+        out.append(np.linalg.norm(got[r][keep] - ref[r][keep]) / (nb if nb > 0 else 1.0))
+    return np.array(out)
+
+
+def accuracy(T, W):
+    """Complete runs of the batched workloads (32-transmitter towline, 64 uneven models) with FP32
+    and with FP64 fields: per-receiver relative trace error of FP32 against FP64, and the wall
+    time of each complete run."""
+    import numpy as np
+    out = {}
+    tow = W.towline(T, ntx=32)
+    res, sec = {}, {}
+    for prec in (T.FP64, T.FP32):
+        with T.Context(T.make_setup(**tow.setup), tow.cond, prec) as c:
+            r = tow.rxtime
+            c.set_receivers(T.make_rxtime(r["T0"], r["Tn"], r["Nt"], r["is_log"], r["comps"]), tow.rx)
+            c.set_transmitters([T.make_tx(**dict(t, txid=i + 1)) for i, t in enumerate(tow.txs)])
+            c.autotune(32, 10)
+            t0 = time.perf_counter()
+            res[prec] = c.run(0, 32)
+            sec[prec] = time.perf_counter() - t0
+            steps = c.steps_done
+    e = np.concatenate([trace_error(res[T.FP32][b], res[T.FP64][b]) for b in range(32)])
+    out["towline32"] = {"steps": steps, "receivers": int(e.size), "max": float(e.max()), "median": float(np.median(e)),
+                        "fp32_run_s": round(sec[T.FP32], 1), "fp64_run_s": round(sec[T.FP64], 1)}
+    models = W.uneven_models(T)
+    res, sec = {}, {}
+    for prec in (T.FP64, T.FP32):
+        ctxs = []
+        for d in models:
+            c = T.Context(T.make_setup(**d.setup), d.cond, prec)
+            r = d.rxtime
+            c.set_receivers(T.make_rxtime(r["T0"], r["Tn"], r["Nt"], r["is_log"], r["comps"]), d.rx)
+            c.set_transmitters([T.make_tx(**dict(t, txid=i + 1)) for i, t in enumerate(d.txs)])
+            ctxs.append(c)
+        with T.Batch(ctxs) as b:
+            b.autotune(5)
+            t0 = time.perf_counter()
+            res[prec] = b.run()
+            sec[prec] = time.perf_counter() - t0
+            steps = b.steps_done
+        for c in ctxs:
+            c.close()
+    e = np.concatenate([trace_error(res[T.FP32][q], res[T.FP64][q]) for q in range(len(models))])
+    out["uneven_models64"] = {"steps": steps, "receivers": int(e.size), "max": float(e.max()),
+                              "median": float(np.median(e)), "fp32_run_s": round(sec[T.FP32], 1),
+                              "fp64_run_s": round(sec[T.FP64], 1)}
+    return out
+
+
 def load_traffic(key):
     """HBM bytes per step of k_march from the committed counter summary (profiles/traffic.json),
     with the march tiling they were counted for; (None, None) when absent."""
@@ -307,6 +367,8 @@ def main():
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="heuristic march tiling instead of the autotuner")
+    ap.add_argument("--accuracy", action="store_true",
+                    help="instead of the throughput line: complete batched runs, FP32 vs FP64 trace errors (minutes)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (multi-GPU sharding is design-only); "
@@ -316,6 +378,10 @@ def main():
     import tcsem_fdtd_amd as T
     from tcsem_fdtd_amd import workloads as W
 
+    if args.accuracy:
+        print(json.dumps({"accuracy": "FP32 vs FP64 per-receiver relative trace error, complete runs",
+                          **accuracy(T, W)}), flush=True)
+        return
     deck = W.marine_3d(T)
     tune = not args.no_tune
     r32 = gpu_run(T, deck, T.FP32, 1, args.steps, args.warmup, tune)
