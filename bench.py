@@ -163,7 +163,8 @@ def roofline(res, esz, nbatch, traffic_entry):
             "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
             "traffic_tiling": traffic_tiling,
             "kernel": "k_march", "algorithmic_bytes_per_launch": int(algo_bytes),
-            "avg_launch_ms": round(ms, 5)}
+            "avg_launch_ms": round(ms, 5),
+            "span": "per step: the lower launch (from its k_zsum pass when the tiling runs one) and the top launch"}
 
 
 def _fft_len(n):
