@@ -1130,7 +1130,7 @@ hipError_t enqueue_march(tcsem_ctx *c, const MarchArgs<T> &a, int nb, hipStream_
         lo.ibw = ib_planes(c, c->ib_par ^ 1);
         lo.ibr = fused ? ib_planes(c, c->ib_par) : nullptr;
       }
-      if (!fused && (e = launch_zsum<T>(lo, nb, st, te[0])) != hipSuccess) return e;
+      if (!fused && (e = launch_zsum<T>(lo, nb, st, nullptr)) != hipSuccess) return e;
       c->ib_par ^= 1;
       c->ib_valid = c->fuse != 0;
       c->ib_nchunk = c->nchunk;
@@ -1138,7 +1138,7 @@ hipError_t enqueue_march(tcsem_ctx *c, const MarchArgs<T> &a, int nb, hipStream_
     } else {
       c->ib_valid = false;
     }
-    if ((e = launch_march<T>(lo, nb, st, (chunked && !fused) ? nullptr : te[0], te[1])) != hipSuccess) return e;
+    if ((e = launch_march<T>(lo, nb, st, te[0], te[1])) != hipSuccess) return e;
   }
   if (wait && (e = hipStreamWaitEvent(st, wait, 0)) != hipSuccess) return e;
   MarchArgs<T> hi = a;
