@@ -11,7 +11,9 @@
 //     sharing the conductivity);
 //   * --precision 32 runs FP32 fields and air-plane FFT (FP64 schedule, coefficients, receivers);
 //   * --autotune times every march tiling for the batch size first and keeps the fastest;
-//   * --graph K replays sample-free stretches of K steps as HIP graphs (same results).
+//   * --graph K replays sample-free stretches of K steps as HIP graphs (same results);
+//   * several --dir decks (models of any grid sizes) run as one model batch, the n-th
+//     transmitter of every deck together, each deck writing its own Tx<id>.bout.
 // All device work goes through the C ABI of libtcsem.so; errors print the library message and
 // exit with the (positive) error code.
 #include <algorithm>
@@ -32,19 +34,126 @@ int die(int code, const char *what) {
 
 void usage() {
   std::printf(
-      "usage: tcsem_fdtd [--dir D] [--precision 64|32] [--batch B] [--max-steps S] [--autotune] [--graph K]\n"
+      "usage: tcsem_fdtd [--dir D]... [--precision 64|32] [--batch B] [--max-steps S] [--autotune] [--graph K]\n"
       "                  [--quiet]\n"
       "  Reads FDTD_setup.in, Cond.bin, Receiver.in, Transmitter.in from D (default: .) and writes\n"
       "  Tx<id>.bout there.  --batch: transmitters advanced together (default 8).\n"
       "  --max-steps: cap on steps per transmitter (default: until the last sample).\n"
       "  --autotune: time every march tiling for the batch size first and keep the fastest.\n"
-      "  --graph K: replay sample-free stretches of K steps (even, <= 32) as HIP graphs.\n");
+      "  --graph K: replay sample-free stretches of K steps (even, <= 32) as HIP graphs.\n"
+      "  Several --dir: the decks (models of any grid size) run as one model batch, transmitter by\n"
+      "  transmitter (the n-th transmitter of every deck together); each deck gets its own files.\n");
+}
+
+// One deck directory: the reference's four input files, read through the C ABI.
+struct Deck {
+  std::string dir;
+  tcsem_setup setup{};
+  tcsem_grid_info gi{};
+  std::vector<double> cond;
+  tcsem_rxtime rt{};
+  int nrx = 0, ntx = 0, ncomp = 0;
+  std::vector<double> rxyz;
+  std::vector<tcsem_tx> txs;
+  std::vector<int> ids;
+  tcsem_ctx *ctx = nullptr;
+  std::string path(const char *f) const { return dir + "/" + f; }
+};
+
+int load_deck(Deck &d) {
+  int rc = tcsem_read_setup(d.path("FDTD_setup.in").c_str(), &d.setup);
+  if (rc) return die(rc, ("reading " + d.path("FDTD_setup.in")).c_str());
+  if ((rc = tcsem_grid_info_of(&d.setup, &d.gi))) return die(rc, "grid setup");
+  const size_t ncell = (size_t)d.gi.L * d.gi.M * d.gi.N;
+  d.cond.assign(ncell, 0.0);
+  if ((rc = tcsem_read_cond(d.path("Cond.bin").c_str(), ncell, d.cond.data())))
+    return die(rc, ("reading " + d.path("Cond.bin")).c_str());
+  if ((rc = tcsem_read_receivers(d.path("Receiver.in").c_str(), &d.rt, &d.nrx, nullptr, 0)))
+    return die(rc, ("reading " + d.path("Receiver.in")).c_str());
+  d.rxyz.assign(3 * (size_t)d.nrx, 0.0);
+  if ((rc = tcsem_read_receivers(d.path("Receiver.in").c_str(), &d.rt, &d.nrx, d.rxyz.data(), d.nrx)))
+    return die(rc, "reading Receiver.in");
+  int nids = 0;
+  if ((rc = tcsem_read_transmitters(d.path("Transmitter.in").c_str(), nullptr, 0, &d.ntx, nullptr, 0, &nids)))
+    return die(rc, ("reading " + d.path("Transmitter.in")).c_str());
+  d.txs.resize(d.ntx);
+  d.ids.assign(nids > 0 ? nids : 1, 0);
+  if ((rc = tcsem_read_transmitters(d.path("Transmitter.in").c_str(), d.txs.data(), d.ntx, &d.ntx, d.ids.data(), nids,
+                                    &nids)))
+    return die(rc, "reading Transmitter.in");
+  d.ncomp = 0;
+  for (int c = 0; c < 6; ++c) d.ncomp += d.rt.is_comp[c] ? 1 : 0;
+  return 0;
+}
+
+int setup_context(Deck &d, int precision) {
+  int rc = tcsem_create(&d.setup, d.cond.data(), precision, &d.ctx);
+  if (rc) return die(rc, ("device setup of " + d.dir).c_str());
+  d.cond.clear();
+  d.cond.shrink_to_fit();
+  if ((rc = tcsem_set_receivers(d.ctx, &d.rt, d.nrx, d.rxyz.data()))) return die(rc, "receiver setup");
+  if ((rc = tcsem_set_transmitters(d.ctx, d.txs.data(), d.ntx))) return die(rc, "transmitter setup");
+  return 0;
+}
+
+int write_traces(const Deck &d, const tcsem_tx &t, const double *traces, int steps, bool quiet) {
+  char name[64];
+  std::snprintf(name, sizeof name, "Tx%d.bout", t.id);
+  const int rc = tcsem_write_bout(d.path(name).c_str(), t.nr, d.ncomp, d.rt.Nt, traces);
+  if (rc) return die(rc, "writing traces");
+  if (!quiet) std::printf("%s: Tx %d: %d steps, %d receivers -> %s\n", d.dir.c_str(), t.id, steps, t.nr, name);
+  return 0;
+}
+
+// Several decks as one model batch: for each transmitter index, every deck that has it is begun
+// with that transmitter and the batch steps them together until all have sampled.
+int run_model_batch(std::vector<Deck> &decks, int max_steps, bool tune, bool quiet) {
+  int most = 0;
+  for (const Deck &d : decks) most = std::max(most, d.ntx);
+  bool tuned = !tune;
+  for (int t = 0; t < most; ++t) {
+    std::vector<tcsem_ctx *> members;
+    std::vector<Deck *> ds;
+    for (Deck &d : decks)
+      if (t < d.ntx) {
+        int rc = tcsem_begin(d.ctx, t, 1);
+        if (rc) return die(rc, "begin");
+        members.push_back(d.ctx);
+        ds.push_back(&d);
+      }
+    tcsem_batch *b = nullptr;
+    int rc = tcsem_batch_create(members.data(), (int)members.size(), &b);
+    if (rc) return die(rc, "model batch");
+    if (!tuned) {
+      int rows = 0, pf = 0;
+      double ms = 0.0;
+      if ((rc = tcsem_batch_autotune(b, 10, &rows, &pf, &ms))) {
+        tcsem_batch_destroy(b);
+        return die(rc, "autotune");
+      }
+      if (!quiet) std::printf("autotuned model batch: %d rows per tile, prefetch %d: %.4f ms per step\n", rows, pf, ms);
+      tuned = true;
+    }
+    int steps = 0;
+    if ((rc = tcsem_batch_run(b, max_steps, &steps))) {
+      tcsem_batch_destroy(b);
+      return die(rc, "time stepping");
+    }
+    tcsem_batch_destroy(b);
+    for (Deck *d : ds) {
+      const tcsem_tx &tx = d->txs[t];
+      std::vector<double> traces(std::max<size_t>((size_t)tx.nr * d->ncomp * d->rt.Nt, 1));
+      if ((rc = tcsem_get_traces(d->ctx, 0, traces.data()))) return die(rc, "reading traces");
+      if ((rc = write_traces(*d, tx, traces.data(), steps, quiet))) return rc;
+    }
+  }
+  return 0;
 }
 
 }  // namespace
 
 int main(int argc, char **argv) {
-  std::string dir = ".";
+  std::vector<std::string> dirs;
   int precision = TCSEM_FP64, batch = 8, max_steps = 0, graph = 0;
   bool quiet = false, tune = false;
   for (int a = 1; a < argc; ++a) {
@@ -56,7 +165,7 @@ int main(int argc, char **argv) {
       }
       return argv[++a];
     };
-    if (s == "--dir") dir = next("--dir");
+    if (s == "--dir") dirs.push_back(next("--dir"));
     else if (s == "--precision") precision = std::atoi(next("--precision"));
     else if (s == "--batch") batch = std::atoi(next("--batch"));
     else if (s == "--max-steps") max_steps = std::atoi(next("--max-steps"));
@@ -75,55 +184,38 @@ int main(int argc, char **argv) {
     usage();
     return 1;
   }
-  auto path = [&](const char *f) { return dir + "/" + f; };
-
-  tcsem_setup setup;
-  int rc = tcsem_read_setup(path("FDTD_setup.in").c_str(), &setup);
-  if (rc) return die(rc, "reading FDTD_setup.in");
-  tcsem_grid_info gi;
-  if ((rc = tcsem_grid_info_of(&setup, &gi))) return die(rc, "grid setup");
-  const size_t ncell = (size_t)gi.L * gi.M * gi.N;
-  std::vector<double> cond(ncell);
-  if ((rc = tcsem_read_cond(path("Cond.bin").c_str(), ncell, cond.data()))) return die(rc, "reading Cond.bin");
-
-  tcsem_rxtime rt;
-  int nrx = 0;
-  if ((rc = tcsem_read_receivers(path("Receiver.in").c_str(), &rt, &nrx, nullptr, 0))) return die(rc, "reading Receiver.in");
-  std::vector<double> rxyz(3 * (size_t)nrx);
-  if ((rc = tcsem_read_receivers(path("Receiver.in").c_str(), &rt, &nrx, rxyz.data(), nrx)))
-    return die(rc, "reading Receiver.in");
-
-  int ntx = 0, nids = 0;
-  if ((rc = tcsem_read_transmitters(path("Transmitter.in").c_str(), nullptr, 0, &ntx, nullptr, 0, &nids)))
-    return die(rc, "reading Transmitter.in");
-  std::vector<tcsem_tx> txs(ntx);
-  std::vector<int> ids(nids > 0 ? nids : 1);
-  if ((rc = tcsem_read_transmitters(path("Transmitter.in").c_str(), txs.data(), ntx, &ntx, ids.data(), nids, &nids)))
-    return die(rc, "reading Transmitter.in");
-  int ncomp = 0;
-  for (int c = 0; c < 6; ++c) ncomp += rt.is_comp[c] ? 1 : 0;
-
-  if (!quiet)
-    std::printf("%s\nmodel %d x %d x %d cells, air plane %d x %d, %d receivers, %d transmitters, FP%d\n",
-                tcsem_version(), gi.L, gi.M, gi.N, gi.L1, gi.M1, nrx, ntx, precision);
-
-  tcsem_ctx *ctx = nullptr;
-  if ((rc = tcsem_create(&setup, cond.data(), precision, &ctx))) return die(rc, "device setup");
-  cond.clear();
-  cond.shrink_to_fit();
-  if ((rc = tcsem_set_receivers(ctx, &rt, nrx, rxyz.data()))) {
-    tcsem_destroy(ctx);
-    return die(rc, "receiver setup");
+  if (dirs.empty()) dirs.push_back(".");
+  std::vector<Deck> decks(dirs.size());
+  for (size_t q = 0; q < dirs.size(); ++q) {
+    decks[q].dir = dirs[q];
+    if (int rc = load_deck(decks[q])) return rc;
+    const Deck &d = decks[q];
+    if (!quiet)
+      std::printf("%s: model %d x %d x %d cells, air plane %d x %d, %d receivers, %d transmitters, FP%d\n",
+                  d.dir.c_str(), d.gi.L, d.gi.M, d.gi.N, d.gi.L1, d.gi.M1, d.nrx, d.ntx, precision);
   }
-  if ((rc = tcsem_set_transmitters(ctx, txs.data(), ntx))) {
-    tcsem_destroy(ctx);
-    return die(rc, "transmitter setup");
+  if (!quiet) std::printf("%s\n", tcsem_version());
+  auto cleanup = [&]() {
+    for (Deck &d : decks) tcsem_destroy(d.ctx);
+  };
+  for (Deck &d : decks)
+    if (int rc = setup_context(d, precision)) {
+      cleanup();
+      return rc;
+    }
+  if (decks.size() > 1) {
+    const int rc = run_model_batch(decks, max_steps, tune, quiet);
+    cleanup();
+    return rc;
   }
-  if (tune && ntx > 0) {
+  Deck &d = decks[0];
+  tcsem_ctx *ctx = d.ctx;
+  int rc = 0;
+  if (tune && d.ntx > 0) {
     int rows = 0, pf = 0, zc = 0;
     double ms = 0.0;
-    if ((rc = tcsem_autotune(ctx, std::min(batch, ntx), 20, &rows, &pf, &zc, nullptr, &ms))) {
-      tcsem_destroy(ctx);
+    if ((rc = tcsem_autotune(ctx, std::min(batch, d.ntx), 20, &rows, &pf, &zc, nullptr, &ms))) {
+      cleanup();
       return die(rc, "autotune");
     }
     if (!quiet)
@@ -131,32 +223,29 @@ int main(int argc, char **argv) {
                   ms);
   }
   if (graph && (rc = tcsem_set_graph(ctx, graph))) {
-    tcsem_destroy(ctx);
+    cleanup();
     return die(rc, "graph setup");
   }
-  for (int first = 0; first < ntx; first += batch) {
-    const int nb = std::min(batch, ntx - first);
+  for (int first = 0; first < d.ntx; first += batch) {
+    const int nb = std::min(batch, d.ntx - first);
     size_t total = 0;
-    for (int b = 0; b < nb; ++b) total += (size_t)txs[first + b].nr * ncomp * rt.Nt;
+    for (int b = 0; b < nb; ++b) total += (size_t)d.txs[first + b].nr * d.ncomp * d.rt.Nt;
     std::vector<double> traces(total > 0 ? total : 1);
     int steps = 0;
     if ((rc = tcsem_run(ctx, first, nb, max_steps, traces.data(), &steps))) {
-      tcsem_destroy(ctx);
+      cleanup();
       return die(rc, "time stepping");
     }
     size_t off = 0;
     for (int b = 0; b < nb; ++b) {
-      const tcsem_tx &t = txs[first + b];
-      char name[64];
-      std::snprintf(name, sizeof name, "Tx%d.bout", t.id);
-      if ((rc = tcsem_write_bout(path(name).c_str(), t.nr, ncomp, rt.Nt, traces.data() + off))) {
-        tcsem_destroy(ctx);
-        return die(rc, "writing traces");
+      const tcsem_tx &t = d.txs[first + b];
+      if ((rc = write_traces(d, t, traces.data() + off, steps, quiet))) {
+        cleanup();
+        return rc;
       }
-      off += (size_t)t.nr * ncomp * rt.Nt;
-      if (!quiet) std::printf("Tx %d: %d steps, %d receivers -> %s\n", t.id, steps, t.nr, name);
+      off += (size_t)t.nr * d.ncomp * d.rt.Nt;
     }
   }
-  tcsem_destroy(ctx);
+  cleanup();
   return 0;
 }

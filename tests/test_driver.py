@@ -118,3 +118,55 @@ def test_driver_autotune_keeps_results():
     for got, ref in zip(outs[1], outs[0]):
         err = decks.trace_error(got, ref)
         assert np.all(err < 1e-10), err
+
+
+def _deck_of(kw, ntx, seed):
+    so = decks.small_setup(O, **kw)
+    m = O.Model(so)
+    g = m.grid()
+    cond = decks.layered_cond((m.N, m.M, m.L), g["Z"][:, None, None] * np.ones((1, m.M, m.L)), seed=seed,
+                              blob=True)
+    rx = np.array([[150.0, 0.0, 400.0], [300.0, 60.0, 400.0], [-200.0, -50.0, 350.0], [400.0, 0.0, 300.0]])
+    txs = [dict(ort=1, bipolen=100.0, x=0.0, y=0.0, z=300.0, rx_ids=[1, 2, 3, 4]),
+           dict(ort=2, bipolen=80.0, x=50.0, y=-25.0, z=320.0, rx_ids=[3, 1])][:ntx]
+    return W.Deck("deck%d" % seed, decks.small_setup(T, **kw).as_dict(), cond,
+                  dict(T0=0.02, Tn=0.25, Nt=16, is_log=1, comps=(1, 1, 1, 1, 1, 1)), rx, txs)
+
+
+def test_driver_model_batch_missing_input_is_an_io_error():
+    with tempfile.TemporaryDirectory() as d1, tempfile.TemporaryDirectory() as d2:
+        _small_deck().write(d1)
+        r = _run(["--dir", d1, "--dir", d2])
+    assert r.returncode == 3 and "FDTD_setup.in" in r.stderr
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision,tol", [(64, 1e-10), (32, 1e-5)])
+def test_driver_model_batch_equals_separate_runs(precision, tol):
+    """Three decks of different grids (2, 1 and 2 transmitters) as one model batch: every
+    Tx<id>.bout equals the one the deck's own run writes."""
+    dks = [_deck_of({}, 2, 3), _deck_of({"nx": 23, "ny": 17}, 1, 4), _deck_of({"nx": 30, "ny": 26, "nz": 10}, 2, 5)]
+    with tempfile.TemporaryDirectory() as root:
+        dirs = []
+        for q, dk in enumerate(dks):
+            for mode in ("batch", "single"):
+                p = os.path.join(root, "%s%d" % (mode, q))
+                os.makedirs(p)
+                dk.write(p)
+            dirs.append(q)
+        args = []
+        for q in dirs:
+            args += ["--dir", os.path.join(root, "batch%d" % q)]
+        print("driver model batch FP%d" % precision, flush=True)
+        r = _run(args + ["--precision", str(precision)])
+        assert r.returncode == 0, r.stderr
+        for q, dk in enumerate(dks):
+            r = _run(["--dir", os.path.join(root, "single%d" % q), "--precision", str(precision), "--quiet"])
+            assert r.returncode == 0, r.stderr
+            for b in range(len(dk.txs)):
+                name = "Tx%d.bout" % (b + 1)
+                got = T.read_bout(os.path.join(root, "batch%d" % q, name))
+                ref = T.read_bout(os.path.join(root, "single%d" % q, name))
+                assert got.shape == ref.shape and np.any(ref != 0)
+                err = decks.trace_error(got, ref)
+                assert np.all(err < tol), (q, b, err)
