@@ -240,3 +240,34 @@ def test_batch_equals_independent_runs_fp64():
     for b in range(3):
         err = decks.trace_error(got[b], ref[b])
         assert np.all(err < 1e-9), (b, err)
+
+
+@pytest.mark.parametrize("zsum", ["fused", "pass"])
+def test_batch_with_z_chunks_equals_independent_runs_fp64(zsum, monkeypatch):
+    """Three transmitter slots with a z-chunked march: the chunk sums of B are planes per slot."""
+    monkeypatch.setenv("TCSEM_MARCH_ZCHUNKS", "3")
+    monkeypatch.setenv("TCSEM_MARCH_ZSUM", zsum)
+    txs = [dict(TX_X, x=x, txid=i + 1) for i, x in enumerate((-100.0, 0.0, 120.0))]
+    ref, got = _deck_run(T.FP64, txs, nsteps=300)
+    for b in range(3):
+        err = decks.trace_error(got[b], ref[b])
+        assert np.all(err < 1e-9), (b, err)
+
+
+def test_batch_graph_replays_equal_plain_steps():
+    """Graph replays with three transmitter slots: the same traces as the plain steps."""
+    txs = [dict(TX_X, x=x, txid=i + 1) for i, x in enumerate((-100.0, 0.0, 120.0))]
+    so, st, m, cond = _pair()
+    rt = T.make_rxtime(0.02, 0.25, 16, 1, (1, 1, 1, 1, 1, 1))
+    rx = np.array([[150.0, 0.0, 400.0], [300.0, 60.0, 400.0], [-200.0, -50.0, 350.0], [400.0, 0.0, 300.0]])
+    out = []
+    for k in (0, 8):
+        with T.Context(st, cond, T.FP32) as c:
+            c.set_receivers(rt, rx)
+            c.set_transmitters([T.make_tx(**kw) for kw in txs])
+            c.set_graph(k)
+            out.append(c.run(0, 3))
+            if k:
+                assert c.graph_replays() > 0
+    for b in range(3):
+        assert np.array_equal(out[0][b], out[1][b]), b
