@@ -271,3 +271,14 @@ def test_batch_graph_replays_equal_plain_steps():
                 assert c.graph_replays() > 0
     for b in range(3):
         assert np.array_equal(out[0][b], out[1][b]), b
+
+
+def test_upcont_plane_long_x_fp32():
+    """The line chain in an FP32 context (FP64 arithmetic, FP32 planes out)."""
+    so, st, m, cond = _pair({"nx": 1100, "ny": 8, "pad": (2, 3), "arg": (1.5, 2.0)})
+    assert m.L1 == 1156
+    bz0 = np.random.default_rng(9).standard_normal((m.M, m.L))
+    bxo, byo = m.upcont_plane(bz0)
+    with T.Context(st, cond, T.FP32) as c:
+        bx, by = c.upcont_plane(bz0)
+    assert decks.rel_l2(bx, bxo) < 2e-6 and decks.rel_l2(by, byo) < 2e-6
