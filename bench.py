@@ -164,7 +164,7 @@ def roofline(res, esz, nbatch, traffic_entry):
             "traffic_tiling": traffic_tiling,
             "kernel": "k_march", "algorithmic_bytes_per_launch": int(algo_bytes),
             "avg_launch_ms": round(ms, 5),
-            "span": "per step: the lower and the top launch of k_march (a k_zsum pass, when the tiling runs one, is a kernel of its own)"}
+            "span": "per step: the lower and the top launch of k_march, the lower one from the start of its k_zsum pass when the tiling runs one"}
 
 
 def _fft_len(n):

@@ -191,7 +191,8 @@ int tcsem_upcont_plane(tcsem_ctx *ctx, const double *bz0, double *bxair, double 
 /* Average device time [ms] of the kernels of the steps taken while timing was enabled, from HIP
  * events that the kernels' dispatch packets record on the context's streams as the steps run
  * (every 8th step; no extra synchronisation, reading waits for them): t[0] volume march per step
- * (lower and top launch), t[1] surface level, t[2] upward continuation chain, t[3] receiver
+ * (lower and top launch; the lower from the start of its k_zsum pass when the step runs one),
+ * t[1] surface level, t[2] upward continuation chain, t[3] receiver
  * gather per launch; n[i] counts. */
 int tcsem_timing(const tcsem_ctx *ctx, double t[4], long n[4]);
 int tcsem_enable_timing(tcsem_ctx *ctx, int on);

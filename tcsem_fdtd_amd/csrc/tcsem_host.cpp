@@ -1117,7 +1117,7 @@ hipError_t enqueue_march(tcsem_ctx *c, const MarchArgs<T> &a, int nb, hipStream_
     lo.k_lo = 2;
     lo.carry = 2;
     const bool chunked = c->nchunk > 1;
-    bool fused = false;
+    bool fused = false, pass = false;
     if (chunked) {
       lo.nchunk = c->nchunk;
       lo.chunk_len = c->chunk_len;
@@ -1130,7 +1130,9 @@ hipError_t enqueue_march(tcsem_ctx *c, const MarchArgs<T> &a, int nb, hipStream_
         lo.ibw = ib_planes(c, c->ib_par ^ 1);
         lo.ibr = fused ? ib_planes(c, c->ib_par) : nullptr;
       }
-      if (!fused && (e = launch_zsum<T>(lo, nb, st, nullptr)) != hipSuccess) return e;
+      // a k_zsum pass belongs to the march: a timed step's march interval starts at its dispatch
+      pass = !fused;
+      if (pass && (e = launch_zsum<T>(lo, nb, st, te[0])) != hipSuccess) return e;
       c->ib_par ^= 1;
       c->ib_valid = c->fuse != 0;
       c->ib_nchunk = c->nchunk;
@@ -1138,7 +1140,7 @@ hipError_t enqueue_march(tcsem_ctx *c, const MarchArgs<T> &a, int nb, hipStream_
     } else {
       c->ib_valid = false;
     }
-    if ((e = launch_march<T>(lo, nb, st, te[0], te[1])) != hipSuccess) return e;
+    if ((e = launch_march<T>(lo, nb, st, pass ? nullptr : te[0], te[1])) != hipSuccess) return e;
   }
   if (wait && (e = hipStreamWaitEvent(st, wait, 0)) != hipSuccess) return e;
   MarchArgs<T> hi = a;

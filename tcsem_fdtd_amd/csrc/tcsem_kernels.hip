@@ -147,16 +147,17 @@ __device__ __forceinline__ void sto(T *__restrict__ base, unsigned off, T v) {
 // store (the host keeps every slot's field below 2 GiB).
 constexpr unsigned kDrop = 0x80000000u;
 constexpr int kRsrcWord3 = 0x00020000;  // gfx9-family raw buffer: 32-bit data format
-// Cache policy of the march's field stores: non-temporal (the NT bit).  The new state is read
-// only in the next step, long after it has left L2, so streaming its stores keeps L2 for the
-// current state's halo rows and columns, which neighbouring tiles read again.  Measured against
-// policy 0: 3-6% per step on every workload; NT loads instead lose 10-18% (they evict the halos).
-constexpr int kStoreNT = 2;
+// Non-temporal cache policy (the NT bit) for the march's field stores and its single-use loads.
+// The new state is read only in the next step, long after it has left L2, so streaming its stores
+// keeps L2 for the current state's halo rows and columns, which neighbouring tiles read again.
+// Measured against policy 0: 3-6% per step on every workload.  NT on every load instead loses
+// 10-18% (it evicts the halos); only the old Ex, Ey, which no other tile reads, are loaded NT.
+constexpr int kNT = 2;
 __device__ inline __amdgpu_buffer_rsrc_t rsrc_of(void *base, unsigned bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)bytes, kRsrcWord3);
 }
 __device__ __forceinline__ void bsto(__amdgpu_buffer_rsrc_t r, unsigned off, bool ok, float v) {
-  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)(ok ? off : kDrop), 0, kStoreNT);
+  __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(v), r, (int)(ok ? off : kDrop), 0, kNT);
 }
 __device__ __forceinline__ void bsto(__amdgpu_buffer_rsrc_t r, unsigned off, bool ok, double v) {
   typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
@@ -164,7 +165,21 @@ __device__ __forceinline__ void bsto(__amdgpu_buffer_rsrc_t r, unsigned off, boo
   u32x2 w;
   w.x = (unsigned)b;
   w.y = (unsigned)(b >> 32);
-  __builtin_amdgcn_raw_buffer_store_b64(w, r, (int)(ok ? off : kDrop), 0, kStoreNT);
+  __builtin_amdgcn_raw_buffer_store_b64(w, r, (int)(ok ? off : kDrop), 0, kNT);
+}
+
+// Loads through a raw buffer resource: an offset past the resource's size (kDrop) returns 0
+// without a memory access, so a lane or wave that does not need a value drops its load without a
+// branch.  AUX is the cache policy (kNT for values read once per step).
+template <int AUX>
+__device__ __forceinline__ float bld(__amdgpu_buffer_rsrc_t r, unsigned off, float) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, AUX));
+}
+template <int AUX>
+__device__ __forceinline__ double bld(__amdgpu_buffer_rsrc_t r, unsigned off, double) {
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, AUX);
+  return __longlong_as_double((long long)(((unsigned long long)w.y << 32) | w.x));
 }
 
 // Per-lane geometry shared by the march and the surface kernel.  Every lane computes every
@@ -311,6 +326,11 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> la
   const unsigned fbytes = (unsigned)(a.fstride * (long long)sizeof(T));
   const __amdgpu_buffer_rsrc_t rex = rsrc_of(ex, fbytes), rey = rsrc_of(ey, fbytes), rez = rsrc_of(ez, fbytes);
   const __amdgpu_buffer_rsrc_t rbx = rsrc_of(bx, fbytes), rby = rsrc_of(by, fbytes);
+  // The old Ex, Ey are used only by the tile's own rows (E updates its own cells; the halo rows
+  // compute Bz sums and Ez), so the halo-row waves drop those loads, and each remaining load is
+  // the value's only read in the step: non-temporal, like the stores.
+  const __amdgpu_buffer_rsrc_t rexi = rsrc_of((void *)exi, fbytes), reyi = rsrc_of((void *)eyi, fbytes);
+  const bool erow = ly >= 1 && ly <= BY - 2;
   const auto cf = make_coef<T>(a.lam, a.dt);
   const T dtb = (T)a.dtb, zero = 0;
 
@@ -338,8 +358,8 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> la
     v.bx = ldo(bxi, o);
     v.by = ldo(byi, o);
     v.s = ldo(sig, o);
-    v.ex = ldo(exi, o);
-    v.ey = ldo(eyi, o);
+    v.ex = bld<kNT>(rexi, erow ? o : kDrop, T(0));
+    v.ey = bld<kNT>(reyi, erow ? o : kDrop, T(0));
     v.ez = ldo(ezi, o);
     v.hbx = ldo(bxi, h);
     v.hby = ldo(byi, h);
