@@ -559,7 +559,7 @@ void choose_tiling(tcsem_ctx *c, int nbatch) {
   c->fuse = fuse;
   if (const char *e = std::getenv("TCSEM_MARCH_ROWS")) {
     const int v = std::atoi(e);
-    if (v == 4 || v == 8 || v == 16) by = v;
+    if (v == 4 || v == 8 || v == 10 || v == 16) by = v;
   }
   if (const char *e = std::getenv("TCSEM_MARCH_PF")) {
     const int v = std::atoi(e);
@@ -1484,7 +1484,7 @@ template <typename T>
 int autotune_march(tcsem_ctx *c, int nbatch, int nsteps, int *best_by, int *best_pf, int *best_nch, int *best_fuse,
                    double *best_ms) {
   const int N = c->g.info.N, nlev = N - 2;
-  std::vector<int> rows_c = c->prec == 64 ? std::vector<int>{4, 8} : std::vector<int>{4, 8, 16};
+  std::vector<int> rows_c = c->prec == 64 ? std::vector<int>{4, 8} : std::vector<int>{4, 8, 10, 16};
   std::vector<int> nch_c = {1};
   if (N >= 6)
     for (int z : {2, 3, 4, 6, 8})
@@ -1641,7 +1641,7 @@ void batch_tiling(const tcsem_batch *b, int &by, int &pf) {
   }
   if (const char *e = std::getenv("TCSEM_MARCH_ROWS")) {
     const int v = std::atoi(e);
-    if (v == 4 || v == 8 || v == 16) by = v;
+    if (v == 4 || v == 8 || v == 10 || v == 16) by = v;
   }
   if (const char *e = std::getenv("TCSEM_MARCH_PF")) {
     const int v = std::atoi(e);
@@ -1932,7 +1932,7 @@ int batch_autotune_t(tcsem_batch *b, int nsteps, int *best_by, int *best_pf, dou
     (void)hipEventDestroy(e0);
     return fail(TCSEM_EDEVICE, "event creation failed");
   }
-  std::vector<int> rows_c = b->prec == 64 ? std::vector<int>{4, 8} : std::vector<int>{4, 8, 16};
+  std::vector<int> rows_c = b->prec == 64 ? std::vector<int>{4, 8} : std::vector<int>{4, 8, 10, 16};
   double best = 1e300;
   int rc = TCSEM_OK;
   for (int by : rows_c) {

@@ -254,8 +254,13 @@ __device__ inline MarchArgs<T> slot_args(const MarchArgs<T> &la, const MarchArgs
 }
 
 // IB: the chunked lower launch that accumulates the chunk sums of B (a.ibw set; a.ibr may be).
+// 10-row tiles (FP32) are compiled for five waves per SIMD: at most 96 VGPRs, with a few values
+// spilled to scratch, so that two 10-wave workgroups share a CU (at the compiler's 108 VGPRs only
+// one fits).  Their halo rows are 20% of the waves, against 25% for 8-row tiles.  Other shapes
+// keep the compiler's register choice.
 template <typename T, int BY, int PF, bool GEO, bool IB>
-__global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> la, const MarchArgs<T> *geo) {
+__global__ void __launch_bounds__(kMarchLanes *BY) __attribute__((amdgpu_waves_per_eu(BY == 10 ? 5 : 1)))
+k_march(const MarchArgs<T> la, const MarchArgs<T> *geo) {
   MarchArgs<T> a;
   int tile;
   if constexpr (GEO) {
@@ -328,7 +333,8 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> la
   const __amdgpu_buffer_rsrc_t rbx = rsrc_of(bx, fbytes), rby = rsrc_of(by, fbytes);
   // The old Ex, Ey are used only by the tile's own rows (E updates its own cells; the halo rows
   // compute Bz sums and Ez), so the halo-row waves drop those loads, and each remaining load is
-  // the value's only read in the step: non-temporal, like the stores.
+  // the value's only read in the step: non-temporal, like the stores.  (Not in the chunk-sum
+  // variant: the select costs it a VGPR, and at 8 rows and prefetch 1 a wave per SIMD.)
   const __amdgpu_buffer_rsrc_t rexi = rsrc_of((void *)exi, fbytes), reyi = rsrc_of((void *)eyi, fbytes);
   const bool erow = ly >= 1 && ly <= BY - 2;
   const auto cf = make_coef<T>(a.lam, a.dt);
@@ -358,8 +364,13 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march(const MarchArgs<T> la
     v.bx = ldo(bxi, o);
     v.by = ldo(byi, o);
     v.s = ldo(sig, o);
-    v.ex = bld<kNT>(rexi, erow ? o : kDrop, T(0));
-    v.ey = bld<kNT>(reyi, erow ? o : kDrop, T(0));
+    if constexpr (IB) {
+      v.ex = ldo(exi, o);
+      v.ey = ldo(eyi, o);
+    } else {
+      v.ex = bld<kNT>(rexi, erow ? o : kDrop, T(0));
+      v.ey = bld<kNT>(reyi, erow ? o : kDrop, T(0));
+    }
     v.ez = ldo(ezi, o);
     v.hbx = ldo(bxi, h);
     v.hby = ldo(byi, h);
@@ -831,6 +842,9 @@ hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEven
   switch (a.rows) {
     case 4: return launch_march_by<T, 4>(a, geo, nslot, s, e0, e1);
     case 8: return launch_march_by<T, 8>(a, geo, nslot, s, e0, e1);
+    case 10:  // FP32 only (FP64 tiles are capped at 8 rows)
+      if constexpr (sizeof(T) == 4) return launch_march_by<T, 10>(a, geo, nslot, s, e0, e1);
+      break;
     case 16: return launch_march_by<T, 16>(a, geo, nslot, s, e0, e1);
   }
   return hipErrorInvalidValue;
