@@ -155,7 +155,7 @@ int tcsem_reset_fields(tcsem_ctx *ctx, int first, int n);
 /* Autotuner of the volume march (no counterpart in the reference, whose launch shapes are
  * fixed): times nsteps nbatch-transmitter steps as they run in production (the march beside the
  * upward continuation and surface level on the second stream; no source, no gathers) for every
- * tiling the library runs -- rows per workgroup 4/8/10/16 (FP64: 4/8), prefetch depth 1-4, z-chunks
+ * tiling the library runs -- rows per workgroup 4/8/10/16 (FP64: 4/8/12), prefetch depth 1-4, z-chunks
  * of the lower launch 1/2/3/4/6/8 as the depth allows, and for chunked tilings the source of the
  * chunks' Bz partial sums (a pass over Bx, By, or the chunk sums the previous step's march
  * accumulated) -- and keeps the fastest for later batches of that size (TCSEM_MARCH_ROWS / _PF /

@@ -508,7 +508,9 @@ int device_cus() {
 // nch z-chunks of the lower launch (1: one pass).
 void apply_tiling(tcsem_ctx *c, int by, int pf, int nch) {
   const int L = c->g.info.L, M = c->g.info.M, N = c->g.info.N, nlev = N - 2;
-  if (c->prec == 64 && by > 8) by = 8;  // FP64 16-row tiles do not fit their registers
+  // rows per tile: FP64 4, 8 or 12 (16 do not fit its registers), FP32 4, 8, 10 or 16
+  if (c->prec == 64 && by != 4 && by != 12) by = 8;
+  if (c->prec == 32 && by == 12) by = 8;
   c->by = by;
   c->pf = pf;
   nch = std::max(1, std::min({nch, kMaxChunks, N >= 6 ? nlev : 1}));
@@ -559,7 +561,7 @@ void choose_tiling(tcsem_ctx *c, int nbatch) {
   c->fuse = fuse;
   if (const char *e = std::getenv("TCSEM_MARCH_ROWS")) {
     const int v = std::atoi(e);
-    if (v == 4 || v == 8 || v == 10 || v == 16) by = v;
+    if (v == 4 || v == 8 || v == 10 || v == 12 || v == 16) by = v;
   }
   if (const char *e = std::getenv("TCSEM_MARCH_PF")) {
     const int v = std::atoi(e);
@@ -1484,7 +1486,7 @@ template <typename T>
 int autotune_march(tcsem_ctx *c, int nbatch, int nsteps, int *best_by, int *best_pf, int *best_nch, int *best_fuse,
                    double *best_ms) {
   const int N = c->g.info.N, nlev = N - 2;
-  std::vector<int> rows_c = c->prec == 64 ? std::vector<int>{4, 8} : std::vector<int>{4, 8, 10, 16};
+  std::vector<int> rows_c = c->prec == 64 ? std::vector<int>{4, 8, 12} : std::vector<int>{4, 8, 10, 16};
   std::vector<int> nch_c = {1};
   if (N >= 6)
     for (int z : {2, 3, 4, 6, 8})
@@ -1641,13 +1643,12 @@ void batch_tiling(const tcsem_batch *b, int &by, int &pf) {
   }
   if (const char *e = std::getenv("TCSEM_MARCH_ROWS")) {
     const int v = std::atoi(e);
-    if (v == 4 || v == 8 || v == 10 || v == 16) by = v;
+    if (v == 4 || v == 8 || v == 10 || v == 12 || v == 16) by = v;
   }
   if (const char *e = std::getenv("TCSEM_MARCH_PF")) {
     const int v = std::atoi(e);
     if (v >= 1 && v <= 4) pf = v;
   }
-  if (b->prec == 64 && by > 8) by = 8;
 }
 
 // Builds and uploads the member records for tiling (by, pf) and returns the launch records: the
@@ -1663,7 +1664,7 @@ int batch_records(tcsem_batch *b, int by, int pf, MarchArgs<T> &la, UpcontArgs<T
   for (int q = 0; q < nm; ++q) {
     tcsem_ctx *c = b->m[q];
     apply_tiling(c, by, pf, 1);
-    by = c->by;  // FP64 tiles clamped to 8 rows
+    by = c->by;  // the row counts of the member's precision (apply_tiling)
     const int cur0 = c->cur;
     for (int p = 0; p < 2; ++p) {
       c->cur = cur0 ^ p;
@@ -1932,7 +1933,7 @@ int batch_autotune_t(tcsem_batch *b, int nsteps, int *best_by, int *best_pf, dou
     (void)hipEventDestroy(e0);
     return fail(TCSEM_EDEVICE, "event creation failed");
   }
-  std::vector<int> rows_c = b->prec == 64 ? std::vector<int>{4, 8} : std::vector<int>{4, 8, 10, 16};
+  std::vector<int> rows_c = b->prec == 64 ? std::vector<int>{4, 8, 12} : std::vector<int>{4, 8, 10, 16};
   double best = 1e300;
   int rc = TCSEM_OK;
   for (int by : rows_c) {

@@ -256,10 +256,13 @@ __device__ inline MarchArgs<T> slot_args(const MarchArgs<T> &la, const MarchArgs
 // IB: the chunked lower launch that accumulates the chunk sums of B (a.ibw set; a.ibr may be).
 // 10-row tiles (FP32) are compiled for five waves per SIMD: at most 96 VGPRs, with a few values
 // spilled to scratch, so that two 10-wave workgroups share a CU (at the compiler's 108 VGPRs only
-// one fits).  Their halo rows are 20% of the waves, against 25% for 8-row tiles.  Other shapes
-// keep the compiler's register choice.
+// one fits).  Their halo rows are 20% of the waves, against 25% for 8-row tiles.  12-row tiles
+// (FP64) are compiled for three waves per SIMD (168 VGPRs instead of 174-200): one 12-wave
+// workgroup per CU with 17% halo rows, against one 8-wave workgroup with 25%.  Other shapes keep
+// the compiler's register choice.
 template <typename T, int BY, int PF, bool GEO, bool IB>
-__global__ void __launch_bounds__(kMarchLanes *BY) __attribute__((amdgpu_waves_per_eu(BY == 10 ? 5 : 1)))
+__global__ void __launch_bounds__(kMarchLanes *BY)
+__attribute__((amdgpu_waves_per_eu(BY == 10 ? 5 : (BY == 12 ? 3 : 1))))
 k_march(const MarchArgs<T> la, const MarchArgs<T> *geo) {
   MarchArgs<T> a;
   int tile;
@@ -842,8 +845,11 @@ hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEven
   switch (a.rows) {
     case 4: return launch_march_by<T, 4>(a, geo, nslot, s, e0, e1);
     case 8: return launch_march_by<T, 8>(a, geo, nslot, s, e0, e1);
-    case 10:  // FP32 only (FP64 tiles are capped at 8 rows)
+    case 10:  // FP32 only
       if constexpr (sizeof(T) == 4) return launch_march_by<T, 10>(a, geo, nslot, s, e0, e1);
+      break;
+    case 12:  // FP64 only
+      if constexpr (sizeof(T) == 8) return launch_march_by<T, 12>(a, geo, nslot, s, e0, e1);
       break;
     case 16: return launch_march_by<T, 16>(a, geo, nslot, s, e0, e1);
   }
