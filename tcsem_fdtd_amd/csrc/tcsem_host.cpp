@@ -533,16 +533,15 @@ void apply_tiling(tcsem_ctx *c, int by, int pf, int nch) {
 }
 
 // Tiling of an nbatch-transmitter step: the autotuned one for this batch size when there is
-// one, else the heuristic (16-row tiles once the batch gives two workgroups per CU, else 4;
-// prefetch 2 for FP32 short tiles; z-chunks when the tiles alone leave CUs idle); the
-// TCSEM_MARCH_ROWS / _PF / _ZCHUNKS environment variables override either.
+// one, else the heuristic (10-row tiles for FP32 and 12-row tiles for FP64 at prefetch 1, the
+// autotuner's pick on every single model and transmitter batch measured; z-chunks when the tiles
+// alone leave CUs idle); the TCSEM_MARCH_ROWS / _PF / _ZCHUNKS environment variables override
+// either.
 void choose_tiling(tcsem_ctx *c, int nbatch) {
   const int L = c->g.info.L, M = c->g.info.M, N = c->g.info.N, nlev = N - 2;
   const int cu = device_cus();
   auto tiles = [&](int b) { return (long long)((L + kMarchLanes - 1) / kMarchLanes) * ((M + (b - 2) - 1) / (b - 2)); };
-  int by = tiles(16) * nbatch >= 2LL * cu ? 16 : 4;
-  if (c->prec == 64 && by > 8) by = 8;
-  int pf = (by == 16 || c->prec == 64) ? 1 : 2;
+  int by = c->prec == 64 ? 12 : 10, pf = 1;
   // enough chunks for two workgroups per CU, each chunk at least 8 levels deep
   int nch = 1;
   const long long wg = tiles(by) * nbatch;
@@ -1627,19 +1626,15 @@ namespace {
 
 int batch_fail_if(bool bad, const char *msg) { return bad ? fail(TCSEM_ESTATE, "%s", msg) : TCSEM_OK; }
 
-// Common tiling of a batch step: the autotuned one, else 16-row tiles once the batch's tiles give
-// two workgroups per CU, else 4 (FP64 at most 8); TCSEM_MARCH_ROWS / _PF override either.
+// Common tiling of a batch step: the autotuned one, else 10-row tiles (FP32) or 8-row tiles
+// (FP64) at prefetch 1; TCSEM_MARCH_ROWS / _PF override either.
 void batch_tiling(const tcsem_batch *b, int &by, int &pf) {
   if (b->by) {
     by = b->by;
     pf = b->pf;
-  } else {
-    long long tiles = 0;
-    for (const tcsem_ctx *c : b->m)
-      tiles += (long long)((c->g.info.L + kMarchLanes - 1) / kMarchLanes) * ((c->g.info.M + 13) / 14);
-    by = tiles >= 2LL * device_cus() ? 16 : 4;
-    if (b->prec == 64 && by > 8) by = 8;
-    pf = (by == 16 || b->prec == 64) ? 1 : 2;
+  } else {  // the batch autotuner's pick on the 64-model workload
+    by = b->prec == 64 ? 8 : 10;
+    pf = 1;
   }
   if (const char *e = std::getenv("TCSEM_MARCH_ROWS")) {
     const int v = std::atoi(e);
