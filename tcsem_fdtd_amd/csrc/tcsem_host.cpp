@@ -534,14 +534,19 @@ void apply_tiling(tcsem_ctx *c, int by, int pf, int nch) {
 
 // Tiling of an nbatch-transmitter step: the autotuned one for this batch size when there is
 // one, else the heuristic (10-row tiles for FP32 and 12-row tiles for FP64 at prefetch 1, the
-// autotuner's pick on every single model and transmitter batch measured; z-chunks when the tiles
-// alone leave CUs idle); the TCSEM_MARCH_ROWS / _PF / _ZCHUNKS environment variables override
-// either.
+// autotuner's pick on every single model and transmitter batch measured; 4-row tiles at prefetch
+// 2 for FP32 steps under 2^21 cell-updates, whose march is launch-latency bound; z-chunks when the
+// tiles alone leave CUs idle); the TCSEM_MARCH_ROWS / _PF / _ZCHUNKS environment variables
+// override either.
 void choose_tiling(tcsem_ctx *c, int nbatch) {
   const int L = c->g.info.L, M = c->g.info.M, N = c->g.info.N, nlev = N - 2;
   const int cu = device_cus();
   auto tiles = [&](int b) { return (long long)((L + kMarchLanes - 1) / kMarchLanes) * ((M + (b - 2) - 1) / (b - 2)); };
   int by = c->prec == 64 ? 12 : 10, pf = 1;
+  if (c->prec == 32 && (long long)L * M * N * nbatch < (1LL << 21)) {
+    by = 4;
+    pf = 2;
+  }
   // enough chunks for two workgroups per CU, each chunk at least 8 levels deep
   int nch = 1;
   const long long wg = tiles(by) * nbatch;
