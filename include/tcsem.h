@@ -178,8 +178,14 @@ int tcsem_steps_remaining(const tcsem_ctx *ctx);
  * max_steps > 0 caps the loop.  *steps_done (nullable) receives the steps taken. */
 int tcsem_run(tcsem_ctx *ctx, int tx_first, int ntx, int max_steps, double *traces,
               int *steps_done);
-/* Traces of batch slot b after tcsem_begin/tcsem_step (nr x NComp x Nt FP64). */
+/* Traces of batch slot b after tcsem_begin/tcsem_step (nr x NComp x Nt FP64, nr of transmitter
+ * tx_first+slot of the begun run).  The copy into efs of main.cu:278 (d_efs of xyzTx[itx].nr
+ * receivers, main.cu:207). */
 int tcsem_get_traces(tcsem_ctx *ctx, int slot, double *traces);
+/* Shape of tcsem_get_traces' output for batch slot `slot` of the begun run: receivers of that
+ * slot's transmitter, recorded components, samples (the reference sizes its efs from the same
+ * numbers, main.cu:207-210).  Callers size their buffer from it. */
+int tcsem_trace_shape(const tcsem_ctx *ctx, int slot, int *nr, int *ncomp, int *nt);
 
 /* Field access for batch slot `slot`; FP32 contexts convert.  Sizes as the component ids. */
 int tcsem_get_field(tcsem_ctx *ctx, int slot, int comp, double *out);

@@ -101,6 +101,8 @@ def lib():
         "orc_add_source": (None, [P, C.POINTER(TxPos), C.c_double]),
         "orc_step": (None, [P, C.c_double, C.c_double, C.c_double, C.POINTER(TxPos), C.c_double]),
         "orc_spline4": (C.c_double, [dp, dp, C.c_double]),
+        "orc_rx_stencil": (None, [P, C.c_double, C.c_double, C.c_double, C.c_int, ip, dp]),
+        "orc_rx_values": (None, [P, C.c_int, dp, C.c_int, ip, dp]),
         "orc_run": (C.c_int, [P, C.POINTER(RxTime), C.c_int, dp, C.POINTER(Tx), C.c_int, dp]),
         "orc_schedule": (C.c_int, [C.POINTER(Setup), C.c_double, C.POINTER(RxTime), C.c_int,
                                    dp, dp, dp, dp]),
@@ -282,6 +284,21 @@ class Model:
     def step(self, lam, dt1, wave, txpos, dtb):
         self.L_.orc_step(self.h, lam, dt1, wave, C.byref(txpos) if txpos is not None else None,
                          dtb)
+
+    def rx_stencil(self, x, y, z, comp):
+        """First node index per axis (x, y, z) and the 4 node coordinates per axis of the receiver
+        stencil of component comp (EFInterp2Rx.cu:75-88)."""
+        s, t = np.zeros(3, dtype=np.int32), np.zeros(12)
+        self.L_.orc_rx_stencil(self.h, float(x), float(y), float(z), int(comp), _ip(s), _dp(t))
+        return s, t.reshape(3, 4)
+
+    def rx_values(self, rx_xyz, comps):
+        """The current fields interpolated in space to the receivers: [nrx][len(comps)]."""
+        rx = np.ascontiguousarray(rx_xyz, dtype=np.float64).reshape(-1, 3)
+        cs = np.ascontiguousarray(comps, dtype=np.int32)
+        out = np.zeros((rx.shape[0], cs.size))
+        self.L_.orc_rx_values(self.h, rx.shape[0], _dp(rx), cs.size, _ip(cs), _dp(out))
+        return out
 
     def run(self, rxtime, rx_xyz, tx, max_steps=0):
         rx = np.ascontiguousarray(rx_xyz, dtype=np.float64).reshape(-1, 3)

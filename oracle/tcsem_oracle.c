@@ -1284,12 +1284,14 @@ typedef struct orc_rx {
   int *ix, *iy, *iz;
 } orc_rx;
 
-/* component c of receiver r: spatial interpolation from the current fields */
-static double orc_rx_value(const orc_model *m, const orc_rx *R, int r, int comp) {
-  const int L = m->L, M = m->M;
+/* Stencil of component comp for a receiver at (x, y, z) whose searchInsert indices are (ix, iy,
+ * iz) (EFInterp2Rx.cu:75-88 and the same choice in every *SpatialInterp kernel): the first node
+ * index s[0..2] of the 4 nodes per axis and the node coordinates t[0..3] (x), t[4..7] (y),
+ * t[8..11] (z).  Centre axes start at i-2; edge axes (nodes at the cell's lower face) use the
+ * half-cell test; the z start is clamped to level 0 (the reference reads level -1 there). */
+static void orc_stencil(const orc_model *m, double x, double y, double z, int ix, int iy, int iz, int comp,
+                        int *s, double *t) {
   const double *X = m->X, *Y = m->Y, *Z = m->Z, *dx = m->dx, *dy = m->dy, *dz = m->dz;
-  const double x = R->x[r], y = R->y[r], z = R->z[r];
-  const int ix = R->ix[r], iy = R->iy[r], iz = R->iz[r];
   /* centre (c) or edge (e) stencils per axis; edge axes use the half-cell test */
   int xe = 0, ye = 0, ze = 0;
   switch (comp) {
@@ -1306,12 +1308,30 @@ static double orc_rx_value(const orc_model *m, const orc_rx *R, int r, int comp)
   if (ze) s3 = (iz >= 1 && (Z[iz - 1] + 0.5 * dz[iz - 1]) > z) ? iz - 2 : iz - 1;
   else s3 = iz - 2;
   if (s3 < 0) s3 = 0;
-  double tx[4], ty[4], tz[4];
+  s[0] = s1; s[1] = s2; s[2] = s3;
   for (int q = 0; q < 4; q++) {
-    tx[q] = xe ? X[s1 + q] - 0.5 * dx[s1 + q] : X[s1 + q];
-    ty[q] = ye ? Y[s2 + q] - 0.5 * dy[s2 + q] : Y[s2 + q];
-    tz[q] = ze ? Z[s3 + q] - 0.5 * dz[s3 + q] : Z[s3 + q];
+    t[q] = xe ? X[s1 + q] - 0.5 * dx[s1 + q] : X[s1 + q];
+    t[4 + q] = ye ? Y[s2 + q] - 0.5 * dy[s2 + q] : Y[s2 + q];
+    t[8 + q] = ze ? Z[s3 + q] - 0.5 * dz[s3 + q] : Z[s3 + q];
   }
+}
+
+/* stencil of one receiver position, indices searched as LoadRxTxPos.cpp:93-121 does */
+void orc_rx_stencil(const orc_model *m, double x, double y, double z, int comp, int *s, double *t) {
+  const int ix = orc_search_insert(m->X, m->L, x), iy = orc_search_insert(m->Y, m->M, y);
+  const int iz = orc_search_insert(m->Z, m->N, z);
+  orc_stencil(m, x, y, z, ix, iy, iz, comp, s, t);
+}
+
+/* component c of receiver r: spatial interpolation from the current fields */
+static double orc_rx_value(const orc_model *m, const orc_rx *R, int r, int comp) {
+  const int L = m->L, M = m->M;
+  const double x = R->x[r], y = R->y[r], z = R->z[r];
+  int s[3];
+  double t[12];
+  orc_stencil(m, x, y, z, R->ix[r], R->iy[r], R->iz[r], comp, s, t);
+  const int s1 = s[0], s2 = s[1], s3 = s[2];
+  const double *tx = t, *ty = t + 4, *tz = t + 8;
   const double *F;
   int pitch, rows;
   switch (comp) {
@@ -1329,6 +1349,24 @@ static double orc_rx_value(const orc_model *m, const orc_rx *R, int r, int comp)
     Y2[ik] = spline4(ty, Y1, y);
   }
   return spline4(tz, Y2, z);
+}
+
+/* Spatial interpolation of the current fields to nrx receivers (x, y, z per receiver), for the
+ * ncomp component ids comps: out[r * ncomp + c] (the gathers of EFXInterp / BFXInterp,
+ * EFInterp2Rx.cu:411-470, without the time interpolation). */
+void orc_rx_values(const orc_model *m, int nrx, const double *rx_xyz, int ncomp, const int *comps,
+                   double *out) {
+  orc_rx R;
+  double x, y, z;
+  int ix, iy, iz;
+  R.nrx = 1; R.x = &x; R.y = &y; R.z = &z; R.ix = &ix; R.iy = &iy; R.iz = &iz;
+  for (int r = 0; r < nrx; r++) {
+    x = rx_xyz[3 * r]; y = rx_xyz[3 * r + 1]; z = rx_xyz[3 * r + 2];
+    ix = orc_search_insert(m->X, m->L, x);
+    iy = orc_search_insert(m->Y, m->M, y);
+    iz = orc_search_insert(m->Z, m->N, z);
+    for (int c = 0; c < ncomp; c++) out[(size_t)r * ncomp + c] = orc_rx_value(m, &R, 0, comps[c]);
+  }
 }
 
 /* ============================================================================================

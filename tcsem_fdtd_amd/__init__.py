@@ -105,6 +105,7 @@ def lib():
         "tcsem_steps_remaining": (C.c_int, [P]),
         "tcsem_run": (C.c_int, [P, C.c_int, C.c_int, C.c_int, dp, ip]),
         "tcsem_get_traces": (C.c_int, [P, C.c_int, dp]),
+        "tcsem_trace_shape": (C.c_int, [P, C.c_int, ip, ip, ip]),
         "tcsem_get_field": (C.c_int, [P, C.c_int, C.c_int, dp]),
         "tcsem_set_field": (C.c_int, [P, C.c_int, C.c_int, dp]),
         "tcsem_upcont_plane": (C.c_int, [P, dp, dp, dp]),
@@ -320,9 +321,16 @@ class Context:
     def step(self, n=1):
         _check(self.L_.tcsem_step(self.h, n))
 
+    def trace_shape(self, slot=0):
+        """(nr, NComp, Nt) of slot `slot` of the begun run, from the library."""
+        nr, nc, nt = C.c_int(), C.c_int(), C.c_int()
+        _check(self.L_.tcsem_trace_shape(self.h, slot, C.byref(nr), C.byref(nc), C.byref(nt)))
+        return nr.value, nc.value, nt.value
+
     def traces(self, slot=0):
-        nr = self.txs[slot].nr if self.txs else 0
-        out = np.zeros((nr, self.rt.ncomp, self.rt.Nt))
+        """Traces of batch slot `slot` of the begun run (its transmitter is tx_first + slot), in a
+        buffer sized by the library's own record of that transmitter's receivers."""
+        out = np.zeros(self.trace_shape(slot))
         _check(self.L_.tcsem_get_traces(self.h, slot, _dp(out)))
         return out
 

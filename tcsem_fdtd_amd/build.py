@@ -45,8 +45,6 @@ def build(verbose=True):
     tmp = LIB + ".tmp"
     _run([hipcc(), "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, LIB)
-    _run([hipcc()] + HIP_FLAGS + ["-x", "hip", os.path.join(CSRC, "hbm_copy.hip"), "-o",
-                                  os.path.join(HERE, "hbm_copy")])
     # the driver is host code over the C ABI: g++ only, linked against the library beside it
     _run(["g++", "-O2", "-std=c++17", "-Wall", os.path.join(CSRC, "tcsem_driver.cpp"), "-o", DRIVER,
           "-L" + HERE, "-ltcsem", "-Wl,-rpath,$ORIGIN"])

@@ -2807,6 +2807,16 @@ int tcsem_get_traces(tcsem_ctx *c, int slot, double *traces) {
   return TCSEM_OK;
 }
 
+int tcsem_trace_shape(const tcsem_ctx *c, int slot, int *nr, int *ncomp, int *nt) {
+  if (!c || !nr || !ncomp || !nt) return fail(TCSEM_EINVAL, "null argument");
+  if (!c->begun || !c->have_rx) return fail(TCSEM_ESTATE, "no run in progress");
+  if (slot < 0 || slot >= c->nbatch) return fail(TCSEM_EINVAL, "slot out of range");
+  *nr = c->tx.empty() ? 0 : (int)c->tx[c->tx_first + slot].rx.size();
+  *ncomp = c->ncomp;
+  *nt = c->rt.Nt;
+  return TCSEM_OK;
+}
+
 int tcsem_run(tcsem_ctx *c, int tx_first, int ntx, int max_steps, double *traces, int *steps_done) {
   if (check_ctx(c)) return TCSEM_EINVAL;
   if (!c->have_rx || c->tx.empty()) return fail(TCSEM_ESTATE, "set receivers and transmitters first");

@@ -1,6 +1,6 @@
 """Fixed-step run of one workload for rocprofv3 (kernel traces, counters).
 
-    python -m tcsem_fdtd_amd.profile_run --workload marine_3d --precision 32 --batch 1 --steps 20
+    python -m tcsem_fdtd_amd.measure.profile_run --workload marine_3d --precision 32 --batch 1 --steps 20
 
 Builds the deck, runs the library's march autotuner for the batch size (as bench.py does; its
 trial steps precede the field resets of tcsem_begin, the last k_fill_zero dispatches), advances `warmup` untimed steps, then
@@ -17,7 +17,7 @@ import os
 import sys
 import time
 
-sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))))
 
 
 def main():

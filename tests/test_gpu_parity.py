@@ -228,10 +228,56 @@ def test_traces_pow2_decks(name, precision, tol):
     dict(ort=5, bipolen=150.0, x=25.0, y=-10.0, z=280.0, azi=30.0, dip=20.0, bipolen1=100.0,
          x1=-40.0, y1=20.0, z1=320.0, azi1=-45.0, dip1=10.0, rx_ids=[2, 3]),
 ])
-def test_traces_orientations_fp64(tx):
-    ref, got = _deck_run(T.FP64, [tx], nsteps=150)
+@pytest.mark.parametrize("precision,tol", [(T.FP64, 1e-9), (T.FP32, FP32_TRACE_TOL)])
+def test_traces_orientations(tx, precision, tol):
+    ref, got = _deck_run(precision, [tx], nsteps=150)
     err = decks.trace_error(got[0], ref[0])
-    assert np.all(err < 1e-9), err
+    assert np.all(err < tol), err
+
+
+@pytest.mark.parametrize("precision", [T.FP64, T.FP32])
+def test_orientation_compositions_through_the_hip_path(precision):
+    """The orientation pins of the oracle, on the HIP path: ort 4 at azimuth 0 and dip 0 is ort 1
+    bit for bit (zero-current cells are dropped, the x weights are the same numbers); azimuth 90
+    is ort 2 and dip 90 is ort 3 up to cos(pi/2) = 6e-17; ort 5 with an identical second bipole is
+    ort 4 scaled by twice its length (ElectricalFiled.cu:532-641)."""
+    pos = dict(x=10.0, y=10.0, z=310.0, rx_ids=[1, 2, 3, 4])
+    txs = [dict(ort=1, bipolen=180.0, **pos), dict(ort=4, bipolen=180.0, azi=0.0, dip=0.0, **pos),
+           dict(ort=2, bipolen=180.0, **pos), dict(ort=4, bipolen=180.0, azi=90.0, dip=0.0, **pos),
+           dict(ort=3, bipolen=180.0, **pos), dict(ort=4, bipolen=180.0, azi=0.0, dip=90.0, **pos),
+           dict(ort=4, bipolen=180.0, azi=30.0, dip=20.0, **pos),
+           dict(ort=5, bipolen=180.0, azi=30.0, dip=20.0, bipolen1=180.0, x1=10.0, y1=10.0, z1=310.0,
+                azi1=30.0, dip1=20.0, **pos)]
+    so, st, m, cond = _pair()
+    rt = T.make_rxtime(0.02, 0.25, 16, 1, (1, 1, 1, 1, 1, 1))
+    rx = np.array([[150.0, 0.0, 400.0], [300.0, 60.0, 400.0], [-200.0, -50.0, 350.0], [400.0, 0.0, 300.0]])
+    with T.Context(st, cond, precision) as c:
+        c.set_receivers(rt, rx)
+        c.set_transmitters([T.make_tx(txid=i + 1, **kw) for i, kw in enumerate(txs)])
+        got = c.run(0, len(txs), max_steps=150)
+    assert np.array_equal(got[0], got[1])
+    tol = 1e-12 if precision == T.FP64 else 1e-5
+    for a, b, f in ((2, 3, 1.0), (4, 5, 1.0), (6, 7, 360.0)):
+        err = decks.trace_error(got[b], f * got[a])
+        assert np.all(err < tol), (a, b, err)
+
+
+def test_traces_of_a_run_begun_past_the_first_transmitter():
+    """A run begun at tx_first = 1 reads the traces of transmitter 1, sized by its own receiver
+    count (2 of the 4 receivers), equal to the oracle's run of that transmitter."""
+    so, st, m, cond = _pair()
+    rt_o, rt_t = O.make_rxtime(0.02, 0.25, 16, 1, (1,) * 6), T.make_rxtime(0.02, 0.25, 16, 1, (1,) * 6)
+    rx = np.array([[150.0, 0.0, 400.0], [300.0, 60.0, 400.0], [-200.0, -50.0, 350.0], [400.0, 0.0, 300.0]])
+    txs = [dict(TX_X, txid=1), dict(TX_X, x=50.0, rx_ids=[3, 2], txid=2)]
+    ref, _ = m.run(rt_o, rx, O.make_tx(**txs[1]), max_steps=120)
+    with T.Context(st, cond, T.FP64) as c:
+        c.set_receivers(rt_t, rx)
+        c.set_transmitters([T.make_tx(**kw) for kw in txs])
+        c.begin(1, 1)
+        c.step(120)
+        assert c.trace_shape(0) == (2, 6, 16)
+        got = c.traces(0)
+    assert np.all(decks.trace_error(got, ref) < 1e-9)
 
 
 def test_batch_equals_independent_runs_fp64():

@@ -117,6 +117,9 @@ def test_error_paths():
         with pytest.raises(T.TcsemError) as e:
             T.read_receivers(p)
         assert e.value.code == -3   # fewer receivers than announced
+    # the trace shape of a context that does not exist is an argument error, not a crash
+    nr, nc, nt = C.c_int(), C.c_int(), C.c_int()
+    assert T.lib().tcsem_trace_shape(None, 0, C.byref(nr), C.byref(nc), C.byref(nt)) == -1
 
 
 def test_create_without_gpu_fails_loudly():

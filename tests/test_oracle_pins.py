@@ -423,3 +423,286 @@ def test_source_injection_moment():
     want = -2 * (3.0 / length) / (1.7 * g["dy"][tp.icty] * g["dz"][tp.ictz])
     assert ex[tp.ictz, tp.icty, tp.ixa] == pytest.approx(want, rel=1e-14)
     assert np.count_nonzero(ex) == 1
+
+
+# ---- a6: composition of the orientations (ElectricalFiled.cu:532-641, LoadRxTxPos.cpp:143-428) ----
+# Bipoles span several cells and sit off the grid nodes, so that the cell search of ort 4/5
+# (searchInsert at both ends, then the larger index decremented, LoadRxTxPos.cpp:297-302) selects
+# the same cells as the ort 1-3 search (searchInsert(end) - 1).  For a bipole inside one cell the two
+# searches differ: ort 1 then spans two cells, ort 4 one.
+_ORT_RX = np.array([[160.0, 10.0, 400.0], [310.0, 70.0, 400.0], [-190.0, -40.0, 350.0], [410.0, 10.0, 290.0]])
+_ORT_POS = dict(x=10.0, y=10.0, z=310.0)
+
+
+def _ort_model():
+    s = decks.small_setup(O)
+    m0 = O.Model(s)
+    Z = m0.grid()["Z"][:, None, None] * np.ones((1, m0.M, m0.L))
+    return O.Model(s, decks.layered_cond((m0.N, m0.M, m0.L), Z, seed=11, blob=True))
+
+
+def _ort_traces(m, **kw):
+    tx = O.make_tx(rx_ids=[1, 2, 3, 4], **dict(_ORT_POS, **kw))
+    efs, _ = m.run(O.make_rxtime(0.005, 0.2, 14, 1, [1] * 6), _ORT_RX, tx, max_steps=160)
+    assert np.abs(efs).max() > 0
+    return efs
+
+
+def _rel_per_receiver(a, b):
+    return np.array([np.linalg.norm(a[r] - b[r]) / np.linalg.norm(b[r]) for r in range(b.shape[0])])
+
+
+def test_ort4_at_zero_angles_is_ort1_bit_exact():
+    """azimuth 0, dip 0: the y and z currents are exactly zero (sin 0), the x cells and the x
+    current equal those of an x-bipole (cos 0 = 1)."""
+    m = _ort_model()
+    a = _ort_traces(m, ort=1, bipolen=180.0)
+    b = _ort_traces(m, ort=4, bipolen=180.0, azi=0.0, dip=0.0)
+    assert np.array_equal(a, b)
+
+
+def test_ort4_at_azimuth_90_is_ort2():
+    """azimuth 90: a y-bipole.  cos(pi/2) = 6e-17 leaves an x current of that relative size."""
+    m = _ort_model()
+    a = _ort_traces(m, ort=2, bipolen=180.0)
+    b = _ort_traces(m, ort=4, bipolen=180.0, azi=90.0, dip=0.0)
+    assert np.all(_rel_per_receiver(b, a) < 1e-12)
+    # and not an x-bipole
+    assert np.all(_rel_per_receiver(_ort_traces(m, ort=1, bipolen=180.0), a) > 0.1)
+
+
+def test_ort4_at_dip_90_is_ort3():
+    m = _ort_model()
+    a = _ort_traces(m, ort=3, bipolen=180.0)
+    b = _ort_traces(m, ort=4, bipolen=180.0, azi=0.0, dip=90.0)
+    assert np.all(_rel_per_receiver(b, a) < 1e-12)
+    assert np.all(_rel_per_receiver(_ort_traces(m, ort=1, bipolen=180.0), a) > 0.1)
+
+
+def test_ort4_splits_the_current_by_azimuth_and_dip():
+    """A tilted bipole whose three projections span the same cells as the axis bipoles at the
+    same centre carries the currents cos(dip)cos(azi), cos(dip)sin(azi), sin(dip) per unit moment
+    of each axis bipole's own cell span, so by linearity it is that sum of ort 1-3 runs, each
+    rescaled by the ratio of the span lengths (ElectricalFiled.cu:561-611)."""
+    m = _ort_model()
+    g = m.grid()
+    azi, dip, bl = 35.0, 25.0, 300.0
+    ca, sa, cd, sd = (np.cos(np.radians(azi)), np.sin(np.radians(azi)), np.cos(np.radians(dip)),
+                      np.sin(np.radians(dip)))
+    tilted = _ort_traces(m, ort=4, bipolen=bl, azi=azi, dip=dip)
+    tp = m.locate_tx(O.make_tx(ort=4, bipolen=bl, azi=azi, dip=dip, rx_ids=[1], **_ORT_POS))
+    # axis bipoles with the same cell spans: ort 1-3 lengths chosen so that their searches select
+    # [ixa, ixb], [iya, iyb], [iza, izb] (each end inside its cell, centred on the tilted centre)
+    X, Y, Z, dx, dy, dz = g["X"], g["Y"], g["Z"], g["dx"], g["dy"], g["dz"]
+    span = {1: X[tp.ixb] - X[tp.ixa] + dx[tp.ixb] / 2 + dx[tp.ixa] / 2,
+            2: Y[tp.iyb] - Y[tp.iya] + dy[tp.iyb] / 2 + dy[tp.iya] / 2,
+            3: Z[tp.izb] - Z[tp.iza] + dz[tp.izb] / 2 + dz[tp.iza] / 2}
+    centre = dict(x=0.5 * (X[tp.ixa] + X[tp.ixb]), y=0.5 * (Y[tp.iya] + Y[tp.iyb]), z=0.5 * (Z[tp.iza] + Z[tp.izb]))
+    assert tp.ictx == tp.ixa or tp.ictx == tp.ixb or tp.ixa < tp.ictx < tp.ixb
+    parts = []
+    for ort, w in ((1, cd * ca), (2, cd * sa), (3, sd)):
+        length = span[ort] - 1.0  # both ends 0.5 m inside the end cells
+        tx = O.make_tx(ort=ort, bipolen=length, rx_ids=[1, 2, 3, 4], **centre)
+        tq = m.locate_tx(tx)
+        sel = {1: (tq.ixa, tq.ixb, tp.ixa, tp.ixb), 2: (tq.iya, tq.iyb, tp.iya, tp.iyb),
+               3: (tq.iza, tq.izb, tp.iza, tp.izb)}[ort]
+        assert (min(sel[:2]), max(sel[:2])) == (sel[2], sel[3]), (ort, sel)
+        if (ort == 1 and (tq.icty, tq.ictz) != (tp.icty, tp.ictz)) or (ort == 2 and (tq.ictx, tq.ictz) != (tp.ictx, tp.ictz)) \
+                or (ort == 3 and (tq.ictx, tq.icty) != (tp.ictx, tp.icty)):
+            pytest.skip("the axis bipole's cross-section cell differs from the tilted one's")
+        efs, _ = m.run(O.make_rxtime(0.005, 0.2, 14, 1, [1] * 6), _ORT_RX, tx, max_steps=160)
+        parts.append(w * efs)
+    assert np.all(_rel_per_receiver(tilted, sum(parts)) < 1e-11)
+
+
+def test_ort5_without_second_bipole_is_ort4_times_its_length():
+    """ort 5 multiplies the first bipole's currents by BipoLen (ElectricalFiled.cu:612-636); a
+    second bipole of zero length adds nothing."""
+    m = _ort_model()
+    a = _ort_traces(m, ort=4, bipolen=180.0, azi=30.0, dip=20.0)
+    b = _ort_traces(m, ort=5, bipolen=180.0, azi=30.0, dip=20.0, bipolen1=0.0, x1=-40.0, y1=20.0, z1=320.0,
+                    azi1=-45.0, dip1=10.0)
+    assert np.all(_rel_per_receiver(b, 180.0 * a) < 1e-12)
+
+
+def test_ort5_second_bipole_is_injected_on_the_first_bipoles_cells():
+    """The reference's AddTx kernels read the first bipole's cell indices for both bipoles (the
+    second bipole's own indices only set its current's span length): a second bipole identical to
+    the first doubles the response, and moving it leaves the cells where they are while its span
+    length sets the current."""
+    m = _ort_model()
+    a = _ort_traces(m, ort=4, bipolen=180.0, azi=30.0, dip=20.0)
+    same = _ort_traces(m, ort=5, bipolen=180.0, azi=30.0, dip=20.0, bipolen1=180.0, x1=10.0, y1=10.0,
+                       z1=310.0, azi1=30.0, dip1=20.0)
+    assert np.all(_rel_per_receiver(same, 360.0 * a) < 1e-12)
+    # the second bipole moved by whole cells (same spans): the same currents, still on the first
+    # bipole's cells, so the same traces
+    moved = _ort_traces(m, ort=5, bipolen=180.0, azi=30.0, dip=20.0, bipolen1=180.0, x1=60.0, y1=60.0,
+                        z1=360.0, azi1=30.0, dip1=20.0)
+    assert np.all(_rel_per_receiver(moved, 360.0 * a) < 1e-12)
+
+
+# ---- a14: receiver sampling (EFInterp2Rx.cu:46-585) ------------------------------------------------
+def _face(c, d):
+    """Lower faces of the cells and the upper face of the last one."""
+    return np.concatenate([c - 0.5 * d, [c[-1] + 0.5 * d[-1]]])
+
+
+# staggered positions of every component (x, y, z): c = cell centre, f = cell face
+_STAGGER = {0: "cff", 1: "fcf", 2: "ffc", 3: "fcc", 4: "cfc", 5: "ccf"}
+
+
+def test_gather_stencil_puts_the_receiver_in_its_middle_interval():
+    """The 4-node stencil of every component brackets the receiver by its two middle nodes on
+    each axis, and its nodes are the component's staggered positions (EFInterp2Rx.cu:75-88, the
+    half-cell test on the face axes).  Above the second node in z (the surface cells) the start
+    is clamped to level 0 and the receiver lies in the first interval."""
+    s = decks.small_setup(O)
+    m = O.Model(s)
+    g = m.grid()
+    pos = {"c": (g["X"], g["Y"], g["Z"]), "f": (_face(g["X"], g["dx"]), _face(g["Y"], g["dy"]), _face(g["Z"], g["dz"]))}
+    rng = np.random.default_rng(21)
+    for comp in range(6):
+        for _ in range(300):
+            x = rng.uniform(g["X"][2], g["X"][-3])
+            y = rng.uniform(g["Y"][2], g["Y"][-3])
+            z = rng.uniform(g["Z"][0], g["Z"][-4])
+            st, t = m.rx_stencil(x, y, z, comp)
+            for ax, v in enumerate((x, y, z)):
+                nodes = pos[_STAGGER[comp][ax]][ax]
+                assert np.array_equal(t[ax], nodes[st[ax]:st[ax] + 4]), (comp, ax)
+                if ax < 2 or nodes[1] <= v:
+                    assert t[ax][1] <= v <= t[ax][2], (comp, ax, v, t[ax])
+                else:  # shallower than the second node: clamped to level 0
+                    assert st[ax] == 0 and t[ax][0] <= v <= t[ax][1] or v < t[ax][0], (comp, v, t[ax])
+
+
+def test_gather_is_the_tensor_natural_spline_of_its_stencil():
+    """Random fields: the gathered value equals scipy's natural cubic spline through the 4 nodes,
+    applied along x, then y, then z over the 4 x 4 x 4 block the stencil selects."""
+    s = decks.small_setup(O)
+    m = O.Model(s)
+    rng = np.random.default_rng(22)
+    for comp in range(6):
+        m.set(comp, rng.standard_normal(m.shape(comp)))
+    g = m.grid()
+    rx = np.stack([rng.uniform(g["X"][2], g["X"][-3], 40), rng.uniform(g["Y"][2], g["Y"][-3], 40),
+                   rng.uniform(g["Z"][1], g["Z"][-4], 40)], axis=1)
+    got = m.rx_values(rx, list(range(6)))
+    for c in range(6):
+        F = m.get(c)
+        for r, (x, y, z) in enumerate(rx):
+            st, t = m.rx_stencil(x, y, z, c)
+            blk = F[st[2]:st[2] + 4, st[1]:st[1] + 4, st[0]:st[0] + 4]
+            vx = CubicSpline(t[0], blk, axis=2, bc_type="natural")(x)
+            vy = CubicSpline(t[1], vx, axis=1, bc_type="natural")(y)
+            want = float(CubicSpline(t[2], vy, bc_type="natural")(z))
+            assert got[r, c] == pytest.approx(want, rel=1e-11, abs=1e-12), (c, r)
+
+
+def test_gather_exact_on_linear_fields():
+    s = decks.small_setup(O)
+    m = O.Model(s)
+    g = m.grid()
+    pos = {"c": (g["X"], g["Y"], g["Z"]), "f": (_face(g["X"], g["dx"]), _face(g["Y"], g["dy"]), _face(g["Z"], g["dz"]))}
+    a, b, c0, d = 0.3, 1.7e-3, -2.2e-3, 0.9e-3
+    for comp in range(6):
+        n = m.shape(comp)
+        xs = pos[_STAGGER[comp][0]][0][:n[2]]
+        ys = pos[_STAGGER[comp][1]][1][:n[1]]
+        zs = pos[_STAGGER[comp][2]][2][:n[0]]
+        m.set(comp, a + b * xs[None, None, :] + c0 * ys[None, :, None] + d * zs[:, None, None])
+    rng = np.random.default_rng(23)
+    rx = np.stack([rng.uniform(g["X"][2], g["X"][-3], 30), rng.uniform(g["Y"][2], g["Y"][-3], 30),
+                   rng.uniform(g["Z"][1], g["Z"][-4], 30)], axis=1)
+    got = m.rx_values(rx, list(range(6)))
+    want = a + b * rx[:, 0] + c0 * rx[:, 1] + d * rx[:, 2]
+    assert np.allclose(got, want[:, None], rtol=1e-12, atol=1e-12)
+
+
+def _stepwise_states(m, s, rt_times, rx, tx, nsteps, ecomps, bcomps):
+    """The field states of a manual run: after step n the E fields stand for time
+    t_E(n) = toT + dt1 and the B fields for t_B(n) = t_E(n) + dt2 / 2, with (toT, dt1, dt2) the
+    values of step n before the time advance (main.cu:227-248, the comments at the E and B
+    calls)."""
+    rt_long = O.make_rxtime(1e-3, 1e3, 4, 1, [1] * 6)
+    lam, dte, dtb, amp = O.schedule(s, m.mindx(), rt_long, nsteps + 2)
+    assert len(dte) >= nsteps + 1
+    m.zero()
+    tp = m.locate_tx(tx)
+    vE, vB = [np.zeros((len(rx), len(ecomps)))], [np.zeros((len(rx), len(bcomps)))]  # the zero state at t = 0
+    tE, tB = [0.0], [0.0]
+    toT = 0.0
+    for n in range(nsteps):
+        m.step(lam[n], dte[n], amp[n], tp, dtb[n])
+        tE.append(toT + dte[n])
+        tB.append(toT + dte[n] + 0.5 * dte[n + 1])
+        toT += dte[n]
+        vE.append(m.rx_values(rx, ecomps))
+        vB.append(m.rx_values(rx, bcomps))
+    return (np.array(tE), np.array(vE)), (np.array(tB), np.array(vB))
+
+
+def _linear_in_time(ts, states):
+    t, v = states
+    assert np.all(np.diff(t) > 0)
+    n = np.searchsorted(t, ts, side="left")  # t[n-1] < ts <= t[n]
+    assert np.all(n >= 1) and np.all(n < len(t))
+    w = (ts - t[n - 1]) / (t[n] - t[n - 1])
+    return (1.0 - w)[:, None, None] * v[n - 1] + w[:, None, None] * v[n], n
+
+
+def _time_case(rt_args):
+    # 200 source sub-steps (the reference's golden waveform size): monotone state times; with far
+    # fewer sub-steps the reference's equal-area construction steps back in time
+    s = decks.small_setup(O, n_src=200)
+    m0 = O.Model(s)
+    Z = m0.grid()["Z"][:, None, None] * np.ones((1, m0.M, m0.L))
+    m = O.Model(s, decks.layered_cond((m0.N, m0.M, m0.L), Z, seed=4))
+    rx = np.array([[160.0, 10.0, 400.0], [310.0, 70.0, 400.0], [-190.0, -40.0, 350.0]])
+    tx = O.make_tx(1, 100.0, 0.0, 0.0, 300.0, [1, 2, 3])
+    rt = O.make_rxtime(*rt_args, [1] * 6)
+    ref, steps = m.run(rt, rx, tx)
+    ts = O.times(rt)
+    E, B = _stepwise_states(m, s, ts, rx, tx, steps, [0, 1, 2], [3, 4, 5])
+    return ref, ts, E, B
+
+
+def test_time_interpolation_is_linear_between_field_states():
+    """Samples interpolated linearly in time between the two field states that bracket them,
+    with the zero state at t = 0 before the first step (EFInterp2Rx.cu:472-585).  A swapped pair
+    of weights or a state shifted by one step moves these by O(1) of the change per step."""
+    ref, ts, E, B = _time_case((0.004, 0.05, 12, 1))
+    for states, cs in ((E, slice(0, 3)), (B, slice(3, 6))):
+        want, n = _linear_in_time(ts, states)
+        gaps = np.diff(np.unique(n))
+        assert np.all(gaps >= 2)  # no two samples in neighbouring step intervals (see the next test)
+        got = np.moveaxis(ref[:, cs, :], 2, 0)  # [t][r][c]
+        scale = np.abs(got).max(axis=0)
+        assert np.all(np.abs(got - want) <= 1e-13 * scale + 1e-300), np.abs(got - want).max()
+
+
+def test_time_interpolation_of_samples_in_neighbouring_intervals():
+    """The reference keeps the earlier state of an interval only when the sample it serves lies
+    beyond the current state (slot 0 of ef0, EFInterp2Rx.cu:493-500); after interpolating (slot 1,
+    :505-521) it does not move the current state into slot 0.  A sample whose predecessor fell in
+    the immediately preceding step interval is therefore interpolated from a stale slot 0.  With
+    dense samples, exactly the samples of such intervals differ from the linear interpolation;
+    every other sample matches it."""
+    ref, ts, E, B = _time_case((0.010, 0.016, 25, 0))
+    for states, cs in ((E, slice(0, 3)), (B, slice(3, 6))):
+        want, n = _linear_in_time(ts, states)
+        got = np.moveaxis(ref[:, cs, :], 2, 0)
+        scale = np.abs(got).max(axis=0)
+        bad = np.abs(got - want) > 1e-12 * scale
+        bad_t = bad.any(axis=(1, 2))
+        prev = np.concatenate([[-10], n[:-1]])
+        groups_first = n != prev  # first sample of each interval's group
+        stale = np.zeros(len(ts), bool)
+        for q in range(len(ts)):
+            first = q if groups_first[q] else np.flatnonzero(groups_first[:q + 1])[-1]
+            stale[q] = first > 0 and n[first] == n[first - 1] + 1
+        assert stale.any() and (~stale).any()
+        # every sample of a stale group differs where its interval's states differ
+        moved = np.abs(states[1][n] - states[1][n - 1]).max(axis=(1, 2)) > 0
+        assert np.array_equal(bad_t, stale & moved)
