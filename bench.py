@@ -1,17 +1,18 @@
 #!/usr/bin/env python3
 """Throughput of the DuFort-Frankel time step on one MI355X.
 
-A step is one full DF step (volume march, upward continuation of Bz(0), surface level, and the
-receiver gathers the sampling schedule asks for) over the marine_3d workload: 256 x 256 x 160
-cells, seeded bathymetry and resistive slab, one x-dipole, 101 receivers.  Inputs are resident in
-HBM before the timed region.  The value is FP32 Gcell-updates/s of the single model; the same line
-carries the plain-FP64 and the FP32 transmitter-batched (32 x-dipoles) throughput, the large
-512 x 512 x 256 model in FP32 and FP64 over a window of steps, the roofline of the volume march
-(HIP events on the library's stream), the LDS and HBM rates of the air-plane chain and of the
-receiver gathers, and the FP64 CPU oracle timed on this host as the baseline.  Every GPU run first
-lets the library's autotuner pick the march tiling for its batch size.  The line also carries
-64 conductivity models of uneven grids advanced as one model batch (FP32 and FP64), and complete
-runs of the small 64 x 64 x 48 model.
+A step is one full DF step (volume march with the source cells, upward continuation of Bz(0),
+surface level, and the receiver gathers the sampling schedule asks for).  The headline workload is
+the largest single-GPU configuration, large_5: 512 x 512 x 256 cells, a seeded layered model with
+two reservoirs under rough bathymetry, the first of its three x-dipoles, 401 receivers.  Inputs
+are resident in HBM before the timed region.  The value is its FP32 Gcell-updates/s over exactly
+`--steps` steps after `--warmup`; the same line carries its FP64 throughput over the same window,
+the 256 x 256 x 160 marine model in FP32 and FP64, the FP32 transmitter batch (32 x-dipoles on
+that grid), 64 conductivity models of uneven grids advanced as one model batch (FP32 and FP64),
+complete runs of the small 64 x 64 x 48 model, the roofline of the volume march (HIP events on the
+library's stream), the LDS and HBM rates of the air-plane chain and of the receiver gathers, and
+the FP64 CPU oracle timed on this host's cores as the baseline.  Every GPU run first lets the
+library's autotuner pick the march tiling for its batch size.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu] [--no-tune]
     python bench.py --accuracy   # complete batched runs, FP32 vs FP64 trace errors
@@ -259,29 +260,89 @@ def side_kernels(res, deck, esz, nbatch):
     return out
 
 
-def cpu_baseline(deck, budget_s=12.0, max_steps=30):
-    """The FP64 oracle (C + OpenMP) on this host: whole steps of the same workload."""
-    from oracle import oracle as O
+def host_info():
+    """nproc, the CPU model name and the physical core count of this host (lscpu)."""
+    import subprocess
+    info = {"nproc": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = {}
+        for line in out.splitlines():
+            if ":" in line:
+                k, v = line.split(":", 1)
+                kv[k.strip()] = v.strip()
+        info["model"] = kv.get("Model name")
+        cps, sockets = kv.get("Core(s) per socket"), kv.get("Socket(s)")
+        if cps and sockets:
+            info["physical_cores"] = int(cps) * int(sockets)
+        info["threads_per_core"] = kv.get("Thread(s) per core")
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    try:
+        info["nproc_affinity"] = len(os.sched_getaffinity(0))
+    except OSError:
+        pass
+    return info
+
+
+def cpu_window(O, deck, nsteps, warmup=5):
+    """The FP64 oracle on this host: `warmup` untimed then `nsteps` timed DuFort-Frankel steps of
+    the deck's first transmitter from the start of its run, with the reference's CPU formulation of
+    the air boundary (SplineGridConvIR2R_cpu, the UpContinuation filter, BilinearGridConvR2IR)."""
     so = O.make_setup(**deck.setup)
     m = O.Model(so, deck.cond)
+    m.set_cpu_twins(True)
     r = deck.rxtime
-    rt = O.make_rxtime(r["T0"], r["Tn"], r["Nt"], r["is_log"], r["comps"])
-    lam, dte, dtb, amp = O.schedule(so, m.mindx(), rt, 400)
-    tx = O.make_tx(**deck.txs[0])
-    tp = m.locate_tx(tx)
-    m.step(lam[0], dte[0], amp[0], tp, dtb[0])     # warm-up
-    n, t0 = 0, time.perf_counter()
-    while n < max_steps:
-        m.step(lam[n + 1], dte[n + 1], amp[n + 1], tp, dtb[n + 1])
-        n += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
+    lam, dte, dtb, amp = O.schedule(so, m.mindx(), O.make_rxtime(r["T0"], r["Tn"], r["Nt"], r["is_log"], r["comps"]),
+                                    warmup + nsteps)
+    tp = m.locate_tx(O.make_tx(**deck.txs[0]))
+    for n in range(warmup):
+        m.step(lam[n], dte[n], amp[n], tp, dtb[n])
+    t0 = time.perf_counter()
+    for n in range(warmup, warmup + nsteps):
+        m.step(lam[n], dte[n], amp[n], tp, dtb[n])
     sec = time.perf_counter() - t0
     cells = m.L * m.M * m.N
-    return {"value": round(cells * n / sec / 1e9, 5), "unit": "Gcell-updates/s",
+    return {"gcell_s": round(cells * nsteps / sec / 1e9, 5), "steps": nsteps, "warmup": warmup,
+            "seconds": round(sec, 3), "cells": cells}
+
+
+def cpu_full_run(O, deck):
+    """A complete run of the deck's first transmitter on the FP64 oracle (every step to the last
+    sample, receivers and all), CPU-formulation air boundary, timed by the wall clock."""
+    so = O.make_setup(**deck.setup)
+    m = O.Model(so, deck.cond)
+    m.set_cpu_twins(True)
+    r = deck.rxtime
+    t0 = time.perf_counter()
+    _, steps = m.run(O.make_rxtime(r["T0"], r["Tn"], r["Nt"], r["is_log"], r["comps"]), deck.rx,
+                     O.make_tx(**deck.txs[0]))
+    sec = time.perf_counter() - t0
+    cells = m.L * m.M * m.N
+    return {"gcell_s": round(cells * steps / sec / 1e9, 5), "steps": steps, "seconds": round(sec, 3),
+            "cells": cells}
+
+
+def cpu_baseline(W, T, window=50):
+    """The CPU path timed on this host's cores: the FP64 C/OpenMP oracle (a restatement of the
+    reference's kernels; the reference's own CUDA path cannot run here) with OpenMP threads bound
+    close.  The small 64 x 64 x 48 model complete; the 256 x 256 x 160 and 512 x 512 x 256 models
+    over `window` steps after 5 warm-up steps.  `value` is the 512 x 512 x 256 window (the
+    headline's workload)."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    from oracle import oracle as O
+    out = {"marine_1d_complete": cpu_full_run(O, W.marine_1d(T)),
+           "marine_3d_window": cpu_window(O, W.marine_3d(T), window),
+           "large_5_window": cpu_window(O, W.large_5(T), window)}
+    host = host_info()
+    return {"value": out["large_5_window"]["gcell_s"], "unit": "Gcell-updates/s",
             "cores": O.threads(), "kind": "port",
-            "sample": "%d FP64 steps of marine_3d %dx%dx%d after 1 warm-up step (oracle, C/OpenMP)"
-                      % (n, m.L, m.M, m.N)}
+            "sample": "FP64 oracle (C/OpenMP, OMP_PROC_BIND=%s), CPU-formulation air boundary: "
+                      "large_5 512x512x256 %d steps after 5 warm-up (value); marine_3d 256x256x160 %d steps "
+                      "after 5 warm-up; marine_1d 64x64x48 complete run" % (os.environ["OMP_PROC_BIND"], window,
+                                                                          window),
+            "host": host, "configs": out}
 
 
 def trace_error(got, ref, floor=1e-6):
@@ -394,21 +455,25 @@ def main():
         print(json.dumps({"accuracy": "FP32 vs FP64 per-receiver relative trace error, complete runs",
                           **accuracy(T, W)}), flush=True)
         return
-    deck = W.marine_3d(T)
     tune = not args.no_tune
+    kern = lambda r: {k: round(v, 5) for k, v in r["timing"].items() if k.endswith("_ms")}
+    # the headline: the large single-GPU model (512 x 512 x 256 cells, 1024 x 1024 air plane) in
+    # FP32, exactly `steps` timed steps after `warmup`; the same window in FP64 beside it
+    big = W.large_5(T)
+    rl32 = gpu_run(T, big, T.FP32, 1, args.steps, args.warmup, tune)
+    rl64 = gpu_run(T, big, T.FP64, 1, args.steps, args.warmup, tune)
+    big_err = float(np.linalg.norm(rl32["ex"] - rl64["ex"]) / max(np.linalg.norm(rl64["ex"]), 1e-300))
+    rl32.pop("ex"), rl64.pop("ex")
+    # the 256 x 256 x 160 marine model, FP32 and FP64, one transmitter
+    deck = W.marine_3d(T)
     r32 = gpu_run(T, deck, T.FP32, 1, args.steps, args.warmup, tune)
     r64 = gpu_run(T, deck, T.FP64, 1, args.steps, args.warmup, tune)
     field_err = float(np.linalg.norm(r32["ex"] - r64["ex"]) / max(np.linalg.norm(r64["ex"]), 1e-300))
+    r32.pop("ex"), r64.pop("ex")
+    # 32 x-dipoles along a towline on the same grid, FP32 batched
     tow = W.towline(T, ntx=args.batch)
     rb = gpu_run(T, tow, T.FP32, args.batch, max(args.steps // 5, 1), args.warmup, tune)
-    # the large single-GPU model (512 x 512 x 256 cells, 1024 x 1024 air plane): FP32 and FP64
-    # over a window of steps from the start of the run, and their field difference after it
-    big = W.large_5(T)
-    nl, wl = max(args.steps // 2, 1), args.warmup
-    rl32 = gpu_run(T, big, T.FP32, 1, nl, wl, tune)
-    rl64 = gpu_run(T, big, T.FP64, 1, nl, wl, tune)
-    big_err = float(np.linalg.norm(rl32["ex"] - rl64["ex"]) / max(np.linalg.norm(rl64["ex"]), 1e-300))
-    rl32.pop("ex"), rl64.pop("ex")
+    rb.pop("ex")
     # 64 conductivity models of uneven grids (96^3 .. 224x224x160), one x-dipole each, in one
     # model batch: FP32 and FP64, FP32 also stepped model by model for comparison
     models = W.uneven_models(T)
@@ -424,63 +489,64 @@ def main():
     s32g = full_run(T, small, T.FP32, graph=16)
     line = {
         "metric": METRIC,
-        "value": round(r32["gcell_s"], 3),
+        "value": round(rl32["gcell_s"], 3),
         "unit": "Gcell-updates/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(r32["ms_per_step"], 5),
+        "ms_per_step": round(rl32["ms_per_step"], 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded model, bathymetry and slab; see tcsem_fdtd_amd/workloads.py)",
-        "config": {"workload": "marine_3d 256x256x160, 1 x-dipole, 101 receivers, FP32 fields",
-                   "cells": r32["L"] * r32["M"] * r32["N"], "air_plane": "512x512",
-                   "batch": 1, "parallelism": "single GPU", "march_tiling": r32["tiling"]},
-        "roofline": roofline(r32, 4, 1, load_traffic("marine_3d_fp32")),
-        "kernel_ms": {k: round(v, 5) for k, v in r32["timing"].items() if k.endswith("_ms")},
-        "side_kernels": side_kernels(r32, deck, 4, 1),
-        "fp64_single": {"value": round(r64["gcell_s"], 3), "ms_per_step": round(r64["ms_per_step"], 5),
-                        "roofline": roofline(r64, 8, 1, load_traffic("marine_3d_fp64")),
-                        "march_tiling": r64["tiling"],
-                        "kernel_ms": {k: round(v, 5) for k, v in r64["timing"].items() if k.endswith("_ms")}},
+        "data": "synthetic (seeded layered model, reservoirs and rough bathymetry; see tcsem_fdtd_amd/workloads.py)",
+        "config": {"workload": "large_5 512x512x256, 1024x1024 air plane, x-dipole 1 of 3, 401 receivers, FP32 fields",
+                   "cells": rl32["L"] * rl32["M"] * rl32["N"], "batch": 1, "parallelism": "single GPU",
+                   "march_tiling": rl32["tiling"]},
+        "roofline": roofline(rl32, 4, 1, load_traffic("large_5_fp32")),
+        "kernel_ms": kern(rl32),
+        "side_kernels": side_kernels(rl32, big, 4, 1),
+        "fp64_single": {"workload": "large_5 512x512x256, FP64 fields, the same window",
+                        "value": round(rl64["gcell_s"], 3), "ms_per_step": round(rl64["ms_per_step"], 5),
+                        "roofline": roofline(rl64, 8, 1, load_traffic("large_5_fp64")),
+                        "march_tiling": rl64["tiling"], "kernel_ms": kern(rl64),
+                        "fp32_vs_fp64_ex_rel_l2_after_window": big_err},
+        "marine_3d": {"workload": "marine_3d 256x256x160, 512x512 air plane, 1 x-dipole, 101 receivers",
+                      "steps": args.steps, "warmup": args.warmup,
+                      "fp32": {"value": round(r32["gcell_s"], 3), "ms_per_step": round(r32["ms_per_step"], 5),
+                               "roofline": roofline(r32, 4, 1, load_traffic("marine_3d_fp32")),
+                               "march_tiling": r32["tiling"], "kernel_ms": kern(r32),
+                               "side_kernels": side_kernels(r32, deck, 4, 1)},
+                      "fp64": {"value": round(r64["gcell_s"], 3), "ms_per_step": round(r64["ms_per_step"], 5),
+                               "roofline": roofline(r64, 8, 1, load_traffic("marine_3d_fp64")),
+                               "march_tiling": r64["tiling"], "kernel_ms": kern(r64)},
+                      "fp32_vs_fp64_ex_rel_l2_after_window": field_err},
         "fp32_batched": {"value": round(rb["gcell_s"], 3), "batch": args.batch,
                          "workload": "towline %d x-dipoles, marine_3d grid" % args.batch,
+                         "steps": max(args.steps // 5, 1), "warmup": args.warmup,
                          "ms_per_step": round(rb["ms_per_step"], 5),
                          "roofline": roofline(rb, 4, args.batch, load_traffic("towline32_fp32")),
-                         "march_tiling": rb["tiling"],
-                         "kernel_ms": {k: round(v, 5) for k, v in rb["timing"].items() if k.endswith("_ms")},
+                         "march_tiling": rb["tiling"], "kernel_ms": kern(rb),
                          "side_kernels": side_kernels(rb, tow, 4, args.batch)},
-        "large_5": {"workload": "large_5 512x512x256, 1024x1024 air plane, 3 x-dipoles (first one run), 401 receivers",
-                    "steps": nl, "warmup": wl,
-                    "fp32": {"value": round(rl32["gcell_s"], 3), "ms_per_step": round(rl32["ms_per_step"], 5),
-                             "roofline": roofline(rl32, 4, 1, None), "march_tiling": rl32["tiling"],
-                             "side_kernels": side_kernels(rl32, big, 4, 1)},
-                    "fp64": {"value": round(rl64["gcell_s"], 3), "ms_per_step": round(rl64["ms_per_step"], 5),
-                             "roofline": roofline(rl64, 8, 1, None), "march_tiling": rl64["tiling"]},
-                    "fp32_vs_fp64_ex_rel_l2_after_window": big_err},
         "uneven_models": {"workload": "64 models, grids 96^3 .. 224x224x160 (%d cells), 1 x-dipole and 41 receivers each"
                                       % m32["cells"], "steps": nm, "warmup": wm,
                           "fp32": {"value": round(m32["gcell_s"], 3), "ms_per_step": round(m32["ms_per_step"], 5),
                                    "roofline": models_roofline(m32, 4, load_traffic("uneven64_fp32")),
-                                   "march_tiling": m32["tiling"],
-                                   "kernel_ms": {k: round(v, 5) for k, v in m32["timing"].items() if k.endswith("_ms")},
+                                   "march_tiling": m32["tiling"], "kernel_ms": kern(m32),
                                    "unbatched": {k: round(v, 5) if isinstance(v, float) else v
                                                  for k, v in m32["unbatched"].items()}},
                           "fp64": {"value": round(m64["gcell_s"], 3), "ms_per_step": round(m64["ms_per_step"], 5),
                                    "roofline": models_roofline(m64, 8), "march_tiling": m64["tiling"],
-                                   "kernel_ms": {k: round(v, 5) for k, v in m64["timing"].items() if k.endswith("_ms")}},
+                                   "kernel_ms": kern(m64)},
                           "fp32_vs_fp64_ex_rel_l2_after_window": models_err},
         "marine_1d": {"workload": "marine_1d 64x64x48, 1 x-dipole, 21 receivers, complete run (tcsem_run)",
                       "fp32": s32, "fp64": s64, "fp32_graph_replays": s32g},
+        # the north-star ratio: FP32 transmitter batch against the plain FP64 single model (the
+        # 256 x 256 x 160 grid they share, and the large model's FP64 window)
         "ratio_fp32_batched_vs_fp64": round(rb["gcell_s"] / r64["gcell_s"], 3),
-        "fp32_vs_fp64_ex_rel_l2_after_run": field_err,
+        "ratio_fp32_batched_vs_fp64_large_5": round(rb["gcell_s"] / rl64["gcell_s"], 3),
     }
-    if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(deck)
-    else:
-        line["cpu_baseline"] = None
+    line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(W, T)
     print(json.dumps(line), flush=True)
 
 

@@ -412,7 +412,8 @@ struct tcsem_ctx {
   char *d_zs = nullptr;      // z-chunk Bz partial sums: (kMaxChunks - 1) x nslots x pstride
   // chunk sums of B written by the chunked lower march for the next step (FP64): 2 parities x
   // (ib_cap chunks) x (x, y) x nslots x pstride; valid while the state is only advanced by steps
-  // of the same chunking (set, reset, begin, source passes and model-batch steps invalidate them)
+  // of the same chunking (set, reset, begin and model-batch steps invalidate them; the source pass
+  // moves them with the B cells it recomputes)
   double *d_ib = nullptr;
   int ib_cap = 0, ib_par = 0, ib_nchunk = 0, ib_chunk_len = 0;
   bool ib_valid = false;
@@ -453,9 +454,11 @@ struct tcsem_ctx {
   double *d_ef0 = nullptr, *d_efs = nullptr;
   SrcCell *d_src = nullptr;
   int *d_srcidx = nullptr;  // first[nbatch], count[nbatch]
-  // source pass after the march (levels 1..N-1): merged E cells and the B cells they touch
+  // source pass after the march (levels 1..N-1): merged E cells, the B cells they touch, and
+  // the (first, count) runs of those B cells, one per component and column
   SrcCell *d_fixe = nullptr, *d_fixb = nullptr;
-  int *d_fixidx = nullptr;  // E first[nbatch], E count[nbatch], B first[nbatch], B count[nbatch]
+  int2 *d_fixg = nullptr;
+  int *d_fixidx = nullptr;  // E first[nbatch], E count[nbatch], run first[nbatch], run count[nbatch]
   bool has_fix = false;
   int *d_rxl = nullptr;     // rx_local, rx_first[nbatch], rx_count[nbatch]
   int *d_ops = nullptr;     // it of every op of the run (E ops then B ops)
@@ -757,8 +760,10 @@ void free_run(tcsem_ctx *c) {
   (void)hipFree(c->d_srcidx);
   (void)hipFree(c->d_fixe);
   (void)hipFree(c->d_fixb);
+  (void)hipFree(c->d_fixg);
   (void)hipFree(c->d_fixidx);
   c->d_fixe = c->d_fixb = nullptr;
+  c->d_fixg = nullptr;
   c->d_fixidx = nullptr;
   c->has_fix = false;
   (void)hipFree(c->d_rxl);
@@ -1279,9 +1284,23 @@ int do_step(tcsem_ctx *c) {
       if (!(e = timing_event(c))) return fail(TCSEM_EDEVICE, "timing event creation failed");
   }
   HIPCHK(enqueue_march<T>(c, a, nb, st, c->xev[1], te));
+  // source cells at levels 1 .. N-1: a pass after the march, which moves the chunk sums of B that a
+  // chunked lower launch has just written along with the B cells it recomputes
   if (c->has_fix && a.amp != 0.0) {
-    HIPCHK(launch_src_fix<T>(a, c->d_fixe, c->d_fixidx, c->d_fixb, c->d_fixidx + 2 * nb, nb, st));
-    c->ib_valid = false;  // the pass rewrites B cells after the march summed them
+    MarchArgs<T> sp = a;
+    sp.fixe = c->d_fixe;
+    sp.fixb = c->d_fixb;
+    sp.fixg = c->d_fixg;
+    sp.fixidx = c->d_fixidx;
+    sp.fixn = nb;
+    if (split && c->nchunk > 1 && c->fuse) {
+      sp.ibw = ib_planes(c, c->ib_par);  // the parity this step's lower launch wrote
+      sp.ibstride = (long long)c->nslots * c->pstride;
+      sp.nchunk = c->nchunk;
+      sp.chunk_len = c->chunk_len;
+      sp.k_hi = N - 1;
+    }
+    HIPCHK(launch_src<T>(sp, nb, st));
   }
   HIPCHK(hipEventRecord(c->xev[0], st));
   HIPCHK(hipStreamWaitEvent(st2, c->xev[0], 0));
@@ -1600,8 +1619,8 @@ int autotune_march(tcsem_ctx *c, int nbatch, int nsteps, int *best_by, int *best
 // together: each step's volume march, air-plane chain and surface level run as one launch over all
 // members, from per-member argument records (2 parities of MarchArgs: the member's own fields,
 // tables, sizes and tiling; one UpcontArgs each), with the step scalars of every member in a
-// device table (field-independent, precomputed per call).  Source passes and receiver gathers
-// stay per member.
+// device table (field-independent, precomputed per call).  The source pass is one launch over the
+// members too; receiver gathers stay per member.
 struct tcsem_batch {
   std::vector<tcsem_ctx *> m;
   int prec = 32;
@@ -1670,6 +1689,13 @@ int batch_records(tcsem_batch *b, int by, int pf, MarchArgs<T> &la, UpcontArgs<T
       c->cur = cur0 ^ p;
       MarchArgs<T> a = march_args<T>(c);
       if (!b->active[q]) a.nper = a.sper = 0;  // every workgroup of the member returns at once
+      if (b->active[q] && c->has_fix) {  // the member's source pass (its amplitude from the step table)
+        a.fixe = c->d_fixe;
+        a.fixb = c->d_fixb;
+        a.fixg = c->d_fixg;
+        a.fixidx = c->d_fixidx;
+        a.fixn = 1;
+      }
       rec[(size_t)p * nm + q] = a;
     }
     c->cur = cur0;
@@ -1775,7 +1801,7 @@ int batch_quiesce(tcsem_batch *b) {
 }
 
 // One batch step on the batch's streams, as do_step for one context: lower march of every member,
-// (after the previous step's stream2 work) the top march, the members' source passes; on stream2
+// (after the previous step's stream2 work) the top march, the members' source pass; on stream2
 // the air-plane chain, the surface level and the members' receiver gathers.  par: record parity.
 template <typename T>
 int batch_enqueue(tcsem_batch *b, const MarchArgs<T> &la, const UpcontArgs<T> &ul, int par, int row,
@@ -1806,14 +1832,17 @@ int batch_enqueue(tcsem_batch *b, const MarchArgs<T> &la, const UpcontArgs<T> &u
   HIPCHK(launch_march<T>(lo, 1, st, te[0], te[1], geo));  // one grid row: the work list
   HIPCHK(hipStreamWaitEvent(st, b->xev[1], 0));
   HIPCHK(launch_march<T>(hi, 1, st, te[2], te[3], geo));
-  if (live)
-    for (int q = 0; q < nm; ++q) {
+  if (live) {  // one source pass for every member with a source cell this step
+    bool src = false;
+    for (int q = 0; q < nm && !src; ++q) {
       tcsem_ctx *c = b->m[q];
       if (!b->active[q] || !c->has_fix) continue;
-      MarchArgs<T> a = march_args<T>(c);
+      MarchArgs<T> a{};
       step_scalars<T>(c, a);
-      if (a.amp != 0.0) HIPCHK(launch_src_fix<T>(a, c->d_fixe, c->d_fixidx, c->d_fixb, c->d_fixidx + 2, 1, st));
+      src = a.amp != 0.0;
     }
+    if (src) HIPCHK(launch_src<T>(hi, nm, st, geo));
+  }
   HIPCHK(hipEventRecord(b->xev[0], st));
   HIPCHK(hipStreamWaitEvent(st2, b->xev[0], 0));
   HIPCHK(launch_upcont<T>(ul, nm, st2, te[4], te[5], ugeo));
@@ -2584,8 +2613,9 @@ int tcsem_reset_fields(tcsem_ctx *c, int first, int n) {
 // Lists of the source pass that follows the volume march (levels 1 .. N-1): every E cell of a
 // slot's table once, with the weights of coincident entries summed in table order (the order of
 // the per-cell loop of ElectricalFiled.cu:347-475), then every Bx/By cell whose curl reads one of
-// them (Bderivate.cu:79-200).  Cells outside the update masks are dropped, as the march drops
-// them; level-0 cells stay with the surface kernel.
+// them (Bderivate.cu:79-200), sorted by (component, j, i, k) and cut into one run per component
+// and column.  Cells outside the update masks are dropped, as the march drops them; level-0 cells
+// stay with the surface kernel.
 static int build_fix_lists(tcsem_ctx *c, int nbatch) {
   const int L = c->g.info.L, M = c->g.info.M, N = c->g.info.N;
   auto e_ok = [&](const SrcCell &s) {
@@ -2600,10 +2630,11 @@ static int build_fix_lists(tcsem_ctx *c, int nbatch) {
     return i < L && j >= 1 && j < M;
   };
   std::vector<SrcCell> ecells, bcells;
+  std::vector<int2> runs;
   std::vector<int> idx(4 * nbatch, 0);
   for (int b = 0; b < nbatch; ++b) {
     idx[b] = (int)ecells.size();
-    idx[2 * nbatch + b] = (int)bcells.size();
+    idx[2 * nbatch + b] = (int)runs.size();
     if (!c->tx.empty()) {
       const TxLoc &t = c->tx[c->tx_first + b];
       const size_t e0 = ecells.size(), b0 = bcells.size();
@@ -2643,19 +2674,36 @@ static int build_fix_lists(tcsem_ctx *c, int nbatch) {
           add_b(4, s.k, s.j, s.i - 1);
         }
       }
+      std::sort(bcells.begin() + b0, bcells.end(), [](const SrcCell &x, const SrcCell &y) {
+        if (x.comp != y.comp) return x.comp < y.comp;
+        if (x.j != y.j) return x.j < y.j;
+        if (x.i != y.i) return x.i < y.i;
+        return x.k < y.k;
+      });
+      for (size_t q = b0; q < bcells.size(); ++q) {
+        const SrcCell &x = bcells[q];
+        const bool same = q > b0 && bcells[q - 1].comp == x.comp && bcells[q - 1].j == x.j && bcells[q - 1].i == x.i;
+        if (same)
+          ++runs.back().y;
+        else
+          runs.push_back(make_int2((int)q, 1));
+      }
     }
     idx[nbatch + b] = (int)ecells.size() - idx[b];
-    idx[3 * nbatch + b] = (int)bcells.size() - idx[2 * nbatch + b];
+    idx[3 * nbatch + b] = (int)runs.size() - idx[2 * nbatch + b];
   }
   c->has_fix = !ecells.empty();
   if (ecells.empty()) ecells.push_back({-1, -1, -1, -1, 0.0});
   if (bcells.empty()) bcells.push_back({-1, -1, -1, -1, 0.0});
+  if (runs.empty()) runs.push_back(make_int2(0, 0));
   if (hipMalloc((void **)&c->d_fixe, ecells.size() * sizeof(SrcCell)) != hipSuccess ||
       hipMalloc((void **)&c->d_fixb, bcells.size() * sizeof(SrcCell)) != hipSuccess ||
+      hipMalloc((void **)&c->d_fixg, runs.size() * sizeof(int2)) != hipSuccess ||
       hipMalloc((void **)&c->d_fixidx, idx.size() * sizeof(int)) != hipSuccess)
     return fail(TCSEM_ENOMEM, "source table allocation");
   HIPCHK(hipMemcpy(c->d_fixe, ecells.data(), ecells.size() * sizeof(SrcCell), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_fixb, bcells.data(), bcells.size() * sizeof(SrcCell), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->d_fixg, runs.data(), runs.size() * sizeof(int2), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->d_fixidx, idx.data(), idx.size() * sizeof(int), hipMemcpyHostToDevice));
   return TCSEM_OK;
 }

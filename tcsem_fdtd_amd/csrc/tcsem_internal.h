@@ -89,6 +89,15 @@ struct MarchArgs {
   const double *ibr;
   double *ibw;
   long long ibstride;
+  // Source pass of a source step (levels 1 .. N-1, k_src): per slot the merged E cells (each cell
+  // once, the weights of coincident table entries summed) fixe[fixidx[s] ..] (count
+  // fixidx[fixn + s]), and the Bx/By cells whose curl reads one of them, grouped by column:
+  // fixg[fixidx[2 fixn + s] ..] (count fixidx[3 fixn + s]) holds (first, count) runs of fixb, one
+  // run per (component, j, i), levels ascending.  Null fixe: no source pass.
+  const SrcCell *fixe, *fixb;
+  const int2 *fixg;
+  const int *fixidx;
+  int fixn;
 };
 
 // Receiver gather descriptor (EFInterp2Rx.cu:60-470) for one field group (E or B) of one event.
@@ -187,9 +196,11 @@ hipError_t launch_surface(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEv
                           hipEvent_t e1 = nullptr, const MarchArgs<T> *geo = nullptr);
 template <typename T>
 hipError_t launch_zsum(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEvent_t e0 = nullptr);
+// The source pass of a source step after the march (geo: one workgroup per model-batch member, else
+// per slot); with a.ibw set it moves the chunk sums of B the march just wrote along with the B cells
+// it recomputes.
 template <typename T>
-hipError_t launch_src_fix(const MarchArgs<T> &a, const SrcCell *ecells, const int *eidx, const SrcCell *bcells,
-                          const int *bidx, int nslot, hipStream_t s);
+hipError_t launch_src(const MarchArgs<T> &a, int nslot, hipStream_t s, const MarchArgs<T> *geo = nullptr);
 template <typename T>
 hipError_t launch_bz_volume(const MarchArgs<T> &a, T *bz, int slot, hipStream_t s);
 template <typename T>

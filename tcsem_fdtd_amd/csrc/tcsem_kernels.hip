@@ -254,15 +254,17 @@ __device__ inline MarchArgs<T> slot_args(const MarchArgs<T> &la, const MarchArgs
 }
 
 // IB: the chunked lower launch that accumulates the chunk sums of B (a.ibw set; a.ibr may be).
-// 10-row tiles (FP32) are compiled for five waves per SIMD: at most 96 VGPRs, with a few values
-// spilled to scratch, so that two 10-wave workgroups share a CU (at the compiler's 108 VGPRs only
-// one fits).  Their halo rows are 20% of the waves, against 25% for 8-row tiles.  12-row tiles
-// (FP64) are compiled for three waves per SIMD (168 VGPRs instead of 174-200): one 12-wave
-// workgroup per CU with 17% halo rows, against one 8-wave workgroup with 25%.  Other shapes keep
-// the compiler's register choice.
+// 10-row tiles (FP32) at prefetch depth 1 are compiled for five waves per SIMD: at most 96 VGPRs,
+// with a few values spilled to scratch, so that two 10-wave workgroups share a CU (at the
+// compiler's 108 VGPRs only one fits).  Their halo rows are 20% of the waves, against 25% for
+// 8-row tiles.  Deeper 10-row prefetch keeps the compiler's register choice (about 145 VGPRs, no
+// spills at depth 2): one workgroup per CU with twice the loads in flight, for grids that offer
+// few workgroups per CU.  12-row tiles (FP64) are compiled for three waves per SIMD (168 VGPRs
+// instead of 174-200): one 12-wave workgroup per CU with 17% halo rows, against one 8-wave
+// workgroup with 25%.  Other shapes keep the compiler's register choice.
 template <typename T, int BY, int PF, bool GEO, bool IB>
 __global__ void __launch_bounds__(kMarchLanes *BY)
-__attribute__((amdgpu_waves_per_eu(BY == 10 ? 5 : (BY == 12 ? 3 : 1))))
+__attribute__((amdgpu_waves_per_eu(BY == 10 && PF == 1 ? 5 : (BY == 12 ? 3 : 1))))
 k_march(const MarchArgs<T> la, const MarchArgs<T> *geo) {
   MarchArgs<T> a;
   int tile;
@@ -609,39 +611,67 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_surface(const MarchArgs<T> 
   if (ln.vBy) sto(by, off, byn);
 }
 
-// Source cells of one step at levels 1 .. N-1, applied after the volume march (level 0 stays in
+// Source cells of a source step at levels 1 .. N-1, after the volume march (level 0 stays with
 // k_surface).  First the E cells take E - amp w (ElectricalFiled.cu:347-475; the weights of
-// coincident cells are summed in table order, as the per-cell loop of the reference does), then
-// every Bx/By cell whose curl reads one of them is recomputed from B_old and the patched E with the
-// march's own expression (Bderivate.cu:79-200).  One workgroup per slot; the host lists each cell
-// once, so no two threads touch one cell.
-template <typename T>
-__global__ void k_src_e(const MarchArgs<T> a, const SrcCell *cells, const int *first, const int *count) {
-  const int slot = blockIdx.x;
-  const long long fo = (long long)slot * a.fstride;
+// coincident table entries are summed in table order, the order in which the reference's per-cell
+// loop adds them), then every Bx/By cell whose curl reads one of them is recomputed from B_old and
+// the patched E with the march's own expression (Bderivate.cu:79-200).  A chunked lower launch has
+// already added dz(k) B(k) of these cells to its chunk sums (a.ibw set): each recomputed cell moves
+// its chunk's sum by dz(k) (B_new - B_march), so that the next step may still start from the sums.
+// One workgroup per slot (GEO: per model-batch member).  The host lists each cell once and the B
+// cells of one (component, column) as one run, levels ascending, which one thread walks: no two
+// threads touch one cell or one chunk sum.
+template <typename T, bool GEO>
+__global__ void __launch_bounds__(256) k_src(const MarchArgs<T> la, const MarchArgs<T> *geo) {
+  MarchArgs<T> a;
+  int s;
+  if constexpr (GEO) {
+    a = slot_args(la, geo[blockIdx.x], (int)blockIdx.x);
+    s = 0;  // a member's lists are those of its slot 0
+  } else {
+    a = la;
+    s = (int)blockIdx.x;
+  }
+  if (!a.fixe || a.amp == 0.0) return;
+  const int n = a.fixn;
+  const long long PS = a.PS, PX = a.PX;
+  const long long fo = GEO ? 0LL : (long long)s * a.fstride, po = GEO ? 0LL : (long long)s * a.pstride;
   T *const f[3] = {a.ex + fo, a.ey + fo, a.ez + fo};
-  for (int c = threadIdx.x; c < count[slot]; c += blockDim.x) {
-    const SrcCell sc = cells[first[slot] + c];
-    T *e = f[sc.comp] + (long long)sc.k * a.PS + (long long)sc.j * a.PX + sc.i;
+  const int e0 = a.fixidx[s], ne = a.fixidx[n + s];
+  for (int q = threadIdx.x; q < ne; q += blockDim.x) {
+    const SrcCell sc = a.fixe[e0 + q];
+    T *e = f[sc.comp] + (long long)sc.k * PS + (long long)sc.j * PX + sc.i;
     *e = (T)((double)*e - a.amp * sc.w);
   }
-}
-template <typename T>
-__global__ void k_src_b(const MarchArgs<T> a, const SrcCell *cells, const int *first, const int *count) {
-  const int slot = blockIdx.x;
-  const long long fo = (long long)slot * a.fstride;
+  __syncthreads();  // workgroup-scope release/acquire: the patched E is visible to every thread
   const T dtb = (T)a.dtb;
-  const long long PS = a.PS, PX = a.PX;
-  for (int c = threadIdx.x; c < count[slot]; c += blockDim.x) {
-    const SrcCell sc = cells[first[slot] + c];
-    const long long o = fo + (long long)sc.k * PS + (long long)sc.j * PX + sc.i;
-    const T rdzk = a.rdz[sc.k];
-    if (sc.comp == 3) {
-      const T rdy = a.rdy[sc.j + 1];
-      a.bx[o] = a.bxi[o] + dtb * ((a.ey[o + PS] - a.ey[o]) * rdzk - (a.ez[o + PX] - a.ez[o]) * rdy);
-    } else {
-      const T rdx = a.rdx[sc.i + 1];
-      a.by[o] = a.byi[o] + dtb * ((a.ez[o + 1] - a.ez[o]) * rdx - (a.ex[o + PS] - a.ex[o]) * rdzk);
+  const T *ex = f[0], *ey = f[1], *ez = f[2];
+  const T *bxi = a.bxi + fo, *byi = a.byi + fo;
+  T *bx = a.bx + fo, *by = a.by + fo;
+  const int g0 = a.fixidx[2 * n + s], ng = a.fixidx[3 * n + s];
+  for (int g = threadIdx.x; g < ng; g += blockDim.x) {
+    const int2 run = a.fixg[g0 + g];
+    for (int q = run.x; q < run.x + run.y; ++q) {
+      const SrcCell sc = a.fixb[q];
+      const long long o = (long long)sc.k * PS + (long long)sc.j * PX + sc.i;
+      const T rdzk = a.rdz[sc.k];
+      T old, nw;
+      if (sc.comp == 3) {
+        old = bx[o];
+        nw = bxi[o] + dtb * ((ey[o + PS] - ey[o]) * rdzk - (ez[o + PX] - ez[o]) * a.rdy[sc.j + 1]);
+        bx[o] = nw;
+      } else {
+        old = by[o];
+        nw = byi[o] + dtb * ((ez[o + 1] - ez[o]) * a.rdx[sc.i + 1] - (ex[o + PS] - ex[o]) * rdzk);
+        by[o] = nw;
+      }
+      if (a.ibw && sc.k >= 2) {  // levels 2 .. N-1 belong to the lower launch's chunks
+        const int c = (a.k_hi - sc.k) / a.chunk_len;
+        if (c < a.nchunk - 1) {
+          double *p = a.ibw + po + (long long)(2 * c + (sc.comp == 4)) * a.ibstride + (long long)sc.j * PX + sc.i;
+          *p += (double)a.dzt[sc.k] * ((double)nw - (double)old);
+        }
+      }
     }
   }
 }
@@ -871,12 +901,11 @@ hipError_t launch_surface(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEv
   return hipGetLastError();
 }
 template <typename T>
-hipError_t launch_src_fix(const MarchArgs<T> &a, const SrcCell *ecells, const int *eidx, const SrcCell *bcells,
-                          const int *bidx, int nslot, hipStream_t s) {
-  hipLaunchKernelGGL((k_src_e<T>), dim3(nslot), dim3(64), 0, s, a, ecells, eidx, eidx + nslot);
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return e;
-  hipLaunchKernelGGL((k_src_b<T>), dim3(nslot), dim3(64), 0, s, a, bcells, bidx, bidx + nslot);
+hipError_t launch_src(const MarchArgs<T> &a, int nslot, hipStream_t s, const MarchArgs<T> *geo) {
+  if (geo)
+    hipLaunchKernelGGL((k_src<T, true>), dim3(nslot), dim3(256), 0, s, a, geo);
+  else
+    hipLaunchKernelGGL((k_src<T, false>), dim3(nslot), dim3(256), 0, s, a, geo);
   return hipGetLastError();
 }
 template <typename T>
@@ -919,10 +948,8 @@ template hipError_t launch_surface<float>(const MarchArgs<float> &, int, hipStre
                                           const MarchArgs<float> *);
 template hipError_t launch_surface<double>(const MarchArgs<double> &, int, hipStream_t, hipEvent_t, hipEvent_t,
                                            const MarchArgs<double> *);
-template hipError_t launch_src_fix<float>(const MarchArgs<float> &, const SrcCell *, const int *, const SrcCell *,
-                                          const int *, int, hipStream_t);
-template hipError_t launch_src_fix<double>(const MarchArgs<double> &, const SrcCell *, const int *, const SrcCell *,
-                                           const int *, int, hipStream_t);
+template hipError_t launch_src<float>(const MarchArgs<float> &, int, hipStream_t, const MarchArgs<float> *);
+template hipError_t launch_src<double>(const MarchArgs<double> &, int, hipStream_t, const MarchArgs<double> *);
 template hipError_t launch_zsum<float>(const MarchArgs<float> &, int, hipStream_t, hipEvent_t);
 template hipError_t launch_zsum<double>(const MarchArgs<double> &, int, hipStream_t, hipEvent_t);
 template hipError_t launch_bz_volume<float>(const MarchArgs<float> &, float *, int, hipStream_t);

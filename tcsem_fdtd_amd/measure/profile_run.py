@@ -25,7 +25,8 @@ def main():
     import tcsem_fdtd_amd as T
     from tcsem_fdtd_amd import workloads as W
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", default="marine_3d", choices=["marine_1d", "marine_3d", "towline", "uneven"])
+    ap.add_argument("--workload", default="marine_3d",
+                    choices=["marine_1d", "marine_3d", "towline", "large_5", "uneven"])
     ap.add_argument("--models", type=int, default=64, help="uneven: number of models in the model batch")
     ap.add_argument("--precision", type=int, default=32, choices=[32, 64])
     ap.add_argument("--batch", type=int, default=1)

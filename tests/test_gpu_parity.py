@@ -300,6 +300,28 @@ def test_batch_with_z_chunks_equals_independent_runs_fp64(zsum, monkeypatch):
         assert np.all(err < 1e-9), (b, err)
 
 
+@pytest.mark.parametrize("zsum", ["fused", "pass"])
+@pytest.mark.parametrize("precision,tol", [(T.FP64, 1e-9), (T.FP32, FP32_TRACE_TOL)])
+def test_source_pass_through_a_chunked_march(zsum, precision, tol, monkeypatch):
+    """The bipole cells of every source step are patched by the pass after the march, here through
+    a lower march cut into three z-chunks of six levels (19-14, 13-8, 7-2).  "fused": each step's
+    chunks start from the chunk sums of B the previous step wrote, which the source pass moves
+    along with the B cells it recomputes; "pass": from a k_zsum pass over the fields.  The sources
+    sit in every chunk: an x-dipole at level 9-10, a vertical bipole across the boundary of the two
+    deepest chunks, an ort-5 pair with one bipole in the middle chunk and one in the shallowest;
+    the 3-transmitter batch equals the oracle's independent runs (ElectricalFiled.cu:347-641)."""
+    monkeypatch.setenv("TCSEM_MARCH_ZCHUNKS", "3")
+    monkeypatch.setenv("TCSEM_MARCH_ZSUM", zsum)
+    txs = [dict(TX_X, x=-100.0, z=500.0, txid=1),
+           dict(ort=3, bipolen=100.0, x=0.0, y=0.0, z=700.0, rx_ids=[4, 3, 2, 1], txid=2),
+           dict(ort=5, bipolen=150.0, x=25.0, y=-10.0, z=560.0, azi=30.0, dip=20.0, bipolen1=100.0,
+                x1=-40.0, y1=20.0, z1=320.0, azi1=-45.0, dip1=10.0, rx_ids=[2, 3], txid=3)]
+    ref, got = _deck_run(precision, txs, nsteps=250)
+    for b in range(3):
+        err = decks.trace_error(got[b], ref[b])
+        assert np.all(err < tol), (b, err)
+
+
 def test_batch_graph_replays_equal_plain_steps():
     """Graph replays with three transmitter slots: the same traces as the plain steps."""
     txs = [dict(TX_X, x=x, txid=i + 1) for i, x in enumerate((-100.0, 0.0, 120.0))]
