@@ -194,6 +194,18 @@ int tcsem_set_field(tcsem_ctx *ctx, int slot, int comp, const double *in);
 /* UpContinuation_gpu on a caller plane: bz0 (M*L) -> bxair (M*(L+1)), byair ((M+1)*L). */
 int tcsem_upcont_plane(tcsem_ctx *ctx, const double *bz0, double *bxair, double *byair);
 
+/* Allocation-free form of tcsem_upcont_plane (no counterpart in the reference): for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc.  Same kernels and same results as tcsem_upcont_plane; the planes are the
+ * caller's device buffers in the context's field precision (double for TCSEM_FP64, float for
+ * TCSEM_FP32), dense rows: d_bz0 M*L, d_bxair M*(L+1), d_byair (M+1)*L.  The work planes live in
+ * a fixed device array of the library (40 MiB per device, held until the process ends); air
+ * planes whose work exceeds it, or whose FFT lengths exceed 2048, return TCSEM_ESIZE.  The call
+ * runs on the context's stream, allocates nothing, and returns once the planes are written.  All
+ * calls on one device share that array: calls that overlap in time on one device (from several
+ * host threads) race. */
+int tcsem_upcont_plane_noalloc(tcsem_ctx *ctx, const void *d_bz0, void *d_bxair, void *d_byair);
+
 /* Average device time [ms] of the kernels of the steps taken while timing was enabled, from HIP
  * events that the kernels' dispatch packets record on the context's streams as the steps run
  * (every 8th step; no extra synchronisation, reading waits for them): t[0] volume march per step

@@ -109,6 +109,7 @@ def lib():
         "tcsem_get_field": (C.c_int, [P, C.c_int, C.c_int, dp]),
         "tcsem_set_field": (C.c_int, [P, C.c_int, C.c_int, dp]),
         "tcsem_upcont_plane": (C.c_int, [P, dp, dp, dp]),
+        "tcsem_upcont_plane_noalloc": (C.c_int, [P, P, P, P]),
         "tcsem_timing": (C.c_int, [P, dp, C.POINTER(C.c_long)]),
         "tcsem_enable_timing": (C.c_int, [P, C.c_int]),
         "tcsem_sync": (C.c_int, [P]),

@@ -387,6 +387,7 @@ void order_pair(int &a, int &b) {  // LoadRxTxPos.cpp:297-302
 // ---- context -------------------------------------------------------------------------------------
 struct tcsem_ctx {
   int prec = 64;
+  int dev = 0;  // the device current at tcsem_create
   size_t esz = 8;
   tcsem_setup setup{};
   HostGrid g;
@@ -660,6 +661,7 @@ UpcontArgs<T> upcont_args(tcsem_ctx *c) {
   u.L = I.L;
   u.M = I.M;
   u.PX = c->PX;
+  u.pz = u.pxa = u.pya = c->PX;
   u.L1 = I.L1;
   u.M1 = I.M1;
   u.AnLx = I.AnLx_New;
@@ -2240,7 +2242,11 @@ int tcsem_create(const tcsem_setup *s, const double *cond, int precision, tcsem_
   HIPCHK(hipGetDeviceProperties(&prop, dev));
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     return fail(TCSEM_EDEVICE, "device %s is not gfx950 (MI355X); this build targets gfx950 only", prop.gcnArchName);
+  // the allocation-free entry point's work planes on this device (looked up once per device)
+  if (hipError_t e = upcont_scratch_init(dev); e != hipSuccess)
+    return fail(TCSEM_EDEVICE, "scratch address on device %d: %s", dev, hipGetErrorString(e));
   tcsem_ctx *c = new tcsem_ctx();
+  c->dev = dev;
   c->prec = precision;
   c->esz = precision == TCSEM_FP64 ? 8 : 4;
   c->setup = *s;
@@ -2993,6 +2999,52 @@ int tcsem_upcont_plane(tcsem_ctx *c, const double *bz0, double *bxair, double *b
   int rc = tcsem_get_field(c, 0, TCSEM_BXAIR, bxair);
   if (rc) return rc;
   return tcsem_get_field(c, 0, TCSEM_BYAIR, byair);
+}
+
+extern "C++" {
+namespace {
+// The tiled chain of tcsem_upcont_plane on caller device planes (dense rows), with its work planes
+// (one slot's A, G, Bx and By regions, the layout of the context's own work buffer) in the
+// library's scratch array at scr.  Every work entry the chain reads is written earlier in the same
+// call, so nothing is cleared.
+template <typename T>
+hipError_t upcont_on_planes(tcsem_ctx *c, void *scr, const void *bz0, void *bxair, void *byair) {
+  UpcontArgs<T> u = up_of<T>(c);
+  const int L = c->g.info.L;
+  u.bz0 = (const T *)bz0;
+  u.bxair = (T *)bxair;
+  u.byair = (T *)byair;
+  u.pstride = 0;
+  u.pz = L;
+  u.pxa = L + 1;
+  u.pya = L;
+  T *w = (T *)scr;
+  u.A = w;
+  u.G = w + 2 * c->sA;
+  u.tbx = w + 2 * (c->sA + c->sG);
+  u.tby = u.tbx + c->sX;
+  hipError_t e = launch_upcont<T>(u, 1, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  return e;
+}
+}  // namespace
+}  // extern "C++"
+
+int tcsem_upcont_plane_noalloc(tcsem_ctx *c, const void *d_bz0, void *d_bxair, void *d_byair) {
+  if (check_ctx(c) || !d_bz0 || !d_bxair || !d_byair) return fail(TCSEM_EINVAL, "null argument");
+  if (!c->up_tiled)
+    return fail(TCSEM_ESIZE, "air plane %d x %d: FFT lengths above 2048 have no allocation-free path", c->g.info.L1,
+                c->g.info.M1);
+  const size_t need = (size_t)(2 * c->sA + 2 * c->sG + 2 * c->sX) * c->esz;
+  if (need > kUpcontScratchBytes)
+    return fail(TCSEM_ESIZE, "air plane %d x %d needs %zu bytes of work planes; the allocation-free limit is %zu",
+                c->g.info.L1, c->g.info.M1, need, kUpcontScratchBytes);
+  void *scr = upcont_scratch(c->dev);
+  if (!scr) return fail(TCSEM_EDEVICE, "no scratch address for device %d", c->dev);
+  const hipError_t e = c->prec == 64 ? upcont_on_planes<double>(c, scr, d_bz0, d_bxair, d_byair)
+                                     : upcont_on_planes<float>(c, scr, d_bz0, d_bxair, d_byair);
+  if (e != hipSuccess) return fail(TCSEM_EDEVICE, "upward continuation: %s", hipGetErrorString(e));
+  return TCSEM_OK;
 }
 
 int tcsem_enable_timing(tcsem_ctx *c, int on) {

@@ -134,10 +134,11 @@ struct FftPlan {
 
 template <typename T>
 struct UpcontArgs {
-  const T *bz0;            // plane origin (slot 0), pitch PX
-  T *bxair, *byair;        // plane origins (slot 0)
+  const T *bz0;            // plane origin (slot 0), row pitch pz
+  T *bxair, *byair;        // plane origins (slot 0), row pitches pxa, pya
   long long pstride;
   int L, M, PX, L1, M1, AnLx, AnRx, Any, AL, AY, Nx, Ny;
+  int pz, pxa, pya;        // row pitches of bz0, bxair, byair (PX for the context's own planes)
   int n1, n2, n3;          // spline window lengths
   const double *wl, *wr;   // x-window weights, tap-major: n1 x AnLx, n2 x AnRx
   const double *wf, *wb;   // y-window weights: Any x n3 (front, back)
@@ -212,6 +213,12 @@ template <typename T>
 hipError_t launch_upcont(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEvent_t e0 = nullptr,
                          hipEvent_t e1 = nullptr, const UpcontArgs<T> *geo = nullptr);
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);
+// Work planes of the allocation-free upward continuation: a file-scope device array of
+// kUpcontScratchBytes per device (tcsem_upcont.hip).  upcont_scratch_init looks its address up on
+// device dev (current) once; upcont_scratch returns that address (null before the look-up).
+constexpr size_t kUpcontScratchBytes = (size_t)40 << 20;
+hipError_t upcont_scratch_init(int dev);
+void *upcont_scratch(int dev);
 // Step scalars of a graph replay: dst[0 .. 4 nsteps) = the values (lam, dt, dtb, amp per step).
 constexpr int kGraphMaxSteps = 32;
 struct GraphScalars {

@@ -46,6 +46,9 @@
 // the twiddle table staged in LDS, one wavefront per line; longer ones run in-place radix-2 with
 // 256 threads per line and the table in global memory.  Lengths that are not powers of two use
 // Bluestein's chirp-z algorithm on a power-of-two length.  Tables are FP64 from the host.
+#include <mutex>
+#include <vector>
+
 #include "tcsem_internal.h"
 
 namespace tcsem {
@@ -277,7 +280,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_up_xpass(const UpcontArgs<T> u) {
   extern __shared__ double srow[];
   const int j = blockIdx.x, slot = blockIdx.y;
-  const T *row = u.bz0 + (long long)slot * u.pstride + (long long)j * u.PX;
+  const T *row = u.bz0 + (long long)slot * u.pstride + (long long)j * u.pz;
   for (int i = threadIdx.x; i < u.L; i += blockDim.x) srow[i] = (double)row[i];
   __syncthreads();
   double *trow = u.tx + (long long)slot * u.wstride + (long long)j * u.L1;
@@ -476,8 +479,8 @@ __global__ void __launch_bounds__(256) k_up_rows_fwd(const UpcontArgs<T> u0, con
   T *s0 = (T *)b, *s1 = s0 + L;
   const T *plane = u.bz0 + (long long)slot * u.pstride;
   for (int i = lane; i < L; i += W) {
-    s0[i] = j0 < M ? plane[(long long)j0 * u.PX + i] : (T)0;
-    s1[i] = j0 + 1 < M ? plane[(long long)(j0 + 1) * u.PX + i] : (T)0;
+    s0[i] = j0 < M ? plane[(long long)j0 * u.pz + i] : (T)0;
+    s1[i] = j0 + 1 < M ? plane[(long long)(j0 + 1) * u.pz + i] : (T)0;
   }
   __syncthreads();
   for (int c = lane; c < L1; c += W) a[c] = mkc<V>(resample_x(u, s0, c), resample_x(u, s1, c));
@@ -643,7 +646,7 @@ __global__ void k_up_bilin(const UpcontArgs<T> u0, const B *fx0, const B *fy0, c
     const double a0 = (1.0 - l1) * (double)F[(long long)(JJ - 1) * XR + II - 1] + l1 * (double)F[(long long)(JJ - 1) * XR + II];
     const double a1 = (1.0 - l1) * (double)F[(long long)JJ * XR + II - 1] + l1 * (double)F[(long long)JJ * XR + II];
     l1 = (u.Y[iy] - u.Yn[JJ - 1]) / (u.Yn[JJ] - u.Yn[JJ - 1]);
-    u.bxair[po + (long long)iy * u.PX + ix] = (T)(((1.0 - l1) * a0 + l1 * a1) / norm);
+    u.bxair[po + (long long)iy * u.pxa + ix] = (T)(((1.0 - l1) * a0 + l1 * a1) / norm);
   } else if (t < nbx + (M0 + 1) * L0) {
     const int tt = t - nbx;
     const int iy = tt / L0, ix = tt % L0;
@@ -654,7 +657,7 @@ __global__ void k_up_bilin(const UpcontArgs<T> u0, const B *fx0, const B *fy0, c
     const double a0 = (1.0 - l1) * (double)F[(long long)(JJ - 1) * XR + II - 1] + l1 * (double)F[(long long)(JJ - 1) * XR + II];
     const double a1 = (1.0 - l1) * (double)F[(long long)JJ * XR + II - 1] + l1 * (double)F[(long long)JJ * XR + II];
     l1 = (Yp - u.Yn[JJ - 1]) / (u.Yn[JJ] - u.Yn[JJ - 1]);
-    u.byair[po + (long long)iy * u.PX + ix] = (T)(((1.0 - l1) * a0 + l1 * a1) / norm);
+    u.byair[po + (long long)iy * u.pya + ix] = (T)(((1.0 - l1) * a0 + l1 * a1) / norm);
   }
 }
 
@@ -793,5 +796,36 @@ template hipError_t launch_upcont<float>(const UpcontArgs<float> &, int, hipStre
                                          const UpcontArgs<float> *);
 template hipError_t launch_upcont<double>(const UpcontArgs<double> &, int, hipStream_t, hipEvent_t, hipEvent_t,
                                           const UpcontArgs<double> *);
+
+// ---- work planes of the allocation-free upward continuation ------------------------------------
+// The tiled chain's work planes (A, G and the regular-plane Bx, By of one slot) for callers that
+// cannot allocate live in this file-scope device array; every device that runs the library holds
+// its own copy until the process ends.  In host code the array's name is not a device address:
+// its per-device address comes from hipGetSymbolAddress, looked up once per device by
+// upcont_scratch_init (tcsem_create calls it with the context's device current), and only that
+// address is passed to kernels.
+static __device__ __align__(256) unsigned char scratch[kUpcontScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::vector<void *> g_scratch;  // device address of `scratch`, per device id
+}  // namespace
+
+hipError_t upcont_scratch_init(int dev) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (dev < 0) return hipErrorInvalidDevice;
+  if ((int)g_scratch.size() <= dev) g_scratch.resize(dev + 1, nullptr);
+  if (g_scratch[dev]) return hipSuccess;
+  void *p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch[dev] = p;
+  return hipSuccess;
+}
+
+void *upcont_scratch(int dev) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return dev >= 0 && dev < (int)g_scratch.size() ? g_scratch[dev] : nullptr;
+}
 
 }  // namespace tcsem
