@@ -18,6 +18,8 @@ library's autotuner pick the march tiling for its batch size.
     python bench.py --accuracy   # complete batched runs, FP32 vs FP64 trace errors
 """
 import argparse
+import contextlib
+import ctypes
 import json
 import os
 import sys
@@ -32,6 +34,31 @@ WORDS_PER_CELL = 13            # read Ex Ey Ez Bx By Bz sigma, write the six fie
 # LDS rate assumed for the air-plane FFT figures: 128 B per clock per CU (the CDNA LDS read rate)
 # x 256 CUs x 2.4 GHz
 PEAK_LDS_GBS = 128 * 256 * 2.4
+
+
+_roctx = []
+
+
+@contextlib.contextmanager
+def timed_range(label):
+    """A rocTX range around a timed region, which `rocprofv3 --marker-trace` records next to the
+    kernel dispatches (so a trace of this command can be cut to the timed steps); a no-op when the
+    rocTX library is absent."""
+    if not _roctx:
+        try:
+            lib = ctypes.CDLL("librocprofiler-sdk-roctx.so.1")
+            lib.roctxRangePushA.argtypes = [ctypes.c_char_p]
+            _roctx.append(lib)
+        except OSError:
+            _roctx.append(None)
+    lib = _roctx[0]
+    if lib is not None:
+        lib.roctxRangePushA(label.encode())
+    try:
+        yield
+    finally:
+        if lib is not None:
+            lib.roctxRangePop()
 
 
 def gpu_run(T, deck, precision, nbatch, steps, warmup, tune=True):
@@ -51,10 +78,11 @@ def gpu_run(T, deck, precision, nbatch, steps, warmup, tune=True):
         # kernel times come from HIP events recorded on the library's streams inside the timed
         # region (read after it)
         c.enable_timing(True)
-        t0 = time.perf_counter()
-        c.step(steps)
-        c.sync()
-        t1 = time.perf_counter()
+        with timed_range("timed %s fp%d x%d" % (deck.name, precision, nbatch)):
+            t0 = time.perf_counter()
+            c.step(steps)
+            c.sync()
+            t1 = time.perf_counter()
         tm = c.timing()
         c.enable_timing(False)
         ex = c.get(T.EX, 0)
@@ -68,11 +96,13 @@ def gpu_run(T, deck, precision, nbatch, steps, warmup, tune=True):
     }
 
 
-def full_run(T, deck, precision, graph=0):
+def full_run(T, deck, precision, graph=None):
     """A complete run (tcsem_run: every step until the last sample, receivers and all), timed by the
-    wall clock around it; graph > 0 replays sample-free stretches of that many steps as graphs."""
+    wall clock around it; graph None keeps the library's default (graph replays of 16 steps for
+    grids under 2^21 cells), 0 runs plain steps, k > 0 replays sample-free stretches of k steps."""
     with T.Context(T.make_setup(**deck.setup), deck.cond, precision) as c:
-        c.set_graph(graph)
+        if graph is not None:
+            c.set_graph(graph)
         r = deck.rxtime
         c.set_receivers(T.make_rxtime(r["T0"], r["Tn"], r["Nt"], r["is_log"], r["comps"]), deck.rx)
         c.set_transmitters([T.make_tx(**dict(t, txid=i + 1)) for i, t in enumerate(deck.txs)])
@@ -85,8 +115,8 @@ def full_run(T, deck, precision, graph=0):
         replays = c.graph_replays()
     out = {"gcell_s": round(cells * steps / sec / 1e9, 3), "ms_per_step": round(1000.0 * sec / steps, 5),
            "steps": steps, "seconds": round(sec, 4)}
-    if graph:
-        out.update(graph_steps=graph, graph_replays_per_run=replays // 2)
+    if replays:
+        out.update(graph_replays_per_run=replays // 2)
     return out
 
 
@@ -124,10 +154,11 @@ def models_run(T, decks, precision, steps, warmup, tune=True, single_steps=0):
             b.step(warmup)
             b.sync()
             b.enable_timing(True)
-            t0 = time.perf_counter()
-            b.step(steps)
-            b.sync()
-            sec = time.perf_counter() - t0
+            with timed_range("timed model batch fp%d x%d" % (precision, len(ctxs))):
+                t0 = time.perf_counter()
+                b.step(steps)
+                b.sync()
+                sec = time.perf_counter() - t0
             out["timing"] = b.timing()
             b.enable_timing(False)
             out["ex"] = np.concatenate([c.get(T.EX, 0).ravel() for c in ctxs])
@@ -329,13 +360,13 @@ def cpu_baseline(W, T, window=50):
     close.  The small 64 x 64 x 48 model complete; the 256 x 256 x 160 and 512 x 512 x 256 models
     over `window` steps after 5 warm-up steps.  `value` is the 512 x 512 x 256 window (the
     headline's workload)."""
+    host = host_info()  # before the first OpenMP region binds this thread to one core
     os.environ.setdefault("OMP_PROC_BIND", "close")
     os.environ.setdefault("OMP_PLACES", "cores")
     from oracle import oracle as O
     out = {"marine_1d_complete": cpu_full_run(O, W.marine_1d(T)),
            "marine_3d_window": cpu_window(O, W.marine_3d(T), window),
            "large_5_window": cpu_window(O, W.large_5(T), window)}
-    host = host_info()
     return {"value": out["large_5_window"]["gcell_s"], "unit": "Gcell-updates/s",
             "cores": O.threads(), "kind": "port",
             "sample": "FP64 oracle (C/OpenMP, OMP_PROC_BIND=%s), CPU-formulation air boundary: "
@@ -486,7 +517,7 @@ def main():
     # per step)
     small = W.marine_1d(T)
     s32, s64 = full_run(T, small, T.FP32), full_run(T, small, T.FP64)
-    s32g = full_run(T, small, T.FP32, graph=16)
+    s32p = full_run(T, small, T.FP32, graph=0)
     line = {
         "metric": METRIC,
         "value": round(rl32["gcell_s"], 3),
@@ -539,8 +570,9 @@ def main():
                                    "roofline": models_roofline(m64, 8), "march_tiling": m64["tiling"],
                                    "kernel_ms": kern(m64)},
                           "fp32_vs_fp64_ex_rel_l2_after_window": models_err},
-        "marine_1d": {"workload": "marine_1d 64x64x48, 1 x-dipole, 21 receivers, complete run (tcsem_run)",
-                      "fp32": s32, "fp64": s64, "fp32_graph_replays": s32g},
+        "marine_1d": {"workload": "marine_1d 64x64x48, 1 x-dipole, 21 receivers, complete run (tcsem_run, "
+                                  "graph replays of 16 steps: the library's default for this grid)",
+                      "fp32": s32, "fp64": s64, "fp32_plain_steps": s32p},
         # the north-star ratio: FP32 transmitter batch against the plain FP64 single model (the
         # 256 x 256 x 160 grid they share, and the large model's FP64 window)
         "ratio_fp32_batched_vs_fp64": round(rb["gcell_s"] / r64["gcell_s"], 3),

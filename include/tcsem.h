@@ -221,7 +221,8 @@ int tcsem_sync(tcsem_ctx *ctx);
  * steps after the source wavelet that holds no receiver sample as one replay of a HIP graph.
  * The graph is captured on first use for its starting state; the step scalars are written to
  * device memory before each replay.  Results equal those of the plain steps (tested).  It pays
- * where the host's launches bound the step (small grids); 0 turns it off (the default).
+ * where the host's launches bound the step (small grids): contexts of fewer than 2^21 cells start
+ * with 16, larger ones with 0 (off); 0 turns it off.
  * tcsem_graph_replays counts the replays of the context so far. */
 int tcsem_set_graph(tcsem_ctx *ctx, int nsteps);
 long tcsem_graph_replays(const tcsem_ctx *ctx);

@@ -378,7 +378,8 @@ class Context:
         _check(self.L_.tcsem_sync(self.h))
 
     def set_graph(self, nsteps):
-        """Replay stretches of nsteps sample-free steady-state steps as HIP graphs (0: off)."""
+        """Replay stretches of nsteps sample-free steady-state steps as HIP graphs (0: off; contexts of
+        fewer than 2^21 cells start with 16)."""
         _check(self.L_.tcsem_set_graph(self.h, nsteps))
 
     def graph_replays(self):

@@ -494,12 +494,7 @@ struct tcsem_ctx {
 
 namespace {
 
-// Workgroup height of the volume march.  Tall tiles waste fewer lanes on the +-1 halo rows but
-// give fewer workgroups: 16 rows (14 owned) once the batch offers at least two workgroups per CU,
-// else 4 rows (2 owned), which measured best for one 256 x 256 model on MI355X (4-16 rows swept);
-// FP64 fields stay at 8 rows or fewer (16 rows would spill registers).  TCSEM_MARCH_ROWS
-// overrides the choice (4, 8 or 16).  The prefetch depth (levels of loads in flight per lane) is
-// 2 for FP32 fields in short tiles and 1 otherwise; TCSEM_MARCH_PF (1 .. 4) overrides it.
+// Compute units of the current device (256 on MI355X): the tiling heuristics size their grids by it.
 int device_cus() {
   int cu = 256, dev = 0;
   hipDeviceProp_t prop;
@@ -508,13 +503,18 @@ int device_cus() {
   return cu;
 }
 
+// Rows per march tile that the precision is built for: FP64 4, 8 or 12 (16 do not fit its
+// registers), FP32 4, 8, 10 or 16; other counts run as 8.
+int supported_rows(int prec, int by) {
+  if (prec == 64) return by == 4 || by == 12 ? by : 8;
+  return by == 4 || by == 10 || by == 16 ? by : 8;
+}
+
 // Launch geometry of a march tiling: `by` rows per workgroup (by - 2 owned), prefetch depth pf,
 // nch z-chunks of the lower launch (1: one pass).
 void apply_tiling(tcsem_ctx *c, int by, int pf, int nch) {
   const int L = c->g.info.L, M = c->g.info.M, N = c->g.info.N, nlev = N - 2;
-  // rows per tile: FP64 4, 8 or 12 (16 do not fit its registers), FP32 4, 8, 10 or 16
-  if (c->prec == 64 && by != 4 && by != 12) by = 8;
-  if (c->prec == 32 && by == 12) by = 8;
+  by = supported_rows(c->prec, by);
   c->by = by;
   c->pf = pf;
   nch = std::max(1, std::min({nch, kMaxChunks, N >= 6 ? nlev : 1}));
@@ -570,6 +570,9 @@ void choose_tiling(tcsem_ctx *c, int nbatch) {
   if (const char *e = std::getenv("TCSEM_MARCH_ROWS")) {
     const int v = std::atoi(e);
     if (v == 4 || v == 8 || v == 10 || v == 12 || v == 16) by = v;
+    if (supported_rows(c->prec, by) != by)
+      std::fprintf(stderr, "tcsem: TCSEM_MARCH_ROWS=%d has no FP%d march; running %d rows\n", by, c->prec,
+                   supported_rows(c->prec, by));
   }
   if (const char *e = std::getenv("TCSEM_MARCH_PF")) {
     const int v = std::atoi(e);
@@ -1665,6 +1668,9 @@ void batch_tiling(const tcsem_batch *b, int &by, int &pf) {
   if (const char *e = std::getenv("TCSEM_MARCH_ROWS")) {
     const int v = std::atoi(e);
     if (v == 4 || v == 8 || v == 10 || v == 12 || v == 16) by = v;
+    if (supported_rows(b->prec, by) != by)
+      std::fprintf(stderr, "tcsem: TCSEM_MARCH_ROWS=%d has no FP%d march; running %d rows\n", by, b->prec,
+                   supported_rows(b->prec, by));
   }
   if (const char *e = std::getenv("TCSEM_MARCH_PF")) {
     const int v = std::atoi(e);
@@ -2479,6 +2485,12 @@ int tcsem_create(const tcsem_setup *s, const double *cond, int precision, tcsem_
     if (hipMalloc((void **)&c->d_srcidx, 2 * sizeof(int)) != hipSuccess) return bail(fail(TCSEM_ENOMEM, "alloc"));
     if (hipMemcpy(c->d_srcidx, z.data(), 2 * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
       return bail(fail(TCSEM_EDEVICE, "upload"));
+  }
+  // graph replays by default where the host's launches bound a step (grids under 2^21 cells)
+  if ((long long)L * M * N < kGraphAutoCells) {
+    if (hipMalloc((void **)&c->d_gs, (size_t)4 * kGraphMaxSteps * sizeof(double)) != hipSuccess)
+      return bail(fail(TCSEM_ENOMEM, "graph scalar allocation"));
+    c->graph_k = kGraphAutoSteps;
   }
   *out = c;
   return TCSEM_OK;

@@ -221,6 +221,10 @@ hipError_t upcont_scratch_init(int dev);
 void *upcont_scratch(int dev);
 // Step scalars of a graph replay: dst[0 .. 4 nsteps) = the values (lam, dt, dtb, amp per step).
 constexpr int kGraphMaxSteps = 32;
+// Contexts of fewer cells start with graph replays of kGraphAutoSteps steps (their steps are bound
+// by the host's launches, not by the device).
+constexpr long long kGraphAutoCells = 1LL << 21;
+constexpr int kGraphAutoSteps = 16;
 struct GraphScalars {
   double v[4 * kGraphMaxSteps];
 };

@@ -93,6 +93,31 @@ def test_batch_fp32_equals_single_models():
         assert np.all(err < 1e-5), (q, err)
 
 
+_ORACLE = {}
+
+
+def _oracle(q, max_steps):
+    if (q, max_steps) not in _ORACLE:
+        m = O.Model(decks.small_setup(O, **MODELS[q]), CONDS[q])
+        _ORACLE[q, max_steps] = m.run(_rt(O), RX, O.make_tx(**TX), max_steps=max_steps)[0]
+    return _ORACLE[q, max_steps]
+
+
+@pytest.mark.parametrize("precision,rows,pf,tol", [(T.FP64, 12, 1, 1e-9), (T.FP64, 4, 2, 1e-9),
+                                                   (T.FP32, 10, 1, 1e-4), (T.FP32, 16, 2, 1e-4)])
+def test_batch_tiling_matches_the_oracle(precision, rows, pf, tol, monkeypatch):
+    """The model-batch march (one launch over every member's tiles) at tile heights other than its
+    default, each member against the oracle (ElectricalFiled.cu:145-172, Bderivate.cu:79-255)."""
+    monkeypatch.setenv("TCSEM_MARCH_ROWS", str(rows))
+    monkeypatch.setenv("TCSEM_MARCH_PF", str(pf))
+    got, steps = _batch(CONDS, precision, [0.25] * len(MODELS), max_steps=400)
+    assert steps == 400
+    for q in range(len(MODELS)):
+        err = decks.trace_error(got[q], _oracle(q, 400))
+        print("member", q, "rows", rows, "FP%d batch vs oracle:" % precision, err.max())
+        assert np.all(err < tol), (q, err)
+
+
 def test_batch_members_finishing_at_different_times():
     """Full runs; members 1 and 3 sample only to 0.08 s, so they finish first and are left out of
     the later launches (their records return at once)."""
