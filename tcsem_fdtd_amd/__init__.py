@@ -115,6 +115,7 @@ def lib():
         "tcsem_sync": (C.c_int, [P]),
         "tcsem_set_graph": (C.c_int, [P, C.c_int]),
         "tcsem_graph_replays": (C.c_long, [P]),
+        "tcsem_sigma_table": (C.c_int, [P]),
         "tcsem_batch_create": (C.c_int, [C.POINTER(P), C.c_int, C.POINTER(P)]),
         "tcsem_batch_autotune": (C.c_int, [P, C.c_int, ip, ip, dp]),
         "tcsem_batch_step": (C.c_int, [P, C.c_int]),
@@ -384,6 +385,10 @@ class Context:
 
     def graph_replays(self):
         return self.L_.tcsem_graph_replays(self.h)
+
+    def sigma_table(self):
+        """Distinct conductivities (0 included) of the 8-bit index the march reads; 0: full values."""
+        return self.L_.tcsem_sigma_table(self.h)
 
 
 # ---- model batch -----------------------------------------------------------------------------

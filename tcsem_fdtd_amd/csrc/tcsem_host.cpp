@@ -15,8 +15,10 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <cstdint>
 #include <cstring>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "tcsem_internal.h"
@@ -409,6 +411,12 @@ struct tcsem_ctx {
   char *d_fields = nullptr;  // 2 states x 5 fields x nslots x fstride elements
   int cur = 0;               // state holding the current fields
   char *d_sig = nullptr;
+  // the conductivity as an 8-bit index into a table of its distinct values (in the field
+  // precision; entry 0 is 0), for models of at most 255 distinct nonzero values: the march reads
+  // one byte per cell instead of a word.  TCSEM_SIGMA8=0 leaves it off.
+  unsigned char *d_sig8 = nullptr;
+  char *d_slut = nullptr;
+  int sig8_n = 0;  // table entries in use
   char *d_planes = nullptr;  // bz0, bxair, byair, carry sum, carry compensation: 5 x nslots x pstride
   char *d_zs = nullptr;      // z-chunk Bz partial sums: (kMaxChunks - 1) x nslots x pstride
   // chunk sums of B written by the chunked lower march for the next step (FP64): 2 parities x
@@ -603,6 +611,8 @@ MarchArgs<T> march_args(tcsem_ctx *c) {
   a.bx = fo + 3 * fs + org;
   a.by = fo + 4 * fs + org;
   a.sig = (const T *)c->d_sig + org;
+  a.sig8 = c->d_sig8 ? c->d_sig8 + org : nullptr;
+  a.slut = (const T *)c->d_slut;
   T *pl = (T *)c->d_planes;
   const long long ps = c->pstride * c->nslots;
   a.bz0 = pl + org;
@@ -2409,6 +2419,54 @@ int tcsem_create(const tcsem_setup *s, const double *cond, int precision, tcsem_
           pad[(size_t)k * c->PS + (size_t)(j + 1) * c->PX + (i + c->xm)] = c->cond[((size_t)k * M + j) * L + i];
     if (hipMalloc(&c->d_sig, pad.size() * c->esz) != hipSuccess) return bail(fail(TCSEM_ENOMEM, "sigma allocation"));
     if ((rc = upload_double_as(c, c->d_sig, pad))) return bail(rc);
+    // the 8-bit index: distinct values as the field precision holds them, 0 first
+    const char *s8 = std::getenv("TCSEM_SIGMA8");
+    if (!(s8 && std::strcmp(s8, "0") == 0)) {
+      auto bits_t = [&](double v) {  // the value's bit pattern in the field precision
+        std::uint64_t b = 0;
+        if (precision == TCSEM_FP64) {
+          std::memcpy(&b, &v, 8);
+        } else {
+          const float f = (float)v;
+          std::uint32_t u;
+          std::memcpy(&u, &f, 4);
+          b = u;
+        }
+        return b;
+      };
+      std::vector<double> lut{0.0};
+      std::unordered_map<std::uint64_t, unsigned char> at{{bits_t(0.0), 0}};
+      std::vector<unsigned char> idx(pad.size(), 0);
+      bool fits = true;
+      std::uint64_t last = bits_t(0.0);
+      unsigned char last_e = 0;
+      for (size_t q = 0; q < pad.size(); ++q) {
+        const std::uint64_t b = bits_t(pad[q]);
+        if (b != last) {
+          auto it = at.find(b);
+          if (it == at.end()) {
+            if (lut.size() == 256) {
+              fits = false;
+              break;
+            }
+            it = at.emplace(b, (unsigned char)lut.size()).first;
+            lut.push_back(pad[q]);
+          }
+          last = b;
+          last_e = it->second;
+        }
+        idx[q] = last_e;
+      }
+      if (fits) {
+        c->sig8_n = (int)lut.size();
+        lut.resize(256, 0.0);
+        if (hipMalloc((void **)&c->d_sig8, idx.size()) != hipSuccess || hipMalloc(&c->d_slut, 256 * c->esz) != hipSuccess)
+          return bail(fail(TCSEM_ENOMEM, "sigma index allocation"));
+        if (hipMemcpy(c->d_sig8, idx.data(), idx.size(), hipMemcpyHostToDevice) != hipSuccess)
+          return bail(fail(TCSEM_EDEVICE, "sigma index upload"));
+        if ((rc = upload_double_as(c, c->d_slut, lut))) return bail(rc);
+      }
+    }
   }
   {
     const HostGrid &g = c->g;
@@ -2503,6 +2561,8 @@ void tcsem_destroy(tcsem_ctx *c) {
   free_run(c);
   (void)hipFree(c->d_fields);
   (void)hipFree(c->d_sig);
+  (void)hipFree(c->d_sig8);
+  (void)hipFree(c->d_slut);
   (void)hipFree(c->d_planes);
   (void)hipFree(c->d_zs);
   (void)hipFree(c->d_ib);
@@ -3210,6 +3270,8 @@ int tcsem_set_graph(tcsem_ctx *c, int nsteps) {
 }
 
 long tcsem_graph_replays(const tcsem_ctx *c) { return c ? c->graph_replays : -1; }
+
+int tcsem_sigma_table(const tcsem_ctx *c) { return c ? c->sig8_n : -1; }
 
 int tcsem_batch_enable_timing(tcsem_batch *b, int on) {
   if (!b) return fail(TCSEM_EINVAL, "null batch");

@@ -45,6 +45,11 @@ struct MarchArgs {
   const T *exi, *eyi, *ezi, *bxi, *byi;  // old state, origin of slot 0
   T *ex, *ey, *ez, *bx, *by;             // new state, origin of slot 0
   const T *sig;               // origin, shared by all slots
+  // the same conductivity as an 8-bit index (origin, same layout, one byte per cell) into the
+  // table slut of the model's distinct values (entry 0 is 0): null when the model has more
+  // than 255 distinct nonzero conductivities (or the index is switched off)
+  const unsigned char *sig8;
+  const T *slut;
   T *bz0;                     // Bz at level 0 (plane origin, slot 0)
   const T *bxair, *byair;     // air planes (origin, slot 0)
   long long fstride;          // elements between slots of one field

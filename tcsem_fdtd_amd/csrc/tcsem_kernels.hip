@@ -1,21 +1,24 @@
 // tcsem_kernels.hip -- gfx950 kernels of the DuFort-Frankel step: the fused E/B volume march,
 // the surface level, the derived Bz volume and the receiver gather.
 //
-// Volume march (levels N-1 .. 1), one workgroup per 62 x 6 column tile, one lane per (i, j)
-// column, marching z bottom-up.  Per level it
+// Volume march (levels N-1 .. 1), one workgroup per tile of 64 aligned columns by BY - 2 rows
+// (plus one halo row each side), one lane per (i, j) column, marching z bottom-up.  Per level it
 //   (a) extends the running Bz integral of the *old* B (Bderivate.cu:227-253): Bz is never
 //       stored, the E update reads it from this register sum and its LDS copy;
 //   (b) prefetches B, sigma and E of the next level up into registers / the other LDS buffer;
 //   (c) updates Ez (ElectricalFiled.cu:277-345) on the tile plus a +x/+y halo;
-//   (d) updates Ex, Ey (ElectricalFiled.cu:89-274) and applies the bipole cells
-//       (ElectricalFiled.cu:347-475);
+//   (d) updates Ex, Ey (ElectricalFiled.cu:89-274) (the bipole cells of a source step are
+//       patched afterwards by k_src, ElectricalFiled.cu:347-475);
 //   (e) updates Bx, By (Bderivate.cu:79-200) from the new E of this level and the level below.
 // A level reads B_old at k and k-1 and E_new at k and k+1, so one bottom-up pass moves every
 // field once: read sigma, E(3), Bx, By and write E(3), Bx, By -- 11 words per cell against the
 // 13 of a stored-Bz design.  The halo ring (one column/row each side) keeps every neighbour
-// read inside the workgroup's LDS; halo lanes recompute Bz sums (-x/-y side) or Ez (+x/+y).
+// read inside the workgroup's LDS; halo rows recompute Bz sums (-y side) or Ez (+y), and the two
+// edge lanes fetch the columns either side of the block.
 // Level 0 needs the air fields from the upward continuation of Bz(0), so the march stops at
 // level 1 and writes Bz_old(0); k_surface finishes level 0 after the air planes are ready.
+#include <type_traits>
+
 #include "tcsem_internal.h"
 
 namespace tcsem {
@@ -225,6 +228,13 @@ struct Lv {
   T bx, by, s, ex, ey, ez;
   T hbx, hby, hs, hez;
 };
+// The same with the conductivity as an 8-bit index into the model's table of distinct values.
+template <typename T>
+struct Lv8 {
+  T bx, by, ex, ey, ez;
+  T hbx, hby, hez;
+  unsigned s, hs;
+};
 
 // Volume march, levels k_hi .. k_lo of every column (tiling and per-level steps in the header
 // comment of this file).  A workgroup is one 128-byte-aligned block of 64 columns (no halo lanes
@@ -257,14 +267,20 @@ __device__ inline MarchArgs<T> slot_args(const MarchArgs<T> &la, const MarchArgs
 // 10-row tiles (FP32) at prefetch depth 1 are compiled for five waves per SIMD: at most 96 VGPRs,
 // with a few values spilled to scratch, so that two 10-wave workgroups share a CU (at the
 // compiler's 108 VGPRs only one fits).  Their halo rows are 20% of the waves, against 25% for
-// 8-row tiles.  Deeper 10-row prefetch keeps the compiler's register choice (about 145 VGPRs, no
-// spills at depth 2): one workgroup per CU with twice the loads in flight, for grids that offer
-// few workgroups per CU.  12-row tiles (FP64) are compiled for three waves per SIMD (168 VGPRs
+// 8-row tiles.  The chunk-sum variant of that shape is compiled for four waves per SIMD (at most
+// 128 VGPRs, no spills; at five it spilled 34): its launches, chunked because the grid offers
+// few tiles, hold one workgroup per CU (measured on 256 x 256 x 160: 10 % faster).  Deeper
+// 10-row prefetch keeps the compiler's register choice (about 145 VGPRs, no spills at depth 2):
+// one workgroup per CU with twice the loads in flight, for grids that offer few workgroups per
+// CU.  12-row tiles (FP64) are compiled for three waves per SIMD (168 VGPRs
 // instead of 174-200): one 12-wave workgroup per CU with 17% halo rows, against one 8-wave
 // workgroup with 25%.  Other shapes keep the compiler's register choice.
-template <typename T, int BY, int PF, bool GEO, bool IB>
+// S8: the conductivity comes from the 8-bit index volume a.sig8 and its table a.slut (models of
+// at most 256 distinct conductivities, the table entry 0 being 0 for the zero margins): one byte
+// per cell and halo column read instead of a word, the same values.
+template <typename T, int BY, int PF, bool GEO, bool IB, bool S8>
 __global__ void __launch_bounds__(kMarchLanes *BY)
-__attribute__((amdgpu_waves_per_eu(BY == 10 && PF == 1 ? 5 : (BY == 12 ? 3 : 1))))
+__attribute__((amdgpu_waves_per_eu(BY == 10 && PF == 1 ? (IB ? 4 : 5) : (BY == 12 ? 3 : 1))))
 k_march(const MarchArgs<T> la, const MarchArgs<T> *geo) {
   MarchArgs<T> a;
   int tile;
@@ -338,8 +354,7 @@ k_march(const MarchArgs<T> la, const MarchArgs<T> *geo) {
   const __amdgpu_buffer_rsrc_t rbx = rsrc_of(bx, fbytes), rby = rsrc_of(by, fbytes);
   // The old Ex, Ey are used only by the tile's own rows (E updates its own cells; the halo rows
   // compute Bz sums and Ez), so the halo-row waves drop those loads, and each remaining load is
-  // the value's only read in the step: non-temporal, like the stores.  (Not in the chunk-sum
-  // variant: the select costs it a VGPR, and at 8 rows and prefetch 1 a wave per SIMD.)
+  // the value's only read in the step: non-temporal, like the stores.
   const __amdgpu_buffer_rsrc_t rexi = rsrc_of((void *)exi, fbytes), reyi = rsrc_of((void *)eyi, fbytes);
   const bool erow = ly >= 1 && ly <= BY - 2;
   const auto cf = make_coef<T>(a.lam, a.dt);
@@ -362,29 +377,38 @@ k_march(const MarchArgs<T> la, const MarchArgs<T> *geo) {
   }
   const int lfloor = klo - 1;
   const long long po = (long long)slot * a.pstride - (a.PX + a.xm);
+  using LvT = std::conditional_t<S8, Lv8<T>, Lv<T>>;
+  const unsigned char *__restrict__ sig8 = S8 ? a.sig8 - (a.PX + a.xm) : nullptr;
+  __shared__ T sLut[S8 ? 256 : 1];
+  if constexpr (S8) {
+    for (int q = ly * kMarchLanes + lx; q < 256; q += kMarchLanes * BY) sLut[q] = a.slut[q];
+    __syncthreads();
+  }
+  constexpr unsigned kEsh = sizeof(T) == 8 ? 3 : 2;  // byte offset of an element >> kEsh = index
   auto load = [&](int lv) {
     const unsigned lb = (unsigned)max(lv, lfloor) * PSb;
     const unsigned o = lb + colb, h = lb + hcolb;
-    Lv<T> v;
+    LvT v;
     v.bx = ldo(bxi, o);
     v.by = ldo(byi, o);
-    v.s = ldo(sig, o);
-    if constexpr (IB) {
-      v.ex = ldo(exi, o);
-      v.ey = ldo(eyi, o);
-    } else {
-      v.ex = bld<kNT>(rexi, erow ? o : kDrop, T(0));
-      v.ey = bld<kNT>(reyi, erow ? o : kDrop, T(0));
-    }
+    if constexpr (S8)
+      v.s = ldo(sig8, o >> kEsh);
+    else
+      v.s = ldo(sig, o);
+    v.ex = bld<kNT>(rexi, erow ? o : kDrop, T(0));
+    v.ey = bld<kNT>(reyi, erow ? o : kDrop, T(0));
     v.ez = ldo(ezi, o);
     v.hbx = ldo(bxi, h);
     v.hby = ldo(byi, h);
-    v.hs = ldo(sig, h);
+    if constexpr (S8)
+      v.hs = ldo(sig8, h >> kEsh);
+    else
+      v.hs = ldo(sig, h);
     v.hez = ldo(ezi, h);
     return v;
   };
   unsigned off = (unsigned)khi * PSb + colb;
-  Lv<T> R[S];
+  LvT R[S];
 #pragma unroll
   for (int d = 0; d < S - 1; ++d) R[d] = load(khi - d);
   // dz, 1/dz, 1/(dz[k-1]+dz[k]) of every level in LDS: per-level reads then wait on the LDS
@@ -393,13 +417,21 @@ k_march(const MarchArgs<T> la, const MarchArgs<T> *geo) {
   for (int q = ly * kMarchLanes + lx; q < 3 * N; q += kMarchLanes * BY) sZ[q] = a.dzt[q];
   // level k_hi into the LDS buffer of its parity; the edge lanes fill the halo slots (the other
   // lanes store their own value again, so the stores need no branch)
-  auto stage = [&](const int pb, const Lv<T> &v) {
+  auto stage = [&](const int pb, const LvT &v) {
+    T vs, vhs;
+    if constexpr (S8) {
+      vs = sLut[v.s];
+      vhs = sLut[v.hs];
+    } else {
+      vs = v.s;
+      vhs = v.hs;
+    }
     sBx[pb][ly][sl] = v.bx;
     sBy[pb][ly][sl] = v.by;
-    sS[pb][ly][sl] = v.s;
+    sS[pb][ly][sl] = vs;
     sBx[pb][ly][hsl] = right ? v.hbx : v.bx;
     sBy[pb][ly][hsl] = left ? v.hby : v.by;
-    sS[pb][ly][hsl] = (left || right) ? v.hs : v.s;
+    sS[pb][ly][hsl] = (left || right) ? vhs : vs;
   };
   stage(khi & 1, R[0]);
   Acc<T> bz, bzL;  // running Bz of the lane's column and (lane 0) of column i0-1
@@ -443,7 +475,7 @@ k_march(const MarchArgs<T> la, const MarchArgs<T> *geo) {
   // One level.  cur = level k, nxt = level k-1, dst = the free slot for the loads of level
   // k-1-PF.  Source cells are not applied here (a separate pass patches them), which keeps the
   // loop free of data-dependent branches, so the compiler's wait counts stay exact.
-  auto level = [&](const int k, const Lv<T> &cur, const Lv<T> &nxt, Lv<T> &dst) {
+  auto level = [&](const int k, const LvT &cur, const LvT &nxt, LvT &dst) {
     const int p = k & 1, q = p ^ 1;
     const bool st = k <= kst;  // false only on the E-only level that opens an upper chunk
     const T dzk = st ? sZ[k] : zero, rdzk = sZ[N + k], rdz2k = sZ[2 * N + k];
@@ -829,33 +861,38 @@ __global__ void k_fill_zero(uint32_t *p, size_t n) {
 
 // ---- launchers -----------------------------------------------------------------------------------
 // Raises a march kernel's dynamic-LDS limit once per size (static + dynamic may pass 64 KiB).
-template <typename T, int BY, int PF, bool GEO, bool IB>
+template <typename T, int BY, int PF, bool GEO, bool IB, bool S8>
 hipError_t march_lds(size_t bytes) {
   static size_t allowed = 0;  // per instantiation; only grows
   if (bytes <= allowed) return hipSuccess;
-  const hipError_t e = hipFuncSetAttribute((const void *)k_march<T, BY, PF, GEO, IB>,
+  const hipError_t e = hipFuncSetAttribute((const void *)k_march<T, BY, PF, GEO, IB, S8>,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
   if (e == hipSuccess) allowed = bytes;
   return e;
 }
-template <typename T, int BY, int PF, bool GEO, bool IB>
+template <typename T, int BY, int PF, bool GEO, bool IB, bool S8 = false>
 hipError_t launch_march_v(const MarchArgs<T> &a, const MarchArgs<T> *geo, dim3 grd, dim3 blk, hipStream_t s,
                           hipEvent_t e0, hipEvent_t e1) {
   const size_t zl = (size_t)3 * a.N * sizeof(T);
   hipError_t e;
-  if ((e = march_lds<T, BY, PF, GEO, IB>(zl)) != hipSuccess) return e;
-  launch_k(k_march<T, BY, PF, GEO, IB>, grd, blk, (std::uint32_t)zl, s, e0, e1, a, geo);
+  if ((e = march_lds<T, BY, PF, GEO, IB, S8>(zl)) != hipSuccess) return e;
+  launch_k(k_march<T, BY, PF, GEO, IB, S8>, grd, blk, (std::uint32_t)zl, s, e0, e1, a, geo);
   return hipGetLastError();
 }
 // geo == nullptr: one model (slots in grid y share a's geometry); else a model batch (a.N, a.nper:
 // the largest over the slots' records, which carry the rest).  A chunked launch with a.ibw set
-// runs the variant that accumulates the chunk sums of B.
+// runs the variant that accumulates the chunk sums of B; with a.sig8 set (one model), the
+// variant that reads the 8-bit conductivity index.
 template <typename T, int BY, int PF>
 hipError_t launch_march_pf(const MarchArgs<T> &a, const MarchArgs<T> *geo, dim3 grd, dim3 blk, hipStream_t s,
                            hipEvent_t e0, hipEvent_t e1) {
   if (geo) return launch_march_v<T, BY, PF, true, false>(a, geo, grd, blk, s, e0, e1);
-  if (a.ibw && a.nchunk > 1) return launch_march_v<T, BY, PF, false, true>(a, geo, grd, blk, s, e0, e1);
-  return launch_march_v<T, BY, PF, false, false>(a, geo, grd, blk, s, e0, e1);
+  const bool ib = a.ibw && a.nchunk > 1;
+  if (a.sig8)
+    return ib ? launch_march_v<T, BY, PF, false, true, true>(a, geo, grd, blk, s, e0, e1)
+              : launch_march_v<T, BY, PF, false, false, true>(a, geo, grd, blk, s, e0, e1);
+  return ib ? launch_march_v<T, BY, PF, false, true>(a, geo, grd, blk, s, e0, e1)
+            : launch_march_v<T, BY, PF, false, false>(a, geo, grd, blk, s, e0, e1);
 }
 template <typename T, int BY>
 hipError_t launch_march_by(const MarchArgs<T> &a, const MarchArgs<T> *geo, int nslot, hipStream_t s, hipEvent_t e0,
@@ -881,7 +918,9 @@ hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEven
     case 12:  // FP64 only
       if constexpr (sizeof(T) == 8) return launch_march_by<T, 12>(a, geo, nslot, s, e0, e1);
       break;
-    case 16: return launch_march_by<T, 16>(a, geo, nslot, s, e0, e1);
+    case 16:  // FP32 only (FP64 would not fit its registers)
+      if constexpr (sizeof(T) == 4) return launch_march_by<T, 16>(a, geo, nslot, s, e0, e1);
+      break;
   }
   return hipErrorInvalidValue;
 }
