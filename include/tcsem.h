@@ -230,8 +230,9 @@ long tcsem_graph_replays(const tcsem_ctx *ctx);
 /* The conductivity as the volume march reads it (no counterpart in the reference): a model of at
  * most 255 distinct nonzero conductivities (as the field precision holds them) is read as an
  * 8-bit index into a table of those values, one byte per cell instead of a word, with the same
- * results; TCSEM_SIGMA8=0 in the environment at tcsem_create turns it off.  Returns the table's
- * entries in use (0 included), 0 when the march reads the full conductivity, -1 for no context. */
+ * results; TCSEM_SIGMA8=0 in the environment at tcsem_create turns it off, and tcsem_autotune
+ * times the march both ways and keeps the faster.  Returns the table's entries in use (0
+ * included), 0 when the march reads the full conductivity, -1 for no context. */
 int tcsem_sigma_table(const tcsem_ctx *ctx);
 
 /* ---- model batch (no counterpart in the reference, which runs one model per process) --------

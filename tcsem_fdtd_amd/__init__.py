@@ -349,7 +349,7 @@ class Context:
         _check(self.L_.tcsem_autotune(self.h, nbatch, nsteps, C.byref(r), C.byref(p), C.byref(z),
                                       C.byref(f), C.byref(ms)))
         return {"rows": r.value, "pf": p.value, "zchunks": z.value,
-                "zsum": "fused" if f.value else "pass", "march_ms": ms.value}
+                "zsum": "fused" if f.value else "pass", "sigma8": self.sigma_table() > 0, "march_ms": ms.value}
 
     def get(self, comp, slot=0):
         a = np.zeros(self.shape(comp))

@@ -398,6 +398,7 @@ struct tcsem_ctx {
   long long PS = 0, fstride = 0, pstride = 0, wstride = 0;
   int by = 8, pf = 2, ntx = 0, nty = 0, nper = 0;  // march tiling
   int tune_nbatch = 0, tune_by = 0, tune_pf = 0, tune_nch = 0, tune_fuse = 1;  // autotuned tiling, its batch size
+  int tune_s8 = 1;
   int stx = 0, sty = 0, sper = 0;                   // surface tiling
   int nchunk = 0, chunk_len = 0;                    // z-chunks of the lower march (0: one pass)
   int fuse = 1;  // chunked march: start chunks from the chunk sums of B (1) or a k_zsum pass (0)
@@ -417,6 +418,7 @@ struct tcsem_ctx {
   unsigned char *d_sig8 = nullptr;
   char *d_slut = nullptr;
   int sig8_n = 0;  // table entries in use
+  bool s8 = true;  // the march reads the index (when there is one): the autotuner's choice
   char *d_planes = nullptr;  // bz0, bxair, byair, carry sum, carry compensation: 5 x nslots x pstride
   char *d_zs = nullptr;      // z-chunk Bz partial sums: (kMaxChunks - 1) x nslots x pstride
   // chunk sums of B written by the chunked lower march for the next step (FP64): 2 parities x
@@ -492,7 +494,7 @@ struct tcsem_ctx {
   int graph_k = 0;
   double *d_gs = nullptr;
   struct Graph {
-    int cur, ib_par, ib_valid, by, pf, nchunk, chunk_len, fuse;  // starting state and tiling
+    int cur, ib_par, ib_valid, by, pf, nchunk, chunk_len, fuse, s8;  // starting state and tiling
     int cur_after, ib_par_after, ib_valid_after;
     hipGraphExec_t exec;
   };
@@ -564,11 +566,13 @@ void choose_tiling(tcsem_ctx *c, int nbatch) {
   const long long wg = tiles(by) * nbatch;
   if (N >= 6 && wg < 2LL * cu) nch = (int)std::min<long long>((2LL * cu + wg - 1) / wg, std::max(1, nlev / 8));
   int fuse = 1;
+  c->s8 = true;
   if (c->tune_nbatch == nbatch) {
     by = c->tune_by;
     pf = c->tune_pf;
     nch = c->tune_nch;
     fuse = c->tune_fuse;
+    c->s8 = c->tune_s8 != 0;
   }
   if (const char *e = std::getenv("TCSEM_MARCH_ZSUM")) {  // "pass": k_zsum pass, "fused": chunk sums
     if (!std::strcmp(e, "pass")) fuse = 0;
@@ -611,7 +615,7 @@ MarchArgs<T> march_args(tcsem_ctx *c) {
   a.bx = fo + 3 * fs + org;
   a.by = fo + 4 * fs + org;
   a.sig = (const T *)c->d_sig + org;
-  a.sig8 = c->d_sig8 ? c->d_sig8 + org : nullptr;
+  a.sig8 = c->d_sig8 && c->s8 ? c->d_sig8 + org : nullptr;
   a.slut = (const T *)c->d_slut;
   T *pl = (T *)c->d_planes;
   const long long ps = c->pstride * c->nslots;
@@ -1428,11 +1432,11 @@ int graph_steps(tcsem_ctx *c) {
   const tcsem_ctx::Graph *g = nullptr;
   for (const auto &x : c->graphs)
     if (x.cur == c->cur && x.ib_par == c->ib_par && x.ib_valid == (int)c->ib_valid && x.by == c->by && x.pf == c->pf &&
-        x.nchunk == c->nchunk && x.chunk_len == c->chunk_len && x.fuse == c->fuse)
+        x.nchunk == c->nchunk && x.chunk_len == c->chunk_len && x.fuse == c->fuse && x.s8 == (int)c->s8)
       g = &x;
   if (!g) {
-    tcsem_ctx::Graph ng{c->cur, c->ib_par, (int)c->ib_valid, c->by, c->pf, c->nchunk, c->chunk_len, c->fuse, 0, 0, 0,
-                        nullptr};
+    tcsem_ctx::Graph ng{c->cur, c->ib_par, (int)c->ib_valid, c->by,        c->pf, c->nchunk, c->chunk_len,
+                        c->fuse, (int)c->s8, 0,          0,           0,     nullptr};
     const int ibn0 = c->ib_nchunk, ibl0 = c->ib_chunk_len;
     HIPCHK(hipStreamWaitEvent(st, c->xev[1], 0));
     HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
@@ -1522,7 +1526,7 @@ int check_ctx(const tcsem_ctx *c) {
 // to the concurrent air-plane chain.
 template <typename T>
 int autotune_march(tcsem_ctx *c, int nbatch, int nsteps, int *best_by, int *best_pf, int *best_nch, int *best_fuse,
-                   double *best_ms) {
+                   int *best_s8, double *best_ms) {
   const int N = c->g.info.N, nlev = N - 2;
   std::vector<int> rows_c = c->prec == 64 ? std::vector<int>{4, 8, 12} : std::vector<int>{4, 8, 10, 16};
   std::vector<int> nch_c = {1};
@@ -1554,12 +1558,13 @@ int autotune_march(tcsem_ctx *c, int nbatch, int nsteps, int *best_by, int *best
   };
   struct Trial {
     double ms;
-    int by, pf, nch, fuse;
+    int by, pf, nch, fuse, s8;
   };
   // time nsteps steps of one tiling (after an untimed step: the first launches of the tiling)
-  auto time_tiling = [&](int by, int pf, int nch, int fuse, double *ms_out) -> hipError_t {
+  auto time_tiling = [&](int by, int pf, int nch, int fuse, int s8, double *ms_out) -> hipError_t {
     apply_tiling(c, by, pf, nch);
     c->fuse = fuse;
+    c->s8 = s8 != 0;
     hipError_t e = one_step();
     if (e == hipSuccess) e = hipStreamWaitEvent(st, c->xev[1], 0);
     if (e == hipSuccess) e = hipEventRecord(e0, st);
@@ -1575,21 +1580,23 @@ int autotune_march(tcsem_ctx *c, int nbatch, int nsteps, int *best_by, int *best
   std::vector<Trial> trials;
   int rc = TCSEM_OK;
   const int fuse0 = c->fuse;
+  const bool s80 = c->s8;
   for (int by : rows_c)
     for (int pf = 1; pf <= 4 && rc == TCSEM_OK; ++pf)
       for (int nch : nch_c)
-        for (int fuse = 1; fuse >= 0 && rc == TCSEM_OK; --fuse) {
-          apply_tiling(c, by, pf, nch);
-          if (c->nchunk == 0 && nch > 1) continue;  // clamped to one pass: already timed
-          if (c->nchunk == 0 && fuse == 0) continue;  // no chunks: nothing to sum
-          double ms = 0.0;
-          const hipError_t e = time_tiling(by, pf, nch, fuse, &ms);
-          if (e != hipSuccess) {
-            rc = fail(TCSEM_EDEVICE, "autotune: %s", hipGetErrorString(e));
-            break;
+        for (int fuse = 1; fuse >= 0 && rc == TCSEM_OK; --fuse)
+          for (int s8 = c->d_sig8 ? 1 : 0; s8 >= 0 && rc == TCSEM_OK; --s8) {  // the index, if any, and not
+            apply_tiling(c, by, pf, nch);
+            if (c->nchunk == 0 && nch > 1) continue;    // clamped to one pass: already timed
+            if (c->nchunk == 0 && fuse == 0) continue;  // no chunks: nothing to sum
+            double ms = 0.0;
+            const hipError_t e = time_tiling(by, pf, nch, fuse, s8, &ms);
+            if (e != hipSuccess) {
+              rc = fail(TCSEM_EDEVICE, "autotune: %s", hipGetErrorString(e));
+              break;
+            }
+            trials.push_back({ms, c->by, pf, c->nchunk > 1 ? c->nchunk : 1, fuse, s8});
           }
-          trials.push_back({ms, c->by, pf, c->nchunk > 1 ? c->nchunk : 1, fuse});
-        }
   // the four fastest are timed twice more, interleaved, and ranked by the mean of their three
   // timings: one pass of short trials lets noise pick a tiling a few percent slower in production
   if (rc == TCSEM_OK && trials.size() > 1) {
@@ -1600,7 +1607,7 @@ int autotune_march(tcsem_ctx *c, int nbatch, int nsteps, int *best_by, int *best
     for (int pass = 0; pass < 2 && rc == TCSEM_OK; ++pass)
       for (size_t t = 0; t < trials.size() && rc == TCSEM_OK; ++t) {
         double ms = 0.0;
-        const hipError_t e = time_tiling(trials[t].by, trials[t].pf, trials[t].nch, trials[t].fuse, &ms);
+        const hipError_t e = time_tiling(trials[t].by, trials[t].pf, trials[t].nch, trials[t].fuse, trials[t].s8, &ms);
         if (e != hipSuccess) rc = fail(TCSEM_EDEVICE, "autotune: %s", hipGetErrorString(e));
         sum[t] += ms;
       }
@@ -1610,6 +1617,7 @@ int autotune_march(tcsem_ctx *c, int nbatch, int nsteps, int *best_by, int *best
   (void)hipEventDestroy(e1);
   c->cur = cur0;
   c->fuse = fuse0;
+  c->s8 = s80;
   c->ib_valid = false;
   if (rc) return rc;
   // the fastest, except that among tilings within 1% of it (timing noise) the one with the fewest
@@ -1625,6 +1633,7 @@ int autotune_march(tcsem_ctx *c, int nbatch, int nsteps, int *best_by, int *best
   *best_pf = pick->pf;
   *best_nch = pick->nch;
   *best_fuse = pick->fuse;
+  *best_s8 = pick->s8;
   *best_ms = pick->ms;
   return TCSEM_OK;
 }
@@ -1640,6 +1649,7 @@ struct tcsem_batch {
   std::vector<tcsem_ctx *> m;
   int prec = 32;
   int by = 0, pf = 0;  // autotuned march tiling (0: heuristic)
+  bool s8 = true;      // the march reads the members' conductivity indices (when all have one)
   hipStream_t stream = nullptr, stream2 = nullptr;
   hipEvent_t xev[2] = {}, done = nullptr;
   char *d_geo = nullptr;     // 2 x nmodel MarchArgs (parity-major)
@@ -1736,6 +1746,11 @@ int batch_records(tcsem_batch *b, int by, int pf, MarchArgs<T> &la, UpcontArgs<T
   la.rows = by;
   la.pf = pf;
   la.nchunk = 0;
+  // the march reads the 8-bit conductivity index when every member's record carries one (the
+  // launch record's pointer is then only that flag)
+  la.sig8 = nullptr;
+  if (b->s8 && std::all_of(b->m.begin(), b->m.end(), [](const tcsem_ctx *c) { return c->d_sig8 != nullptr; }))
+    la.sig8 = b->m[0]->d_sig8;
   la.k_hi = la.N - 1;
   la.k_lo = 1;
   ul.tiled = 1;
@@ -1960,7 +1975,7 @@ int batch_step_t(tcsem_batch *b, int nsteps) {
 // Times batch steps (zeroed fields, no source, no sampling) for each rows x prefetch tiling and
 // keeps the fastest; the members' fields are zeroed again afterwards.
 template <typename T>
-int batch_autotune_t(tcsem_batch *b, int nsteps, int *best_by, int *best_pf, double *best_ms) {
+int batch_autotune_t(tcsem_batch *b, int nsteps, int *best_by, int *best_pf, int *best_s8, double *best_ms) {
   const int nm = (int)b->m.size();
   if (int rc = batch_quiesce(b)) return rc;
   for (int q = 0; q < nm; ++q) b->active[q] = 1;
@@ -1983,37 +1998,44 @@ int batch_autotune_t(tcsem_batch *b, int nsteps, int *best_by, int *best_pf, dou
   std::vector<int> rows_c = b->prec == 64 ? std::vector<int>{4, 8, 12} : std::vector<int>{4, 8, 10, 16};
   double best = 1e300;
   int rc = TCSEM_OK;
+  const bool any8 = std::all_of(b->m.begin(), b->m.end(), [](const tcsem_ctx *c) { return c->d_sig8 != nullptr; });
+  const bool s80 = b->s8;
+  *best_s8 = 0;
   for (int by : rows_c) {
-    for (int pf = 1; pf <= 4 && rc == TCSEM_OK; ++pf) {
-      MarchArgs<T> la;
-      UpcontArgs<T> ul;
-      if ((rc = batch_records<T>(b, by, pf, la, ul))) break;
-      int par = 0;
-      if ((rc = batch_enqueue<T>(b, la, ul, par, 0, false))) break;  // first launches of this tiling
-      par ^= 1;
-      hipError_t e = hipStreamWaitEvent(b->stream, b->xev[1], 0);
-      if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
-      for (int k = 0; k < nsteps && e == hipSuccess && rc == TCSEM_OK; ++k, par ^= 1)
-        rc = batch_enqueue<T>(b, la, ul, par, 0, false);
-      if (rc) break;
-      if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->xev[1], 0);
-      if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
-      if (e == hipSuccess) e = hipEventSynchronize(e1);
-      float ms = 0.f;
-      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-      if (e != hipSuccess) {
-        rc = fail(TCSEM_EDEVICE, "batch autotune: %s", hipGetErrorString(e));
-        break;
+    for (int pf = 1; pf <= 4 && rc == TCSEM_OK; ++pf)
+      for (int s8 = any8 ? 1 : 0; s8 >= 0 && rc == TCSEM_OK; --s8) {  // the members' indices, and not
+        b->s8 = s8 != 0;
+        MarchArgs<T> la;
+        UpcontArgs<T> ul;
+        if ((rc = batch_records<T>(b, by, pf, la, ul))) break;
+        int par = 0;
+        if ((rc = batch_enqueue<T>(b, la, ul, par, 0, false))) break;  // first launches of this tiling
+        par ^= 1;
+        hipError_t e = hipStreamWaitEvent(b->stream, b->xev[1], 0);
+        if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
+        for (int k = 0; k < nsteps && e == hipSuccess && rc == TCSEM_OK; ++k, par ^= 1)
+          rc = batch_enqueue<T>(b, la, ul, par, 0, false);
+        if (rc) break;
+        if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->xev[1], 0);
+        if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) {
+          rc = fail(TCSEM_EDEVICE, "batch autotune: %s", hipGetErrorString(e));
+          break;
+        }
+        const double per = (double)ms / nsteps;
+        if (per < best) {
+          best = per;
+          *best_by = supported_rows(b->prec, by);
+          *best_pf = pf;
+          *best_s8 = s8;
+        }
       }
-      const double per = (double)ms / nsteps;
-      if (per < best) {
-        best = per;
-        *best_by = b->prec == 64 ? std::min(by, 8) : by;
-        *best_pf = pf;
-      }
-    }
     if (rc) break;
   }
+  b->s8 = s80;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   b->timing = timing0;
@@ -2895,10 +2917,10 @@ int tcsem_autotune(tcsem_ctx *c, int nbatch, int nsteps, int *rows, int *pf, int
   if ((rc = tcsem_reserve_batch(c, nbatch))) return rc;
   free_run(c);  // a begun run is discarded; tcsem_begin / tcsem_run start a new one
   if ((rc = tcsem_reset_fields(c, 0, nbatch))) return rc;
-  int by = 0, p = 0, nch = 0, fuse = 1;
+  int by = 0, p = 0, nch = 0, fuse = 1, s8 = 1;
   double t = 0.0;
-  rc = c->prec == 64 ? autotune_march<double>(c, nbatch, nsteps, &by, &p, &nch, &fuse, &t)
-                     : autotune_march<float>(c, nbatch, nsteps, &by, &p, &nch, &fuse, &t);
+  rc = c->prec == 64 ? autotune_march<double>(c, nbatch, nsteps, &by, &p, &nch, &fuse, &s8, &t)
+                     : autotune_march<float>(c, nbatch, nsteps, &by, &p, &nch, &fuse, &s8, &t);
   if (rc) return rc;
   if ((rc = tcsem_reset_fields(c, 0, nbatch))) return rc;
   c->tune_nbatch = nbatch;
@@ -2906,6 +2928,7 @@ int tcsem_autotune(tcsem_ctx *c, int nbatch, int nsteps, int *rows, int *pf, int
   c->tune_pf = p;
   c->tune_nch = nch;
   c->tune_fuse = fuse;
+  c->tune_s8 = s8;
   choose_tiling(c, nbatch);
   if (rows) *rows = by;
   if (pf) *pf = p;
@@ -3213,13 +3236,14 @@ int tcsem_batch_autotune(tcsem_batch *b, int nsteps, int *rows, int *pf, double 
   if (nsteps < 1 || nsteps > 1000) return fail(TCSEM_EINVAL, "1 <= steps <= 1000");
   for (const tcsem_ctx *c : b->m)
     if (int rc = batch_fail_if(c->steps != 0, "autotune a batch before its first step")) return rc;
-  int by = 0, p = 0;
+  int by = 0, p = 0, s8 = 0;
   double t = 0.0;
-  const int rc = b->prec == 64 ? batch_autotune_t<double>(b, nsteps, &by, &p, &t)
-                               : batch_autotune_t<float>(b, nsteps, &by, &p, &t);
+  const int rc = b->prec == 64 ? batch_autotune_t<double>(b, nsteps, &by, &p, &s8, &t)
+                               : batch_autotune_t<float>(b, nsteps, &by, &p, &s8, &t);
   if (rc) return rc;
   b->by = by;
   b->pf = p;
+  b->s8 = s8 != 0;
   if (rows) *rows = by;
   if (pf) *pf = p;
   if (ms) *ms = t;
@@ -3271,7 +3295,7 @@ int tcsem_set_graph(tcsem_ctx *c, int nsteps) {
 
 long tcsem_graph_replays(const tcsem_ctx *c) { return c ? c->graph_replays : -1; }
 
-int tcsem_sigma_table(const tcsem_ctx *c) { return c ? c->sig8_n : -1; }
+int tcsem_sigma_table(const tcsem_ctx *c) { return c ? (c->s8 ? c->sig8_n : 0) : -1; }
 
 int tcsem_batch_enable_timing(tcsem_batch *b, int on) {
   if (!b) return fail(TCSEM_EINVAL, "null batch");

@@ -881,12 +881,14 @@ hipError_t launch_march_v(const MarchArgs<T> &a, const MarchArgs<T> *geo, dim3 g
 }
 // geo == nullptr: one model (slots in grid y share a's geometry); else a model batch (a.N, a.nper:
 // the largest over the slots' records, which carry the rest).  A chunked launch with a.ibw set
-// runs the variant that accumulates the chunk sums of B; with a.sig8 set (one model), the
-// variant that reads the 8-bit conductivity index.
+// runs the variant that accumulates the chunk sums of B; with a.sig8 set (a model batch: when
+// every member's record carries its index), the variant that reads the 8-bit conductivity index.
 template <typename T, int BY, int PF>
 hipError_t launch_march_pf(const MarchArgs<T> &a, const MarchArgs<T> *geo, dim3 grd, dim3 blk, hipStream_t s,
                            hipEvent_t e0, hipEvent_t e1) {
-  if (geo) return launch_march_v<T, BY, PF, true, false>(a, geo, grd, blk, s, e0, e1);
+  if (geo)
+    return a.sig8 ? launch_march_v<T, BY, PF, true, false, true>(a, geo, grd, blk, s, e0, e1)
+                  : launch_march_v<T, BY, PF, true, false>(a, geo, grd, blk, s, e0, e1);
   const bool ib = a.ibw && a.nchunk > 1;
   if (a.sig8)
     return ib ? launch_march_v<T, BY, PF, false, true, true>(a, geo, grd, blk, s, e0, e1)

@@ -35,7 +35,8 @@ def main():
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--every-step-rx", action="store_true")
     ap.add_argument("--tiling", default=None,
-                    help="fixed march tiling rows,pf[,zchunks[,pass|fused]] instead of the autotuner")
+                    help="fixed march tiling rows,pf[,zchunks[,pass|fused[,1|0]]] instead of the autotuner "
+                         "(the last field: the 8-bit conductivity index on or off)")
     a = ap.parse_args()
     if a.tiling:
         f = a.tiling.split(",")
@@ -44,6 +45,8 @@ def main():
             os.environ["TCSEM_MARCH_ZCHUNKS"] = f[2]
         if len(f) > 3:
             os.environ["TCSEM_MARCH_ZSUM"] = f[3]
+        if len(f) > 4:
+            os.environ["TCSEM_SIGMA8"] = f[4]
         a.no_tune = True
     if a.workload == "uneven":
         return models(T, W, a)

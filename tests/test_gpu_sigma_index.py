@@ -76,3 +76,33 @@ def test_index_holds_at_most_255_nonzero_values(monkeypatch, distinct, expect):
     ref, _ = O.Model(decks.small_setup(O), cond).run(_rt(O), RX, O.make_tx(**TX), max_steps=300)
     err = decks.trace_error(got, ref)
     assert np.all(err < 1e-9), err
+
+
+def _batch_run(monkeypatch, precision, sigma8):
+    monkeypatch.setenv("TCSEM_SIGMA8", "1" if sigma8 else "0")
+    ctxs = []
+    try:
+        for kw in ({}, {"nx": 23, "ny": 17}, {"nx": 30, "ny": 26, "nz": 10}):
+            m = O.Model(decks.small_setup(O, **kw))
+            g = m.grid()
+            cond = decks.layered_cond((m.N, m.M, m.L), g["Z"][:, None, None] * np.ones((1, m.M, m.L)), blob=True)
+            c = T.Context(decks.small_setup(T, **kw), cond, precision)
+            assert c.sigma_table() == (4 if sigma8 else 0)
+            c.set_receivers(_rt(T), RX)
+            c.set_transmitters([T.make_tx(**TX)])
+            ctxs.append(c)
+        with T.Batch(ctxs) as b:
+            return b.run(max_steps=300)
+    finally:
+        for c in ctxs:
+            c.close()
+
+
+@pytest.mark.parametrize("precision", [T.FP64, T.FP32])
+def test_model_batch_index_equals_the_full_conductivity(monkeypatch, precision):
+    """The model batch's march (one launch over every member's tiles) reads each member's index."""
+    on = _batch_run(monkeypatch, precision, True)
+    off = _batch_run(monkeypatch, precision, False)
+    for q in range(len(on)):
+        assert np.abs(on[q]).max() > 0
+        assert np.array_equal(on[q], off[q]), q
