@@ -356,6 +356,11 @@ k_march(const MarchArgs<T> la, const MarchArgs<T> *geo) {
   // compute Bz sums and Ez), so the halo-row waves drop those loads, and each remaining load is
   // the value's only read in the step: non-temporal, like the stores.
   const __amdgpu_buffer_rsrc_t rexi = rsrc_of((void *)exi, fbytes), reyi = rsrc_of((void *)eyi, fbytes);
+  // The -y halo row computes no Ez, so its wave drops the old-Ez loads too (that row is the tile
+  // below's last owned row: a second read of it, often an L2 miss); only lane 63 uses the old Ez
+  // of column i0+64.
+  const __amdgpu_buffer_rsrc_t rezi = rsrc_of((void *)ezi, fbytes);
+  const bool ezrow = ly >= 1;
   const bool erow = ly >= 1 && ly <= BY - 2;
   const auto cf = make_coef<T>(a.lam, a.dt);
   const T dtb = (T)a.dtb, zero = 0;
@@ -397,14 +402,14 @@ k_march(const MarchArgs<T> la, const MarchArgs<T> *geo) {
       v.s = ldo(sig, o);
     v.ex = bld<kNT>(rexi, erow ? o : kDrop, T(0));
     v.ey = bld<kNT>(reyi, erow ? o : kDrop, T(0));
-    v.ez = ldo(ezi, o);
+    v.ez = bld<0>(rezi, ezrow ? o : kDrop, T(0));
     v.hbx = ldo(bxi, h);
     v.hby = ldo(byi, h);
     if constexpr (S8)
       v.hs = ldo(sig8, h >> kEsh);
     else
       v.hs = ldo(sig, h);
-    v.hez = ldo(ezi, h);
+    v.hez = bld<0>(rezi, ezrow && right ? h : kDrop, T(0));
     return v;
   };
   unsigned off = (unsigned)khi * PSb + colb;
