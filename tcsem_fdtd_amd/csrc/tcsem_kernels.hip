@@ -308,7 +308,12 @@ k_march(const MarchArgs<T> la, const MarchArgs<T> *geo) {
   __shared__ T sBx[2][BY][LW], sBy[2][BY][LW], sS[2][BY][LW];
   __shared__ T sBz[BY][LW], sEz[BY][LW];
   const int tx = tile % a.ntx, ty = tile / a.ntx;
-  const int lx = threadIdx.x, ly = threadIdx.y;
+  // One wavefront per row (64 lanes in x), so the row index is uniform across the wave.  Telling
+  // the compiler so (scalar registers for the row's tables and LDS row offsets) frees 6-15 VGPRs
+  // and most spills of the FP64 and model-batch variants (measured faster there); the FP32
+  // single-model variants ran 1-2% slower with it, so they keep the per-lane index.
+  const int lx = threadIdx.x;
+  const int ly = (GEO || sizeof(T) == 8) ? __builtin_amdgcn_readfirstlane((int)threadIdx.y) : (int)threadIdx.y;
   const int L = a.L, M = a.M, N = a.N;
   const int i0 = tx * kMarchLanes, i = i0 + lx, iR = i0 + kMarchLanes;
   const int j = ty * (BY - 2) - 1 + ly;
