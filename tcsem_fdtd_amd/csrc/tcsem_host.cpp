@@ -513,10 +513,10 @@ int device_cus() {
   return cu;
 }
 
-// Rows per march tile that the precision is built for: FP64 4, 8 or 12 (16 do not fit its
+// Rows per march tile that the precision is built for: FP64 4, 8, 10 or 12 (16 do not fit its
 // registers), FP32 4, 8, 10 or 16; other counts run as 8.
 int supported_rows(int prec, int by) {
-  if (prec == 64) return by == 4 || by == 12 ? by : 8;
+  if (prec == 64) return by == 4 || by == 10 || by == 12 ? by : 8;
   return by == 4 || by == 10 || by == 16 ? by : 8;
 }
 
@@ -1528,7 +1528,7 @@ template <typename T>
 int autotune_march(tcsem_ctx *c, int nbatch, int nsteps, int *best_by, int *best_pf, int *best_nch, int *best_fuse,
                    int *best_s8, double *best_ms) {
   const int N = c->g.info.N, nlev = N - 2;
-  std::vector<int> rows_c = c->prec == 64 ? std::vector<int>{4, 8, 12} : std::vector<int>{4, 8, 10, 16};
+  std::vector<int> rows_c = c->prec == 64 ? std::vector<int>{4, 8, 10, 12} : std::vector<int>{4, 8, 10, 16};
   std::vector<int> nch_c = {1};
   if (N >= 6)
     for (int z : {2, 3, 4, 6, 8})
@@ -1995,7 +1995,7 @@ int batch_autotune_t(tcsem_batch *b, int nsteps, int *best_by, int *best_pf, int
     (void)hipEventDestroy(e0);
     return fail(TCSEM_EDEVICE, "event creation failed");
   }
-  std::vector<int> rows_c = b->prec == 64 ? std::vector<int>{4, 8, 12} : std::vector<int>{4, 8, 10, 16};
+  std::vector<int> rows_c = b->prec == 64 ? std::vector<int>{4, 8, 10, 12} : std::vector<int>{4, 8, 10, 16};
   double best = 1e300;
   int rc = TCSEM_OK;
   const bool any8 = std::all_of(b->m.begin(), b->m.end(), [](const tcsem_ctx *c) { return c->d_sig8 != nullptr; });

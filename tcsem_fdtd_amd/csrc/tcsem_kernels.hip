@@ -274,13 +274,15 @@ __device__ inline MarchArgs<T> slot_args(const MarchArgs<T> &la, const MarchArgs
 // one workgroup per CU with twice the loads in flight, for grids that offer few workgroups per
 // CU.  12-row tiles (FP64) are compiled for three waves per SIMD (168 VGPRs
 // instead of 174-200): one 12-wave workgroup per CU with 17% halo rows, against one 8-wave
-// workgroup with 25%.  Other shapes keep the compiler's register choice.
+// workgroup with 25%.  10-row FP64 tiles get the same limit: 8 owned rows divide 512-row grids
+// into whole rounds of one workgroup per CU (512 tiles for 512 x 512 against 416 of 12 rows).  Other shapes keep the compiler's register choice.
 // S8: the conductivity comes from the 8-bit index volume a.sig8 and its table a.slut (models of
 // at most 256 distinct conductivities, the table entry 0 being 0 for the zero margins): one byte
 // per cell and halo column read instead of a word, the same values.
 template <typename T, int BY, int PF, bool GEO, bool IB, bool S8>
 __global__ void __launch_bounds__(kMarchLanes *BY)
-__attribute__((amdgpu_waves_per_eu(BY == 10 && PF == 1 ? (IB ? 4 : 5) : (BY == 12 ? 3 : 1))))
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? (BY == 10 || BY == 12 ? 3 : 1)
+                                                    : (BY == 10 && PF == 1 ? (IB ? 4 : 5) : 1))))
 k_march(const MarchArgs<T> la, const MarchArgs<T> *geo) {
   MarchArgs<T> a;
   int tile;
@@ -924,9 +926,7 @@ hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEven
   switch (a.rows) {
     case 4: return launch_march_by<T, 4>(a, geo, nslot, s, e0, e1);
     case 8: return launch_march_by<T, 8>(a, geo, nslot, s, e0, e1);
-    case 10:  // FP32 only
-      if constexpr (sizeof(T) == 4) return launch_march_by<T, 10>(a, geo, nslot, s, e0, e1);
-      break;
+    case 10: return launch_march_by<T, 10>(a, geo, nslot, s, e0, e1);
     case 12:  // FP64 only
       if constexpr (sizeof(T) == 8) return launch_march_by<T, 12>(a, geo, nslot, s, e0, e1);
       break;
