@@ -235,6 +235,20 @@ long tcsem_graph_replays(const tcsem_ctx *ctx);
  * included), 0 when the march reads the full conductivity, -1 for no context. */
 int tcsem_sigma_table(const tcsem_ctx *ctx);
 
+/* Two steps per lower pass (no counterpart in the reference, whose steps each read and write
+ * the whole state): with rows 12 or 16 (workgroup rows of the pass) and pf 1 or 2 (prefetch
+ * depth), tcsem_step and tcsem_run take each two consecutive steps after the source wavelet
+ * whose first holds no receiver sample as one pass over levels N-1 .. 3 of both steps (the
+ * intermediate state stays on chip there), then the top levels of each step as usual.  The
+ * results equal, bit for bit, those of one step at a time (tested).  rows 0 turns it off (the
+ * default); graph replays, where on, take precedence.  Needs N >= 6.
+ * tcsem_pair_info reports the setting and the pairs taken so far; tcsem_pair_timing the average
+ * device time [ms] of the two-step pass alone (timing enabled; its top launches count in
+ * tcsem_timing's t[0], per step). */
+int tcsem_set_pairs(tcsem_ctx *ctx, int rows, int pf);
+int tcsem_pair_info(const tcsem_ctx *ctx, int *rows, int *pf, long *pairs);
+int tcsem_pair_timing(const tcsem_ctx *ctx, double *ms, long *n);
+
 /* ---- model batch (no counterpart in the reference, which runs one model per process) --------
  * Conductivity models of uneven grids, one transmitter each, advanced together: every step's
  * volume march, air-plane chain and surface level run as one launch over all members.  Each

@@ -116,6 +116,9 @@ def lib():
         "tcsem_set_graph": (C.c_int, [P, C.c_int]),
         "tcsem_graph_replays": (C.c_long, [P]),
         "tcsem_sigma_table": (C.c_int, [P]),
+        "tcsem_set_pairs": (C.c_int, [P, C.c_int, C.c_int]),
+        "tcsem_pair_info": (C.c_int, [P, ip, ip, C.POINTER(C.c_long)]),
+        "tcsem_pair_timing": (C.c_int, [P, dp, C.POINTER(C.c_long)]),
         "tcsem_batch_create": (C.c_int, [C.POINTER(P), C.c_int, C.POINTER(P)]),
         "tcsem_batch_autotune": (C.c_int, [P, C.c_int, ip, ip, dp]),
         "tcsem_batch_step": (C.c_int, [P, C.c_int]),
@@ -389,6 +392,23 @@ class Context:
     def sigma_table(self):
         """Distinct conductivities (0 included) of the 8-bit index the march reads; 0: full values."""
         return self.L_.tcsem_sigma_table(self.h)
+
+    def set_pairs(self, rows, pf=1):
+        """Take two steady-state steps per lower pass (rows 12 or 16, prefetch pf 1 or 2; rows 0: off,
+        the default).  Results equal one step at a time, bit for bit."""
+        _check(self.L_.tcsem_set_pairs(self.h, rows, pf))
+
+    def pair_info(self):
+        """(rows, prefetch, pairs taken so far) of the two-step pass."""
+        r, p, n = C.c_int(), C.c_int(), C.c_long()
+        _check(self.L_.tcsem_pair_info(self.h, C.byref(r), C.byref(p), C.byref(n)))
+        return r.value, p.value, n.value
+
+    def pair_timing(self):
+        """(average device time [ms] of the two-step pass alone, passes timed)."""
+        ms, n = C.c_double(), C.c_long()
+        _check(self.L_.tcsem_pair_timing(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
 
 
 # ---- model batch -----------------------------------------------------------------------------

@@ -419,7 +419,9 @@ struct tcsem_ctx {
   char *d_slut = nullptr;
   int sig8_n = 0;  // table entries in use
   bool s8 = true;  // the march reads the index (when there is one): the autotuner's choice
-  char *d_planes = nullptr;  // bz0, bxair, byair, carry sum, carry compensation: 5 x nslots x pstride
+  // bz0, bxair, byair, carry sum, carry compensation, and the second carry sum and compensation of
+  // a two-step pass: 7 x nslots x pstride
+  char *d_planes = nullptr;
   char *d_zs = nullptr;      // z-chunk Bz partial sums: (kMaxChunks - 1) x nslots x pstride
   // chunk sums of B written by the chunked lower march for the next step (FP64): 2 parities x
   // (ib_cap chunks) x (x, y) x nslots x pstride; valid while the state is only advanced by steps
@@ -481,18 +483,24 @@ struct tcsem_ctx {
   // synchronisation), folded into tacc/tn when read
   bool timing = false;
   struct Span {
-    int kind, count;  // kind: 0 march, 1 surface, 2 upward continuation, 3 receiver gathers
+    int kind, count;  // kind: 0 march, 1 surface, 2 upward continuation, 3 receiver gathers, 4 two-step pass
     hipEvent_t a, b;
   };
   std::vector<hipEvent_t> evpool;
   size_t evused = 0;
   std::vector<Span> spans;
-  double tacc[4] = {0, 0, 0, 0};
-  long tn[4] = {0, 0, 0, 0};
+  double tacc[5] = {0, 0, 0, 0, 0};
+  long tn[5] = {0, 0, 0, 0, 0};
   // graph replays (tcsem_set_graph): runs of graph_k steady-state steps with no receiver sample
   // are captured once per starting state and replayed; their step scalars come from d_gs
   int graph_k = 0;
   double *d_gs = nullptr;
+  // two steps per lower pass (tcsem_set_pairs): workgroup rows (0: off) and prefetch depth of
+  // k_march2, its tiles, and the intermediate state (5 fields x nslots x fstride, allocated when
+  // first used; only levels 0 .. 2 are written)
+  int pair_by = 0, pair_pf = 1, pair_ntx = 0, pair_nty = 0, pair_nper = 0;
+  char *d_mid = nullptr;
+  long pairs_done = 0;
   struct Graph {
     int cur, ib_par, ib_valid, by, pf, nchunk, chunk_len, fuse, s8;  // starting state and tiling
     int cur_after, ib_par_after, ib_valid_after;
@@ -804,6 +812,8 @@ int alloc_slots(tcsem_ctx *c, int nslots) {
   (void)hipFree(c->d_work);
   (void)hipFree(c->d_zs);
   (void)hipFree(c->d_ib);
+  (void)hipFree(c->d_mid);
+  c->d_mid = nullptr;
   c->d_ib = nullptr;
   c->ib_cap = 0;
   c->ib_valid = false;
@@ -811,7 +821,7 @@ int alloc_slots(tcsem_ctx *c, int nslots) {
   c->d_work = nullptr;
   c->nslots = 0;
   const size_t fb = (size_t)2 * 5 * nslots * c->fstride * c->esz;
-  const size_t pb = (size_t)5 * nslots * c->pstride * c->esz;
+  const size_t pb = (size_t)7 * nslots * c->pstride * c->esz;
   // bytes per slot: tiled chain A and G (complex of the field type) and Bx, By (field type);
   // line chain cplx (complex FP64), tx, bxr (also the y-padding rows), byr (FP64)
   const size_t per_slot = c->up_tiled ? (size_t)(2 * c->sA + 2 * c->sG + 2 * c->sX) * c->esz
@@ -1484,6 +1494,139 @@ int graph_steps(tcsem_ctx *c) {
   return TCSEM_OK;
 }
 
+// ---- two steps per lower pass ---------------------------------------------------------------------
+// A pair of steps: k_march2 takes both steps' levels N-1 .. 3 (step A's level 2 too) in one pass
+// over the old state; then, in stream order, step A's top launch (level 1, into the intermediate
+// state), its upward continuation and surface level, and step B's top launch (levels 2 and 1,
+// from the intermediate state); step B's upward continuation, surface level and receiver gathers
+// go to stream2 and overlap the next pass.  The pass reads the old state at levels >= 1 only and
+// writes the intermediate state at level 2 only, so it too runs beside the previous step's
+// stream2 work.
+void pair_tiling(tcsem_ctx *c, int by, int pf) {
+  const int L = c->g.info.L, M = c->g.info.M;
+  c->pair_by = by;
+  c->pair_pf = pf;
+  if (!by) return;
+  c->pair_ntx = (L + kMarchLanes - 5) / (kMarchLanes - 4);
+  c->pair_nty = (M + by - 5) / (by - 4);
+  c->pair_nper = (c->pair_ntx * c->pair_nty + 7) / 8;
+}
+
+// Whether the next two steps may run as a pair: pairs on, past the source (no amplitude, no source
+// pass), and no receiver sample after the first of them (one after the second is taken from the
+// new state as usual).
+bool pair_ok(tcsem_ctx *c, long room) {
+  return c->pair_by > 0 && room >= 2 && c->g.info.N >= 6 && c->n >= 2 * (c->setup.n_src / 2) &&
+         !events_ahead(c, 1);
+}
+
+template <typename T>
+int do_pair(tcsem_ctx *c) {
+  const int nb = c->nbatch;
+  hipStream_t st = c->stream, st2 = c->stream2;
+  const UpcontArgs<T> &u = up_of<T>(c);
+  if (!c->d_mid) {
+    const size_t mb = (size_t)5 * c->nslots * c->fstride * c->esz;
+    if (hipMalloc(&c->d_mid, mb) != hipSuccess) return fail(TCSEM_ENOMEM, "intermediate-state allocation failed");
+    HIPCHK(launch_fill_zero(c->d_mid, mb, st));  // cells outside the grid stay zero
+  }
+  MarchArgs<T> a = march_args<T>(c);  // old state -> new state, step A's scalars
+  step_scalars<T>(c, a);
+  MarchArgs<T> sb{};  // step B's scalars: the time state after step A
+  {
+    const int n0 = c->n, st0 = c->steps;
+    const double toT0 = c->toT, d10 = c->dt1, d20 = c->dt2;
+    advance_time(c);
+    step_scalars<T>(c, sb);
+    c->n = n0;
+    c->steps = st0;
+    c->toT = toT0;
+    c->dt1 = d10;
+    c->dt2 = d20;
+  }
+  const long long org = c->PX + c->xm, fs = c->fstride * c->nslots, ps = c->pstride * c->nslots;
+  T *fm = (T *)c->d_mid, *pl = (T *)c->d_planes;
+  PairArgs<T> p{};
+  p.a = a;
+  p.a.pf = c->pair_pf;
+  p.mex = fm + org;
+  p.mey = fm + fs + org;
+  p.mez = fm + 2 * fs + org;
+  p.mbx = fm + 3 * fs + org;
+  p.mby = fm + 4 * fs + org;
+  p.cs2 = pl + 5 * ps + org;
+  p.cc2 = pl + 6 * ps + org;
+  p.lamB = sb.lam;
+  p.dtB = sb.dt;
+  p.dtbB = sb.dtb;
+  p.rows = c->pair_by;
+  p.ntx = c->pair_ntx;
+  p.nty = c->pair_nty;
+  p.nper = c->pair_nper;
+  // top launch and surface of step A: old state -> intermediate state
+  MarchArgs<T> ta = a;
+  ta.ex = p.mex;
+  ta.ey = p.mey;
+  ta.ez = p.mez;
+  ta.bx = p.mbx;
+  ta.by = p.mby;
+  ta.k_hi = 1;
+  ta.carry = 1;
+  // top launch (levels 2, 1) and surface of step B: intermediate state -> new state
+  MarchArgs<T> tb = a;
+  tb.exi = p.mex;
+  tb.eyi = p.mey;
+  tb.ezi = p.mez;
+  tb.bxi = p.mbx;
+  tb.byi = p.mby;
+  tb.lam = sb.lam;
+  tb.dt = sb.dt;
+  tb.dtb = sb.dtb;
+  tb.amp = 0.0;
+  tb.k_hi = 2;
+  tb.carry = 1;
+  tb.cs = p.cs2;
+  tb.cc = p.cc2;
+  // every pair that starts in the first two steps of a timing stride is timed
+  const bool timed = c->timing && c->steps % kTimingStride < 2;
+  hipEvent_t te[14] = {};
+  if (timed) {
+    if (c->spans.size() > 50000 && (rc_collect(c))) return TCSEM_EDEVICE;
+    for (auto &e : te)
+      if (!(e = timing_event(c))) return fail(TCSEM_EDEVICE, "timing event creation failed");
+  }
+  HIPCHK(launch_march2<T>(p, nb, st, te[0], te[1]));
+  HIPCHK(hipStreamWaitEvent(st, c->xev[1], 0));  // the previous step's stream2 work
+  HIPCHK(launch_march<T>(ta, nb, st, te[2], te[3]));
+  HIPCHK(launch_upcont<T>(u, nb, st, te[4], te[5]));
+  HIPCHK(launch_surface<T>(ta, nb, st, te[6], te[7]));
+  HIPCHK(launch_march<T>(tb, nb, st, te[8], te[9]));
+  HIPCHK(hipEventRecord(c->xev[0], st));
+  HIPCHK(hipStreamWaitEvent(st2, c->xev[0], 0));
+  HIPCHK(launch_upcont<T>(u, nb, st2, te[10], te[11]));
+  HIPCHK(launch_surface<T>(tb, nb, st2, te[12], te[13]));
+  c->cur ^= 1;
+  c->ib_valid = false;
+  advance_time(c);
+  if (int rc = sample_rx<T>(c, st2, false)) return rc;  // no sample fires after step A
+  advance_time(c);
+  if (int rc = sample_rx<T>(c, st2, timed)) return rc;
+  HIPCHK(hipEventRecord(c->xev[1], st2));
+  ++c->pairs_done;
+  if (timed) {
+    // the march of the two steps: the pass and both top launches (two steps counted)
+    c->spans.push_back({0, 0, te[0], te[1]});
+    c->spans.push_back({0, 1, te[2], te[3]});
+    c->spans.push_back({0, 1, te[8], te[9]});
+    c->spans.push_back({2, 1, te[4], te[5]});
+    c->spans.push_back({2, 1, te[10], te[11]});
+    c->spans.push_back({1, 1, te[6], te[7]});
+    c->spans.push_back({1, 1, te[12], te[13]});
+    c->spans.push_back({4, 1, te[0], te[1]});
+  }
+  return TCSEM_OK;
+}
+
 // One step, or graph_k steps as a graph replay when allowed and at most `room` steps may be taken.
 // Returns the steps taken (negative: error code).
 template <typename T>
@@ -1491,6 +1634,10 @@ int advance(tcsem_ctx *c, long room) {
   if (room >= c->graph_k && graph_ok(c)) {
     const int rc = graph_steps<T>(c);
     return rc ? rc : c->graph_k;
+  }
+  if (pair_ok(c, room)) {
+    const int rc = do_pair<T>(c);
+    return rc ? rc : 2;
   }
   const int rc = do_step<T>(c);
   return rc ? rc : 1;
@@ -2588,6 +2735,7 @@ void tcsem_destroy(tcsem_ctx *c) {
   (void)hipFree(c->d_planes);
   (void)hipFree(c->d_zs);
   (void)hipFree(c->d_ib);
+  (void)hipFree(c->d_mid);
   (void)hipFree(c->d_gs);
   (void)hipFree(c->d_rd);
   (void)hipFree(c->d_grid);
@@ -2704,7 +2852,7 @@ int tcsem_reset_fields(tcsem_ctx *c, int first, int n) {
   const size_t fb = (size_t)c->fstride * c->esz, pb = (size_t)c->pstride * c->esz;
   for (int f = 0; f < 10; ++f)
     HIPCHK(launch_fill_zero(c->d_fields + ((size_t)f * c->nslots + first) * fb, fb * n, c->stream));
-  for (int f = 0; f < 5; ++f)
+  for (int f = 0; f < 7; ++f)
     HIPCHK(launch_fill_zero(c->d_planes + ((size_t)f * c->nslots + first) * pb, pb * n, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return TCSEM_OK;
@@ -3147,7 +3295,7 @@ int tcsem_enable_timing(tcsem_ctx *c, int on) {
   if (on) {
     int rc = rc_collect(c);
     if (rc) return rc;
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < 5; ++q) {
       c->tacc[q] = 0;
       c->tn[q] = 0;
     }
@@ -3168,6 +3316,33 @@ int tcsem_timing(const tcsem_ctx *cc, double t[4], long n[4]) {
     n[q] = c->tn[q];
     t[q] = c->tn[q] ? c->tacc[q] / c->tn[q] : 0.0;
   }
+  return TCSEM_OK;
+}
+
+int tcsem_pair_timing(const tcsem_ctx *cc, double *ms, long *n) {
+  if (!cc || !ms || !n) return fail(TCSEM_EINVAL, "null argument");
+  tcsem_ctx *c = const_cast<tcsem_ctx *>(cc);
+  int rc = rc_collect(c);
+  if (rc) return rc;
+  *n = c->tn[4];
+  *ms = c->tn[4] ? c->tacc[4] / c->tn[4] : 0.0;
+  return TCSEM_OK;
+}
+
+int tcsem_set_pairs(tcsem_ctx *c, int rows, int pf) {
+  if (check_ctx(c)) return TCSEM_EINVAL;
+  if (rows != 0 && (!pair_rows_supported(c->prec, rows) || pf < 1 || pf > 2))
+    return fail(TCSEM_EINVAL, "two-step pass: rows 0 (off), 12 or 16; prefetch 1 or 2");
+  if (rows && c->g.info.N < 6) return fail(TCSEM_EINVAL, "two-step pass needs at least 6 levels");
+  pair_tiling(c, rows, pf);
+  return TCSEM_OK;
+}
+
+int tcsem_pair_info(const tcsem_ctx *c, int *rows, int *pf, long *pairs) {
+  if (check_ctx(c) || !rows || !pf || !pairs) return fail(TCSEM_EINVAL, "null argument");
+  *rows = c->pair_by;
+  *pf = c->pair_pf;
+  *pairs = c->pairs_done;
   return TCSEM_OK;
 }
 

@@ -105,6 +105,23 @@ struct MarchArgs {
   int fixn;
 };
 
+// Two steps in one lower pass (k_march2): step A advances the old state (a.exi ..) to the
+// intermediate state, step B the intermediate state to the new one (a.ex ..), two levels behind
+// step A in the same bottom-up pass, so the intermediate state never goes through HBM below level
+// 2.  Step A stores its level 2 into the intermediate buffer (mex ..) and its running Bz sums
+// through level 2 into a.cs/a.cc; step B stores levels N-1 .. 3 and its sums through level 3 into
+// cs2/cc2.  The top levels of both steps are the ordinary top launches and surface kernels.
+// Tiles: 60 owned columns (64 lanes: two halo columns each side) by rows - 4 owned rows (two halo
+// rows each side); ntx, nty, nper count them (tiles per XCD group).
+template <typename T>
+struct PairArgs {
+  MarchArgs<T> a;                  // step A's scalars; old and new state; carry planes of step A
+  T *mex, *mey, *mez, *mbx, *mby;  // intermediate state (origin, slot 0)
+  T *cs2, *cc2;                    // carry planes of step B (plane origin, slot 0)
+  double lamB, dtB, dtbB;          // step B's scalars
+  int rows, ntx, nty, nper;
+};
+
 // Receiver gather descriptor (EFInterp2Rx.cu:60-470) for one field group (E or B) of one event.
 struct GatherArgs {
   const void *f[6];          // Ex Ey Ez Bx By (origin, slot 0); Bz entry unused
@@ -197,6 +214,11 @@ inline void launch_k(F kern, dim3 grid, dim3 block, std::uint32_t lds, hipStream
 template <typename T>
 hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEvent_t e0 = nullptr,
                         hipEvent_t e1 = nullptr, const MarchArgs<T> *geo = nullptr);
+// Rows of the two-step march's workgroups that a precision supports (0: none).
+int pair_rows_supported(int prec, int rows);
+template <typename T>
+hipError_t launch_march2(const PairArgs<T> &p, int nslot, hipStream_t s, hipEvent_t e0 = nullptr,
+                         hipEvent_t e1 = nullptr);
 template <typename T>
 hipError_t launch_surface(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEvent_t e0 = nullptr,
                           hipEvent_t e1 = nullptr, const MarchArgs<T> *geo = nullptr);

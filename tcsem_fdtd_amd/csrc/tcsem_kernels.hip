@@ -574,6 +574,255 @@ k_march(const MarchArgs<T> la, const MarchArgs<T> *geo) {
   if (own) sto(a.bz0 + po, colb, bz.val());
 }
 
+// ---- two steps in one lower pass ----------------------------------------------------------------
+// k_march2 (PairArgs): step A (old -> intermediate state) at levels N-1 .. 2 and step B
+// (intermediate -> new state) at levels N-1 .. 3 in one bottom-up pass.  Step B at level m reads
+// the intermediate B at m and m-1 and E at m, so it runs two levels behind step A: iteration k
+// takes step A at level k and step B at level k+2, and the intermediate state below level 2 lives
+// in registers and LDS only.  Per cell-update that halves the HBM traffic of the lower levels
+// (read sigma and the old state once, write the new state once per two steps) for 1.6x the
+// lanes of the one-step march's owned cells.
+// Every quantity is the one-step march's expression in the same operation order, so the two
+// steps give the bits of two one-step passes.  The tile is 64 lanes (columns i0-2 .. i0+61) by
+// BY rows (j0-2 .. j0+BY-3): step A is exact on lanes and rows 1 .. 62 / BY-2 (the neighbours its
+// curls read are one cell further out), step B on lanes 2 .. 61 and rows 2 .. BY-3, the cells a
+// workgroup owns.  Cells outside the grid read and compute as zero, as memory holds them.
+// Per iteration, three phases with a barrier after the first two (the third runs into the next
+// iteration's first, which touches none of its LDS):
+//   P1  A: Bz sum of level k (B_old of level k from LDS); stage B_old and sigma of level k-1;
+//          loads of level k-1-PF.   B: Bz sum of level k+2 (intermediate B from LDS).
+//   P2  A: intermediate E of level k.   B: new E of level k+2 (stored).
+//   P3  A: intermediate B of level k (into LDS for step B).   B: new B of level k+2 (stored).
+// Step B idles on the first two iterations (levels N+1, N: dz 0, nothing stored, inputs zero)
+// and step A on the last (k = 1: nothing kept), so every iteration runs the same code.  The
+// lower pass never reads level 0 of the old state (the previous pair's surface kernel writes it).
+template <typename T, bool S8>
+struct Lv2 {
+  T bx, by, ex, ey, ez;
+  std::conditional_t<S8, unsigned, T> s;
+};
+template <typename T, int BY, int PF, bool S8>
+__global__ void __launch_bounds__(kMarchLanes *BY) k_march2(const PairArgs<T> pa) {
+  const MarchArgs<T> &a = pa.a;
+  const int b = blockIdx.x;
+  const int tile = (b & 7) * pa.nper + (b >> 3);  // XCD-contiguous bands of tiles
+  if (tile >= pa.ntx * pa.nty) return;
+  constexpr int W = kMarchLanes, TX = W - 4, TY = BY - 4;
+  extern __shared__ __align__(16) unsigned char zdyn[];  // 3 N entries of T
+  __shared__ T sBx[2][BY][W], sBy[2][BY][W], sS[2][BY][W];
+  __shared__ T sB1x[2][BY][W], sB1y[2][BY][W];
+  __shared__ T sBzA[BY][W], sEzA[BY][W], sBzB[BY][W], sEzB[BY][W];
+  __shared__ T sLut[S8 ? 256 : 1];
+  const int tx = tile % pa.ntx, ty = tile / pa.ntx;
+  const int lx = threadIdx.x;
+  const int ly = __builtin_amdgcn_readfirstlane((int)threadIdx.y);  // one wave per row
+  const int L = a.L, M = a.M, N = a.N;
+  const int i = tx * TX - 2 + lx, j = ty * TY - 2 + ly;
+  // grid cells of each component (zero outside, as memory holds them): Ex i in [0,L) j in [1,M),
+  // Ey and Bx i in [1,L) j in [0,M), Ez i in [1,L) j in [1,M), By as Ex
+  const bool xin = i >= 0 && i < L, xin1 = i >= 1 && i < L, yin = j >= 0 && j < M, yin1 = j >= 1 && j < M;
+  const bool dEx = xin && yin1, dEy = xin1 && yin, dEz = xin1 && yin1, dBx = dEy, dBy = dEx;
+  const bool own = lx >= 2 && lx < W - 2 && ly >= 2 && ly < BY - 2;
+  const int ic = min(max(i, -1), L + 1), jc = min(max(j, -1), M + 1);
+  const unsigned colb = (unsigned)((jc + 1) * a.PX + a.xm + ic) * (unsigned)sizeof(T);
+  const int lxm = max(lx - 1, 0), lxp = min(lx + 1, W - 1), lym = max(ly - 1, 0), lyp = min(ly + 1, BY - 1);
+  const T rdx = a.rdx[ic + 1], rdx2 = a.rdx2[ic + 1], rdy = a.rdy[jc + 1], rdy2 = a.rdy2[jc + 1];
+  const int slot = (int)blockIdx.y;
+  const long long fo = (long long)slot * a.fstride - (a.PX + a.xm);  // allocation base of the slot
+  const long long po = (long long)slot * a.pstride - (a.PX + a.xm);
+  const T *__restrict__ bxi = a.bxi + fo;
+  const T *__restrict__ byi = a.byi + fo;
+  const T *__restrict__ sig = a.sig - (a.PX + a.xm);
+  const unsigned char *__restrict__ sig8 = S8 ? a.sig8 - (a.PX + a.xm) : nullptr;
+  const unsigned PSb = (unsigned)a.PS * (unsigned)sizeof(T);
+  const unsigned fbytes = (unsigned)(a.fstride * (long long)sizeof(T));
+  const __amdgpu_buffer_rsrc_t rex = rsrc_of(a.ex + fo, fbytes), rey = rsrc_of(a.ey + fo, fbytes),
+                               rez = rsrc_of(a.ez + fo, fbytes), rbx = rsrc_of(a.bx + fo, fbytes),
+                               rby = rsrc_of(a.by + fo, fbytes);
+  const __amdgpu_buffer_rsrc_t mex = rsrc_of(pa.mex + fo, fbytes), mey = rsrc_of(pa.mey + fo, fbytes),
+                               mez = rsrc_of(pa.mez + fo, fbytes), mbx = rsrc_of(pa.mbx + fo, fbytes),
+                               mby = rsrc_of(pa.mby + fo, fbytes);
+  // old E: step A computes Ex, Ey on rows 1 .. BY-2 and Ez on rows 1 .. BY-1 (row 0 only sums Bz)
+  const __amdgpu_buffer_rsrc_t rexi = rsrc_of((void *)(a.exi + fo), fbytes),
+                               reyi = rsrc_of((void *)(a.eyi + fo), fbytes),
+                               rezi = rsrc_of((void *)(a.ezi + fo), fbytes);
+  const bool erow = ly >= 1 && ly <= BY - 2, ezrow = ly >= 1;
+  const auto cfA = make_coef<T>(a.lam, a.dt);
+  const auto cfB = make_coef<T>(pa.lamB, pa.dtB);
+  const T dtbA = (T)a.dtb, dtbB = (T)pa.dtbB, zero = 0;
+  constexpr int S = PF + 2;
+  constexpr unsigned kEsh = sizeof(T) == 8 ? 3 : 2;
+  constexpr int klo = 2, lfloor = 1;
+  const int khi = N - 1;
+  using LvT = Lv2<T, S8>;
+  if constexpr (S8) {
+    for (int q = ly * W + lx; q < 256; q += W * BY) sLut[q] = a.slut[q];
+  }
+  T *sZ = (T *)zdyn;
+  for (int q = ly * W + lx; q < 3 * N; q += W * BY) sZ[q] = a.dzt[q];
+  auto load = [&](int lv) {
+    const unsigned o = (unsigned)max(lv, lfloor) * PSb + colb;
+    LvT v;
+    v.bx = ldo(bxi, o);
+    v.by = ldo(byi, o);
+    if constexpr (S8)
+      v.s = ldo(sig8, o >> kEsh);
+    else
+      v.s = ldo(sig, o);
+    v.ex = bld<0>(rexi, erow ? o : kDrop, T(0));
+    v.ey = bld<0>(reyi, erow ? o : kDrop, T(0));
+    v.ez = bld<0>(rezi, ezrow ? o : kDrop, T(0));
+    return v;
+  };
+  LvT R[S];
+#pragma unroll
+  for (int d = 0; d < S - 1; ++d) R[d] = load(khi - d);
+  auto stage = [&](const int k, const LvT &v) {  // B_old and sigma of level k into LDS
+    sBx[k & 1][ly][lx] = v.bx;
+    sBy[k & 1][ly][lx] = v.by;
+    if constexpr (S8)
+      sS[k & 1][ly][lx] = sLut[v.s];
+    else
+      sS[k & 1][ly][lx] = v.s;
+  };
+  // step B's inputs of its idle iterations read as zero (each lane clears its own cells)
+  sB1x[0][ly][lx] = sB1x[1][ly][lx] = zero;
+  sB1y[0][ly][lx] = sB1y[1][ly][lx] = zero;
+  __syncthreads();  // the table sLut
+  stage(khi, R[0]);
+  Acc<T> bzA, bzB;  // running Bz sums of the old and of the intermediate B
+  bzA.init();
+  bzB.init();
+  // intermediate B of levels k+2 (c) and k+1 (n) and intermediate E of levels k+2 (c) and k+1 (n)
+  // of this lane's column; new E of level k+3 (step B's level below)
+  T b1xc = 0, b1yc = 0, b1xn = 0, b1yn = 0;
+  T e1xc = 0, e1yc = 0, e1zc = 0, e1xn = 0, e1yn = 0, e1zn = 0;
+  T e2x = 0, e2y = 0;
+  // conductivity averages (Ez, Ex, Ey) of levels k+2 (c) and k+1 (n)
+  T azc = 0, axc = 0, ayc = 0, azn = 0, axn = 0, ayn = 0;
+  __syncthreads();
+  __builtin_amdgcn_s_waitcnt(0);
+
+  auto level = [&](const int k, const LvT &cur, const LvT &nxt, LvT &dst) {
+    const int kB = k + 2, p = k & 1;  // level k+2 has the parity of k
+    const bool aon = k >= klo, bon = kB <= N - 1;
+    const int kBc = min(kB, N - 1);
+    const T dzA = aon ? sZ[k] : zero, rdzA = sZ[N + k], rdz2A = sZ[2 * N + k];
+    const T dzB = bon ? sZ[kBc] : zero, rdzB = sZ[N + kBc], rdz2B = sZ[2 * N + kBc];
+    const unsigned offA = (unsigned)k * PSb + colb, offB = (unsigned)kBc * PSb + colb;
+    // ---- P1
+    const T bx_k = cur.bx, by_k = cur.by, bx_m = nxt.bx, by_m = nxt.by;
+    bzA.add(dzA * ((sBx[p][ly][lxp] - bx_k) * rdx + (sBy[p][lyp][lx] - by_k) * rdy));
+    const T bzAk = bzA.val();
+    sBzA[ly][lx] = bzAk;
+    bzB.add(dzB * ((sB1x[p][ly][lxp] - b1xc) * rdx + (sB1y[p][lyp][lx] - b1yc) * rdy));
+    const T bzBk = bzB.val();
+    sBzB[ly][lx] = bzBk;
+    stage(k - 1, nxt);
+    const T ezo = cur.ez, exo = cur.ex, eyo = cur.ey;
+    dst = load(k - 1 - PF);
+    __syncthreads();
+    // ---- P2: intermediate E of level k (step A), new E of level k+2 (step B).  The conductivity
+    // averages of a level are the same for both steps: step A forms them, step B takes them two
+    // iterations later from registers.
+    T ezA, exA, eyA, azA, axA, ayA;
+    {
+      const int sp = k & 1, sq = sp ^ 1;
+      azA = (T)0.25 * (sS[sp][lym][lxm] + sS[sp][lym][lx] + sS[sp][ly][lxm] + sS[sp][ly][lx]);
+      {
+        const T sxk = (T)0.5 * (sS[sp][lym][lx] + sS[sp][ly][lx]);
+        const T sxm = (T)0.5 * (sS[sq][lym][lx] + sS[sq][ly][lx]);
+        axA = (T)0.5 * (sxk + sxm);
+      }
+      ayA = (T)0.25 * (sS[sp][ly][lxm] + sS[sp][ly][lx] + sS[sq][ly][lxm] + sS[sq][ly][lx]);
+      {
+        const T curl = (by_k - sBy[p][ly][lxm]) * rdx2 - (bx_k - sBx[p][lym][lx]) * rdy2;
+        ezA = dEz ? df_new(ezo, azA, cfA, curl) : zero;
+      }
+      {
+        const T curl = (bzAk - sBzA[lym][lx]) * rdy2 - (by_k - by_m) * rdz2A;
+        exA = dEx ? df_new(exo, axA, cfA, curl) : zero;
+      }
+      {
+        const T curl = (bx_k - bx_m) * rdz2A - (bzAk - sBzA[ly][lxm]) * rdx2;
+        eyA = dEy ? df_new(eyo, ayA, cfA, curl) : zero;
+      }
+    }
+    sEzA[ly][lx] = ezA;
+    T ezB, exB, eyB;
+    {
+      {
+        const T curl = (b1yc - sB1y[p][ly][lxm]) * rdx2 - (b1xc - sB1x[p][lym][lx]) * rdy2;
+        ezB = bon && dEz ? df_new(e1zc, azc, cfB, curl) : zero;
+      }
+      {
+        const T curl = (bzBk - sBzB[lym][lx]) * rdy2 - (b1yc - b1yn) * rdz2B;
+        exB = bon && dEx ? df_new(e1xc, axc, cfB, curl) : zero;
+      }
+      {
+        const T curl = (b1xc - b1xn) * rdz2B - (bzBk - sBzB[ly][lxm]) * rdx2;
+        eyB = bon && dEy ? df_new(e1yc, ayc, cfB, curl) : zero;
+      }
+    }
+    sEzB[ly][lx] = ezB;
+    const bool stB = own && bon;
+    bsto(rez, offB, stB && dEz, ezB);
+    bsto(rex, offB, stB && dEx, exB);
+    bsto(rey, offB, stB && dEy, eyB);
+    __syncthreads();
+    // ---- P3: intermediate B of level k (step A, into LDS), new B of level k+2 (step B)
+    const T bxA = bx_k + dtbA * ((e1yn - eyA) * rdzA - (sEzA[lyp][lx] - ezA) * rdy);
+    const T byA = by_k + dtbA * ((sEzA[ly][lxp] - ezA) * rdx - (e1xn - exA) * rdzA);
+    const T b1xA = dBx ? bxA : zero, b1yA = dBy ? byA : zero;
+    sB1x[p][ly][lx] = b1xA;
+    sB1y[p][ly][lx] = b1yA;
+    const bool stA = own && k == klo;  // level 2 of the intermediate state, for the top launches
+    bsto(mex, offA, stA && dEx, exA);
+    bsto(mey, offA, stA && dEy, eyA);
+    bsto(mez, offA, stA && dEz, ezA);
+    bsto(mbx, offA, stA && dBx, b1xA);
+    bsto(mby, offA, stA && dBy, b1yA);
+    const T bxB = b1xc + dtbB * ((e2y - eyB) * rdzB - (sEzB[lyp][lx] - ezB) * rdy);
+    const T byB = b1yc + dtbB * ((sEzB[ly][lxp] - ezB) * rdx - (e2x - exB) * rdzB);
+    bsto(rbx, offB, stB && dBx, bxB);
+    bsto(rby, offB, stB && dBy, byB);
+    e2x = exB;
+    e2y = eyB;
+    b1xc = b1xn;
+    b1yc = b1yn;
+    b1xn = b1xA;
+    b1yn = b1yA;
+    e1xc = e1xn;
+    e1yc = e1yn;
+    e1zc = e1zn;
+    e1xn = exA;
+    e1yn = eyA;
+    e1zn = ezA;
+    azc = azn;
+    axc = axn;
+    ayc = ayn;
+    azn = azA;
+    axn = axA;
+    ayn = ayA;
+    // no barrier here: the next iteration's first phase writes only the Bz sums and the staging
+    // buffers of level k-2, which no wave reads after this iteration's second barrier, and reads
+    // the intermediate B of level k+1, which this phase does not write
+  };
+  int k = khi;
+  for (; k - (S - 1) >= 1; k -= S) {
+#pragma unroll
+    for (int u = 0; u < S; ++u) level(k - u, R[u], R[(u + 1) % S], R[(u + S - 1) % S]);
+  }
+#pragma unroll
+  for (int u = 0; u < S - 1; ++u)
+    if (k - u >= 1) level(k - u, R[u], R[(u + 1) % S], R[(u + S - 1) % S]);
+  // the running sums for the top launches: step A's through level 2, step B's through level 3
+  if (own && xin && yin) {
+    bzA.save(a.cs + po, a.cc + po, colb);
+    bzB.save(pa.cs2 + po, pa.cc2 + po, colb);
+  }
+}
+
 // Level 0: the air cells count as sigma = 0 and the level above is BxAir/ByAir at -dz0/2
 // (ElectricalFiled.cu:107-139, 204-234).
 template <typename T, int BY, bool GEO>
@@ -936,6 +1185,46 @@ hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEven
   }
   return hipErrorInvalidValue;
 }
+int pair_rows_supported(int prec, int rows) {
+  (void)prec;
+  return rows == 12 || rows == 16 ? rows : 0;
+}
+template <typename T, int BY, int PF, bool S8>
+hipError_t launch_march2_v(const PairArgs<T> &p, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  const size_t zl = (size_t)3 * p.a.N * sizeof(T);
+  static size_t allowed = 0;  // per instantiation; only grows
+  if (zl > allowed) {
+    const hipError_t e = hipFuncSetAttribute((const void *)k_march2<T, BY, PF, S8>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)zl);
+    if (e != hipSuccess) return e;
+    allowed = zl;
+  }
+  dim3 blk(kMarchLanes, BY), grd(8 * p.nper, nslot);
+  launch_k(k_march2<T, BY, PF, S8>, grd, blk, (std::uint32_t)zl, s, e0, e1, p);
+  return hipGetLastError();
+}
+template <typename T, int BY>
+hipError_t launch_march2_by(const PairArgs<T> &p, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  const bool s8 = p.a.sig8 != nullptr;
+  switch (p.a.pf) {
+    case 1:
+      return s8 ? launch_march2_v<T, BY, 1, true>(p, nslot, s, e0, e1)
+                : launch_march2_v<T, BY, 1, false>(p, nslot, s, e0, e1);
+    case 2:
+      return s8 ? launch_march2_v<T, BY, 2, true>(p, nslot, s, e0, e1)
+                : launch_march2_v<T, BY, 2, false>(p, nslot, s, e0, e1);
+  }
+  return hipErrorInvalidValue;
+}
+template <typename T>
+hipError_t launch_march2(const PairArgs<T> &p, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  if (p.a.N < 6 || p.ntx < 1 || p.nty < 1) return hipErrorInvalidValue;
+  switch (p.rows) {
+    case 12: return launch_march2_by<T, 12>(p, nslot, s, e0, e1);
+    case 16: return launch_march2_by<T, 16>(p, nslot, s, e0, e1);
+  }
+  return hipErrorInvalidValue;
+}
 template <typename T>
 hipError_t launch_zsum(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEvent_t e0) {
   launch_k(k_zsum<T>, dim3((a.L + 255) / 256, a.M, (a.nchunk - 1) * nslot), dim3(256), 0u, s, e0, nullptr, a);
@@ -995,6 +1284,8 @@ template hipError_t launch_march<float>(const MarchArgs<float> &, int, hipStream
                                         const MarchArgs<float> *);
 template hipError_t launch_march<double>(const MarchArgs<double> &, int, hipStream_t, hipEvent_t, hipEvent_t,
                                          const MarchArgs<double> *);
+template hipError_t launch_march2<float>(const PairArgs<float> &, int, hipStream_t, hipEvent_t, hipEvent_t);
+template hipError_t launch_march2<double>(const PairArgs<double> &, int, hipStream_t, hipEvent_t, hipEvent_t);
 template hipError_t launch_surface<float>(const MarchArgs<float> &, int, hipStream_t, hipEvent_t, hipEvent_t,
                                           const MarchArgs<float> *);
 template hipError_t launch_surface<double>(const MarchArgs<double> &, int, hipStream_t, hipEvent_t, hipEvent_t,
