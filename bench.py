@@ -6,13 +6,16 @@ surface level, and the receiver gathers the sampling schedule asks for).  The he
 the largest single-GPU configuration, large_5: 512 x 512 x 256 cells, a seeded layered model with
 two reservoirs under rough bathymetry, the first of its three x-dipoles, 401 receivers.  Inputs
 are resident in HBM before the timed region.  The value is its FP32 Gcell-updates/s over exactly
-`--steps` steps after `--warmup`; the same line carries its FP64 throughput over the same window,
+`--steps` steady-state steps: the source wavelet (the first 200 steps, which also run the source
+pass) and then `--warmup` steps run untimed before them.  The same line carries its FP64
+throughput over the same window,
 the 256 x 256 x 160 marine model in FP32 and FP64, the FP32 transmitter batch (32 x-dipoles on
 that grid), 64 conductivity models of uneven grids advanced as one model batch (FP32 and FP64),
 complete runs of the small 64 x 64 x 48 model, the roofline of the volume march (HIP events on the
-library's stream), the LDS and HBM rates of the air-plane chain and of the receiver gathers, and
-the FP64 CPU oracle timed on this host's cores as the baseline.  Every GPU run first lets the
-library's autotuner pick the march tiling for its batch size.
+library's stream; with two steps per pass, the pass kernel's), the LDS and HBM rates of the
+air-plane chain and of the receiver gathers, and the FP64 CPU oracle timed on this host's cores as
+the baseline.  Every GPU run first lets the library's autotuner pick the march tiling for its
+batch size, and whether steady-state steps run two per pass.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B] [--no-cpu] [--no-tune]
     python bench.py --accuracy   # complete batched runs, FP32 vs FP64 trace errors
@@ -73,17 +76,23 @@ def gpu_run(T, deck, precision, nbatch, steps, warmup, tune=True):
         txs = [T.make_tx(**dict(t, txid=i + 1)) for i, t in enumerate(deck.txs)]
         c.set_transmitters(txs)
         c.begin(0, nbatch)
+        # the source wavelet (its 2 (n_src/2) steps also run the source pass) before the warm-up,
+        # untimed: the timed steps are steady-state steps, 99 % of a run's steps
+        c.step(source_steps(deck))
         c.step(warmup)
         c.sync()
         # kernel times come from HIP events recorded on the library's streams inside the timed
         # region (read after it)
         c.enable_timing(True)
+        pairs0 = c.pair_info()[2]
         with timed_range("timed %s fp%d x%d" % (deck.name, precision, nbatch)):
             t0 = time.perf_counter()
             c.step(steps)
             c.sync()
             t1 = time.perf_counter()
         tm = c.timing()
+        pass_ms, npass = c.pair_timing()
+        rows, pf, pairs1 = c.pair_info()
         c.enable_timing(False)
         ex = c.get(T.EX, 0)
     cells = c.L * c.M * c.N
@@ -93,7 +102,14 @@ def gpu_run(T, deck, precision, nbatch, steps, warmup, tune=True):
         "ms_per_step": 1000.0 * sec / steps,
         "timing": tm, "L": c.L, "M": c.M, "N": c.N, "ex_norm": float(np.linalg.norm(ex)),
         "ex": ex, "tiling": tiling,
+        "pairs": {"rows": rows, "pf": pf, "pairs_in_window": pairs1 - pairs0, "pass_ms": pass_ms,
+                  "passes_timed": npass} if rows and pairs1 > pairs0 else None,
     }
+
+
+def source_steps(deck):
+    """Steps of the source wavelet (main.cu:223: amp from the waveform while n < 2 (n_src/2))."""
+    return 2 * (deck.setup["n_src"] // 2)
 
 
 def full_run(T, deck, precision, graph=None):
@@ -135,10 +151,11 @@ def models_run(T, decks, precision, steps, warmup, tune=True, single_steps=0):
     cells = sum(c.L * c.M * c.N for c in ctxs)
     out = {"models": len(ctxs), "cells": cells}
     try:
+        lead = max(source_steps(d) for d in decks)  # the source wavelet, untimed
         if single_steps > 0:
             for c in ctxs:
                 c.begin(0, 1)
-                c.step(warmup)
+                c.step(lead + warmup)
             for c in ctxs:
                 c.sync()
             t0 = time.perf_counter()
@@ -151,7 +168,7 @@ def models_run(T, decks, precision, steps, warmup, tune=True, single_steps=0):
                                 "ms_per_step": 1000.0 * sec / single_steps, "steps": single_steps}
         with T.Batch(ctxs) as b:
             out["tiling"] = b.autotune(5) if tune else None
-            b.step(warmup)
+            b.step(lead + warmup)
             b.sync()
             b.enable_timing(True)
             with timed_range("timed model batch fp%d x%d" % (precision, len(ctxs))):
@@ -183,20 +200,49 @@ def models_roofline(res, esz, traffic_entry=(None, None)):
 
 
 def roofline(res, esz, nbatch, traffic_entry):
-    # the march advances levels 1..N-1 of every column: 13 words per cell, sigma shared by the
-    # batch ((12 + 1/B) words per cell-update)
+    # the one-step march advances levels 1..N-1 of every column: 13 words per cell, sigma shared by
+    # the batch ((12 + 1/B) words per cell-update)
     cells = res["L"] * res["M"] * (res["N"] - 1) * nbatch
     words = (12.0 + 1.0 / nbatch) if nbatch > 1 else WORDS_PER_CELL
+    traffic, traffic_tiling = traffic_entry if isinstance(traffic_entry, tuple) else (traffic_entry, None)
+    pr = res.get("pairs")
+    if pr and pr["passes_timed"] > 0:
+        # the two-step pass (k_march2) dominates: per launch it reads sigma and the old state and
+        # writes the new state of every cell once for two cell-updates, 10 words + sigma (11, or
+        # 10 + 1/B shared by the batch); its top launches (levels 2 and 1) are separate launches
+        pwords = 10.0 + 1.0 / nbatch
+        algo = cells * pwords * esz
+        ms = pr["pass_ms"]
+        achieved = algo / (ms * 1e-3) / 1e9
+        step_ms = res["timing"]["march_ms"]
+        equiv = cells * words * esz / (step_ms * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+                "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                "traffic_tiling": traffic_tiling,
+                "kernel": "k_march2 (two steps per pass, rows %d, prefetch %d)" % (pr["rows"], pr["pf"]),
+                "algorithmic_bytes_per_launch": int(algo), "cell_updates_per_launch": 2 * cells,
+                "avg_launch_ms": round(ms, 5),
+                "march_ms_per_step": round(step_ms, 5),
+                "one_step_equivalent": {"GBs": round(equiv, 1), "frac": round(equiv / PEAK_HBM_GBS, 4),
+                                        "note": "the one-step march's %.4g words per cell-update over the "
+                                                "march's time per step (pass and top launches, halved)" % words},
+                "span": "per launch of k_march2; march_ms_per_step adds the top launches of both steps"}
     algo_bytes = cells * words * esz
     ms = res["timing"]["march_ms"]
+    if not ms > 0:  # no step of the window was timed
+        return None
     achieved = algo_bytes / (ms * 1e-3) / 1e9
-    traffic, traffic_tiling = traffic_entry if isinstance(traffic_entry, tuple) else (traffic_entry, None)
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
             "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
             "traffic_tiling": traffic_tiling,
             "kernel": "k_march", "algorithmic_bytes_per_launch": int(algo_bytes),
             "avg_launch_ms": round(ms, 5),
             "span": "per step: the lower and the top launch of k_march, the lower one from the start of its k_zsum pass when the tiling runs one"}
+
+
+def traffic_key(key, res):
+    """The counter summary's key for a run: the two-step pass has entries of its own."""
+    return key + "_pairs" if res.get("pairs") else key
 
 
 def _fft_len(n):
@@ -464,10 +510,11 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=500)
-    # the untimed warm-up covers the source wavelet (the first n_src = 200 steps of these decks,
-    # where a step also runs the source passes): the timed steps are the steady-state steps that
-    # make up 99 % of a run (18 802 steps to the last sample of the 256 x 256 x 160 model)
-    ap.add_argument("--warmup", type=int, default=200)
+    # untimed steps after the source wavelet (the first n_src = 200 steps of these decks, where a
+    # step also runs the source passes, which always run untimed first): the timed steps are the
+    # steady-state steps that make up 99 % of a run (18 802 steps to the last sample of the
+    # 256 x 256 x 160 model)
+    ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--batch", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="heuristic march tiling instead of the autotuner")
@@ -503,7 +550,8 @@ def main():
     r32.pop("ex"), r64.pop("ex")
     # 32 x-dipoles along a towline on the same grid, FP32 batched
     tow = W.towline(T, ntx=args.batch)
-    rb = gpu_run(T, tow, T.FP32, args.batch, max(args.steps // 5, 1), args.warmup, tune)
+    nb = max(args.steps // 2, 8)  # at least one timed pass or step (every 8th step is timed)
+    rb = gpu_run(T, tow, T.FP32, args.batch, nb, args.warmup, tune)
     rb.pop("ex")
     # 64 conductivity models of uneven grids (96^3 .. 224x224x160), one x-dipole each, in one
     # model batch: FP32 and FP64, FP32 also stepped model by model for comparison
@@ -534,29 +582,29 @@ def main():
         "config": {"workload": "large_5 512x512x256, 1024x1024 air plane, x-dipole 1 of 3, 401 receivers, FP32 fields",
                    "cells": rl32["L"] * rl32["M"] * rl32["N"], "batch": 1, "parallelism": "single GPU",
                    "march_tiling": rl32["tiling"]},
-        "roofline": roofline(rl32, 4, 1, load_traffic("large_5_fp32")),
+        "roofline": roofline(rl32, 4, 1, load_traffic(traffic_key("large_5_fp32", rl32))),
         "kernel_ms": kern(rl32),
         "side_kernels": side_kernels(rl32, big, 4, 1),
         "fp64_single": {"workload": "large_5 512x512x256, FP64 fields, the same window",
                         "value": round(rl64["gcell_s"], 3), "ms_per_step": round(rl64["ms_per_step"], 5),
-                        "roofline": roofline(rl64, 8, 1, load_traffic("large_5_fp64")),
+                        "roofline": roofline(rl64, 8, 1, load_traffic(traffic_key("large_5_fp64", rl64))),
                         "march_tiling": rl64["tiling"], "kernel_ms": kern(rl64),
                         "fp32_vs_fp64_ex_rel_l2_after_window": big_err},
         "marine_3d": {"workload": "marine_3d 256x256x160, 512x512 air plane, 1 x-dipole, 101 receivers",
                       "steps": args.steps, "warmup": args.warmup,
                       "fp32": {"value": round(r32["gcell_s"], 3), "ms_per_step": round(r32["ms_per_step"], 5),
-                               "roofline": roofline(r32, 4, 1, load_traffic("marine_3d_fp32")),
+                               "roofline": roofline(r32, 4, 1, load_traffic(traffic_key("marine_3d_fp32", r32))),
                                "march_tiling": r32["tiling"], "kernel_ms": kern(r32),
                                "side_kernels": side_kernels(r32, deck, 4, 1)},
                       "fp64": {"value": round(r64["gcell_s"], 3), "ms_per_step": round(r64["ms_per_step"], 5),
-                               "roofline": roofline(r64, 8, 1, load_traffic("marine_3d_fp64")),
+                               "roofline": roofline(r64, 8, 1, load_traffic(traffic_key("marine_3d_fp64", r64))),
                                "march_tiling": r64["tiling"], "kernel_ms": kern(r64)},
                       "fp32_vs_fp64_ex_rel_l2_after_window": field_err},
         "fp32_batched": {"value": round(rb["gcell_s"], 3), "batch": args.batch,
                          "workload": "towline %d x-dipoles, marine_3d grid" % args.batch,
-                         "steps": max(args.steps // 5, 1), "warmup": args.warmup,
+                         "steps": nb, "warmup": args.warmup,
                          "ms_per_step": round(rb["ms_per_step"], 5),
-                         "roofline": roofline(rb, 4, args.batch, load_traffic("towline32_fp32")),
+                         "roofline": roofline(rb, 4, args.batch, load_traffic(traffic_key("towline32_fp32", rb))),
                          "march_tiling": rb["tiling"], "kernel_ms": kern(rb),
                          "side_kernels": side_kernels(rb, tow, 4, args.batch)},
         "uneven_models": {"workload": "64 models, grids 96^3 .. 224x224x160 (%d cells), 1 x-dipole and 41 receivers each"

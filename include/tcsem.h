@@ -155,11 +155,13 @@ int tcsem_reset_fields(tcsem_ctx *ctx, int first, int n);
 /* Autotuner of the volume march (no counterpart in the reference, whose launch shapes are
  * fixed): times nsteps nbatch-transmitter steps as they run in production (the march beside the
  * upward continuation and surface level on the second stream; no source, no gathers) for every
- * tiling the library runs -- rows per workgroup 4/8/10/16 (FP64: 4/8/12), prefetch depth 1-4, z-chunks
- * of the lower launch 1/2/3/4/6/8 as the depth allows, and for chunked tilings the source of the
- * chunks' Bz partial sums (a pass over Bx, By, or the chunk sums the previous step's march
- * accumulated) -- and keeps the fastest for later batches of that size (TCSEM_MARCH_ROWS / _PF /
- * _ZCHUNKS / _ZSUM still override it).  It runs on the fields of slots [0, nbatch): a begun run
+ * tiling the library runs -- rows per workgroup 4/8/10/16 (FP64: 4/8/10/12), prefetch depth 1-4,
+ * z-chunks of the lower launch 1/2/3/4/6/8 as the depth allows, for chunked tilings the source of
+ * the chunks' Bz partial sums (a pass over Bx, By, or the chunk sums the previous step's march
+ * accumulated), and the 8-bit conductivity index on and off -- then, with that tiling for the top
+ * levels, pairs of steps in the two-step pass (rows 8/12/16, prefetch 1-2; tcsem_set_pairs), kept
+ * when 2% faster per step -- and keeps the choice for later batches of that size
+ * (TCSEM_MARCH_ROWS / _PF / _ZCHUNKS / _ZSUM / _PAIRS and tcsem_set_pairs still override it).  It runs on the fields of slots [0, nbatch): a begun run
  * is discarded and the fields are left zeroed.  Outputs (nullable): the chosen rows, prefetch
  * depth, z-chunks, partial-sum source (1 chunk sums, 0 pass) and the time per step [ms]. */
 int tcsem_autotune(tcsem_ctx *ctx, int nbatch, int nsteps, int *rows, int *pf, int *zchunks, int *zsum,
@@ -208,10 +210,11 @@ int tcsem_upcont_plane_noalloc(tcsem_ctx *ctx, const void *d_bz0, void *d_bxair,
 
 /* Average device time [ms] of the kernels of the steps taken while timing was enabled, from HIP
  * events that the kernels' dispatch packets record on the context's streams as the steps run
- * (every 8th step; no extra synchronisation, reading waits for them): t[0] volume march per step
- * (lower and top launch; the lower from the start of its k_zsum pass when the step runs one),
- * t[1] surface level, t[2] upward continuation chain, t[3] receiver
- * gather per launch; n[i] counts. */
+ * (every 8th step, or the pair of steps holding it; no extra synchronisation, reading waits for
+ * them): t[0] volume march per step (lower and top launch; the lower from the start of its
+ * k_zsum pass when the step runs one; for a pair of steps half of the two-step pass and both top
+ * launches), t[1] surface level, t[2] upward continuation chain, t[3] receiver gather per launch;
+ * n[i] counts. */
 int tcsem_timing(const tcsem_ctx *ctx, double t[4], long n[4]);
 int tcsem_enable_timing(tcsem_ctx *ctx, int on);
 int tcsem_sync(tcsem_ctx *ctx);
@@ -236,12 +239,15 @@ long tcsem_graph_replays(const tcsem_ctx *ctx);
 int tcsem_sigma_table(const tcsem_ctx *ctx);
 
 /* Two steps per lower pass (no counterpart in the reference, whose steps each read and write
- * the whole state): with rows 12 or 16 (workgroup rows of the pass) and pf 1 or 2 (prefetch
+ * the whole state): with rows 8, 12 or 16 (workgroup rows of the pass) and pf 1 or 2 (prefetch
  * depth), tcsem_step and tcsem_run take each two consecutive steps after the source wavelet
  * whose first holds no receiver sample as one pass over levels N-1 .. 3 of both steps (the
  * intermediate state stays on chip there), then the top levels of each step as usual.  The
- * results equal, bit for bit, those of one step at a time (tested).  rows 0 turns it off (the
- * default); graph replays, where on, take precedence.  Needs N >= 6.
+ * results equal, bit for bit, those of one step at a time with an unchunked march (tested).
+ * rows 0 turns it off.  Without a call the library decides at tcsem_begin: the autotuner's choice
+ * for the batch size, else TCSEM_MARCH_PAIRS ("0" or "rows,pf"), else 12-row passes where they
+ * give at least two workgroups per CU; a call holds for the context's later runs.  Graph replays,
+ * where on, take precedence.  Needs N >= 6.
  * tcsem_pair_info reports the setting and the pairs taken so far; tcsem_pair_timing the average
  * device time [ms] of the two-step pass alone (timing enabled; its top launches count in
  * tcsem_timing's t[0], per step). */

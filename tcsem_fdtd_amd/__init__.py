@@ -351,8 +351,10 @@ class Context:
         r, p, z, f, ms = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_double()
         _check(self.L_.tcsem_autotune(self.h, nbatch, nsteps, C.byref(r), C.byref(p), C.byref(z),
                                       C.byref(f), C.byref(ms)))
+        pr, ppf, _ = self.pair_info()
         return {"rows": r.value, "pf": p.value, "zchunks": z.value,
-                "zsum": "fused" if f.value else "pass", "sigma8": self.sigma_table() > 0, "march_ms": ms.value}
+                "zsum": "fused" if f.value else "pass", "sigma8": self.sigma_table() > 0,
+                "pairs": [pr, ppf] if pr else None, "march_ms": ms.value}
 
     def get(self, comp, slot=0):
         a = np.zeros(self.shape(comp))
@@ -394,8 +396,9 @@ class Context:
         return self.L_.tcsem_sigma_table(self.h)
 
     def set_pairs(self, rows, pf=1):
-        """Take two steady-state steps per lower pass (rows 12 or 16, prefetch pf 1 or 2; rows 0: off,
-        the default).  Results equal one step at a time, bit for bit."""
+        """Take two steady-state steps per lower pass (rows 8, 12 or 16, prefetch pf 1 or 2; rows 0:
+        off), over the library's own choice.  Results equal one step at a time (unchunked), bit for
+        bit."""
         _check(self.L_.tcsem_set_pairs(self.h, rows, pf))
 
     def pair_info(self):

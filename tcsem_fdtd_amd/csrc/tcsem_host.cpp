@@ -499,6 +499,8 @@ struct tcsem_ctx {
   // k_march2, its tiles, and the intermediate state (5 fields x nslots x fstride, allocated when
   // first used; only levels 0 .. 2 are written)
   int pair_by = 0, pair_pf = 1, pair_ntx = 0, pair_nty = 0, pair_nper = 0;
+  int pair_set = -1, pair_set_pf = 1;  // tcsem_set_pairs' choice (-1: none), over any other
+  int tune_pair_by = 0, tune_pair_pf = 1;  // the autotuner's, for its batch size tune_nbatch
   char *d_mid = nullptr;
   long pairs_done = 0;
   struct Graph {
@@ -554,6 +556,17 @@ void apply_tiling(tcsem_ctx *c, int by, int pf, int nch) {
   c->sper = (c->stx * c->sty + 7) / 8;
 }
 
+// Tiles of the two-step pass (rows 0: pairs off).
+void pair_tiling(tcsem_ctx *c, int by, int pf) {
+  const int L = c->g.info.L, M = c->g.info.M;
+  c->pair_by = by;
+  c->pair_pf = pf;
+  if (!by) return;
+  c->pair_ntx = (L + kMarchLanes - 5) / (kMarchLanes - 4);
+  c->pair_nty = (M + by - 5) / (by - 4);
+  c->pair_nper = (c->pair_ntx * c->pair_nty + 7) / 8;
+}
+
 // Tiling of an nbatch-transmitter step: the autotuned one for this batch size when there is
 // one, else the heuristic (10-row tiles for FP32 and 12-row tiles for FP64 at prefetch 1, the
 // autotuner's pick on every single model and transmitter batch measured; 4-row tiles at prefetch
@@ -603,6 +616,31 @@ void choose_tiling(tcsem_ctx *c, int nbatch) {
     if (v >= 1) nch = v;
   }
   apply_tiling(c, by, pf, nch);
+  // Two steps per pass: tcsem_set_pairs' choice, else TCSEM_MARCH_PAIRS ("0", or "rows,pf"), else
+  // the autotuner's for this batch size, else 12-row passes at prefetch 1 when they give at least
+  // two workgroups per CU (faster than one step at a time on every such workload measured; fewer
+  // leave CUs idle, and the one-step march's z-chunks serve those grids better).
+  int pby = 0, ppf = 1;
+  if ((long long)((L + kMarchLanes - 5) / (kMarchLanes - 4)) * ((M + 7) / 8) * nbatch >= 2LL * cu) pby = 12;
+  if (c->tune_nbatch == nbatch) {
+    pby = c->tune_pair_by;
+    ppf = c->tune_pair_pf;
+  }
+  if (const char *e = std::getenv("TCSEM_MARCH_PAIRS")) {
+    int r = 0, q = 1;
+    if (std::sscanf(e, "%d,%d", &r, &q) >= 1 && (r == 0 || (pair_rows_supported(c->prec, r) && q >= 1 && q <= 2))) {
+      pby = r;
+      ppf = q;
+    } else {
+      std::fprintf(stderr, "tcsem: TCSEM_MARCH_PAIRS=%s is not 0 or rows,pf (rows 8, 12, 16; pf 1, 2)\n", e);
+    }
+  }
+  if (c->pair_set >= 0) {
+    pby = c->pair_set;
+    ppf = c->pair_set_pf;
+  }
+  if (N < 6) pby = 0;
+  pair_tiling(c, pby, ppf);
 }
 
 template <typename T>
@@ -1502,16 +1540,6 @@ int graph_steps(tcsem_ctx *c) {
 // go to stream2 and overlap the next pass.  The pass reads the old state at levels >= 1 only and
 // writes the intermediate state at level 2 only, so it too runs beside the previous step's
 // stream2 work.
-void pair_tiling(tcsem_ctx *c, int by, int pf) {
-  const int L = c->g.info.L, M = c->g.info.M;
-  c->pair_by = by;
-  c->pair_pf = pf;
-  if (!by) return;
-  c->pair_ntx = (L + kMarchLanes - 5) / (kMarchLanes - 4);
-  c->pair_nty = (M + by - 5) / (by - 4);
-  c->pair_nper = (c->pair_ntx * c->pair_nty + 7) / 8;
-}
-
 // Whether the next two steps may run as a pair: pairs on, past the source (no amplitude, no source
 // pass), and no receiver sample after the first of them (one after the second is taken from the
 // new state as usual).
@@ -1520,30 +1548,29 @@ bool pair_ok(tcsem_ctx *c, long room) {
          !events_ahead(c, 1);
 }
 
+hipError_t ensure_mid(tcsem_ctx *c) {
+  if (c->d_mid) return hipSuccess;
+  const size_t mb = (size_t)5 * c->nslots * c->fstride * c->esz;
+  hipError_t e = hipMalloc(&c->d_mid, mb);
+  if (e != hipSuccess) {
+    c->d_mid = nullptr;
+    return e;
+  }
+  return launch_fill_zero(c->d_mid, mb, c->stream);  // cells outside the grid stay zero
+}
+
+// The kernels of a pair of steps on nb slots: a carries step A's scalars (old state -> new state
+// pointers), sb step B's; te: null or 14 events (start/end of the pass, step A's top launch,
+// chain and surface, step B's top launch, chain and surface).  Stream2's work of the pair is left
+// unjoined.
 template <typename T>
-int do_pair(tcsem_ctx *c) {
-  const int nb = c->nbatch;
+hipError_t enqueue_pair(tcsem_ctx *c, const MarchArgs<T> &a, const MarchArgs<T> &sb, int nb, const hipEvent_t *te) {
+  static const hipEvent_t none[14] = {};
+  if (!te) te = none;
   hipStream_t st = c->stream, st2 = c->stream2;
   const UpcontArgs<T> &u = up_of<T>(c);
-  if (!c->d_mid) {
-    const size_t mb = (size_t)5 * c->nslots * c->fstride * c->esz;
-    if (hipMalloc(&c->d_mid, mb) != hipSuccess) return fail(TCSEM_ENOMEM, "intermediate-state allocation failed");
-    HIPCHK(launch_fill_zero(c->d_mid, mb, st));  // cells outside the grid stay zero
-  }
-  MarchArgs<T> a = march_args<T>(c);  // old state -> new state, step A's scalars
-  step_scalars<T>(c, a);
-  MarchArgs<T> sb{};  // step B's scalars: the time state after step A
-  {
-    const int n0 = c->n, st0 = c->steps;
-    const double toT0 = c->toT, d10 = c->dt1, d20 = c->dt2;
-    advance_time(c);
-    step_scalars<T>(c, sb);
-    c->n = n0;
-    c->steps = st0;
-    c->toT = toT0;
-    c->dt1 = d10;
-    c->dt2 = d20;
-  }
+  hipError_t e;
+  if ((e = ensure_mid(c)) != hipSuccess) return e;
   const long long org = c->PX + c->xm, fs = c->fstride * c->nslots, ps = c->pstride * c->nslots;
   T *fm = (T *)c->d_mid, *pl = (T *)c->d_planes;
   PairArgs<T> p{};
@@ -1587,31 +1614,50 @@ int do_pair(tcsem_ctx *c) {
   tb.carry = 1;
   tb.cs = p.cs2;
   tb.cc = p.cc2;
-  // every pair that starts in the first two steps of a timing stride is timed
-  const bool timed = c->timing && c->steps % kTimingStride < 2;
+  if ((e = launch_march2<T>(p, nb, st, te[0], te[1])) != hipSuccess) return e;
+  if ((e = hipStreamWaitEvent(st, c->xev[1], 0)) != hipSuccess) return e;  // the previous step's stream2 work
+  if ((e = launch_march<T>(ta, nb, st, te[2], te[3])) != hipSuccess) return e;
+  if ((e = launch_upcont<T>(u, nb, st, te[4], te[5])) != hipSuccess) return e;
+  if ((e = launch_surface<T>(ta, nb, st, te[6], te[7])) != hipSuccess) return e;
+  if ((e = launch_march<T>(tb, nb, st, te[8], te[9])) != hipSuccess) return e;
+  if ((e = hipEventRecord(c->xev[0], st)) != hipSuccess) return e;
+  if ((e = hipStreamWaitEvent(st2, c->xev[0], 0)) != hipSuccess) return e;
+  if ((e = launch_upcont<T>(u, nb, st2, te[10], te[11])) != hipSuccess) return e;
+  return launch_surface<T>(tb, nb, st2, te[12], te[13]);
+}
+
+template <typename T>
+int do_pair(tcsem_ctx *c) {
+  MarchArgs<T> a = march_args<T>(c);  // old state -> new state, step A's scalars
+  step_scalars<T>(c, a);
+  MarchArgs<T> sb{};  // step B's scalars: the time state after step A
+  {
+    const int n0 = c->n, st0 = c->steps;
+    const double toT0 = c->toT, d10 = c->dt1, d20 = c->dt2;
+    advance_time(c);
+    step_scalars<T>(c, sb);
+    c->n = n0;
+    c->steps = st0;
+    c->toT = toT0;
+    c->dt1 = d10;
+    c->dt2 = d20;
+  }
+  // a pair is timed when one of its steps would be (every kTimingStride-th step)
+  const bool timed = c->timing && (c->steps % kTimingStride == 0 || (c->steps + 1) % kTimingStride == 0);
   hipEvent_t te[14] = {};
   if (timed) {
     if (c->spans.size() > 50000 && (rc_collect(c))) return TCSEM_EDEVICE;
     for (auto &e : te)
       if (!(e = timing_event(c))) return fail(TCSEM_EDEVICE, "timing event creation failed");
   }
-  HIPCHK(launch_march2<T>(p, nb, st, te[0], te[1]));
-  HIPCHK(hipStreamWaitEvent(st, c->xev[1], 0));  // the previous step's stream2 work
-  HIPCHK(launch_march<T>(ta, nb, st, te[2], te[3]));
-  HIPCHK(launch_upcont<T>(u, nb, st, te[4], te[5]));
-  HIPCHK(launch_surface<T>(ta, nb, st, te[6], te[7]));
-  HIPCHK(launch_march<T>(tb, nb, st, te[8], te[9]));
-  HIPCHK(hipEventRecord(c->xev[0], st));
-  HIPCHK(hipStreamWaitEvent(st2, c->xev[0], 0));
-  HIPCHK(launch_upcont<T>(u, nb, st2, te[10], te[11]));
-  HIPCHK(launch_surface<T>(tb, nb, st2, te[12], te[13]));
+  HIPCHK(enqueue_pair<T>(c, a, sb, c->nbatch, timed ? te : nullptr));
   c->cur ^= 1;
   c->ib_valid = false;
   advance_time(c);
-  if (int rc = sample_rx<T>(c, st2, false)) return rc;  // no sample fires after step A
+  if (int rc = sample_rx<T>(c, c->stream2, false)) return rc;  // no sample fires after step A
   advance_time(c);
-  if (int rc = sample_rx<T>(c, st2, timed)) return rc;
-  HIPCHK(hipEventRecord(c->xev[1], st2));
+  if (int rc = sample_rx<T>(c, c->stream2, timed)) return rc;
+  HIPCHK(hipEventRecord(c->xev[1], c->stream2));
   ++c->pairs_done;
   if (timed) {
     // the march of the two steps: the pass and both top launches (two steps counted)
@@ -1783,6 +1829,67 @@ int autotune_march(tcsem_ctx *c, int nbatch, int nsteps, int *best_by, int *best
   *best_s8 = pick->s8;
   *best_ms = pick->ms;
   return TCSEM_OK;
+}
+
+// Times pairs of steps as they run in production (the two-step pass, then the top levels of both
+// steps, step B's chain and surface beside the next pass) for each pass shape, with the one-step
+// tiling already applied for the top launches; returns the fastest shape and its time per step
+// (rows 0: none timed).
+template <typename T>
+int autotune_pairs(tcsem_ctx *c, int nbatch, int nsteps, int *best_by, int *best_pf, double *best_ms) {
+  *best_by = 0;
+  *best_pf = 1;
+  *best_ms = 1e300;
+  if (c->g.info.N < 6) return TCSEM_OK;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(TCSEM_EDEVICE, "event creation failed");
+  }
+  hipStream_t st = c->stream;
+  const int cur0 = c->cur, pby0 = c->pair_by, ppf0 = c->pair_pf;
+  auto one_pair = [&]() -> hipError_t {
+    MarchArgs<T> a = march_args<T>(c);
+    a.lam = 1.0;
+    a.dt = 1e-6;
+    a.dtb = 1e-6;
+    a.amp = 0.0;
+    hipError_t e = enqueue_pair<T>(c, a, a, nbatch, nullptr);
+    if (e == hipSuccess) e = hipEventRecord(c->xev[1], c->stream2);
+    c->cur ^= 1;
+    return e;
+  };
+  const int npair = std::max(1, nsteps / 2);
+  int rc = TCSEM_OK;
+  for (int by : {12, 16, 8})
+    for (int pf = 1; pf <= 2 && rc == TCSEM_OK; ++pf) {
+      if (!pair_rows_supported(c->prec, by)) continue;
+      pair_tiling(c, by, pf);
+      hipError_t e = one_pair();  // the first launches of the shape, untimed
+      if (e == hipSuccess) e = hipStreamWaitEvent(st, c->xev[1], 0);
+      if (e == hipSuccess) e = hipEventRecord(e0, st);
+      for (int q = 0; q < npair && e == hipSuccess; ++q) e = one_pair();
+      if (e == hipSuccess) e = hipStreamWaitEvent(st, c->xev[1], 0);
+      if (e == hipSuccess) e = hipEventRecord(e1, st);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) {
+        rc = fail(TCSEM_EDEVICE, "autotune: %s", hipGetErrorString(e));
+        break;
+      }
+      const double per = (double)ms / (2.0 * npair);
+      if (per < *best_ms) {
+        *best_ms = per;
+        *best_by = by;
+        *best_pf = pf;
+      }
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  c->cur = cur0;
+  pair_tiling(c, pby0, ppf0);
+  return rc;
 }
 
 // ---- model batch ----------------------------------------------------------------------------------
@@ -3070,6 +3177,30 @@ int tcsem_autotune(tcsem_ctx *c, int nbatch, int nsteps, int *rows, int *pf, int
   rc = c->prec == 64 ? autotune_march<double>(c, nbatch, nsteps, &by, &p, &nch, &fuse, &s8, &t)
                      : autotune_march<float>(c, nbatch, nsteps, &by, &p, &nch, &fuse, &s8, &t);
   if (rc) return rc;
+  // two steps per pass, with the one-step tiling just picked for its top launches: kept when at
+  // least 2% faster per step
+  int pby = 0, ppf = 1;
+  {
+    const int fuse0 = c->fuse;
+    const bool s80 = c->s8;
+    apply_tiling(c, by, p, nch);
+    c->fuse = fuse;
+    c->s8 = s8 != 0;
+    double pt = 0.0;
+    rc = c->prec == 64 ? autotune_pairs<double>(c, nbatch, nsteps, &pby, &ppf, &pt)
+                       : autotune_pairs<float>(c, nbatch, nsteps, &pby, &ppf, &pt);
+    c->fuse = fuse0;
+    c->s8 = s80;
+    if (rc) return rc;
+    if (pby && pt < 0.98 * t) {
+      t = pt;
+    } else {
+      pby = 0;
+      ppf = 1;
+    }
+  }
+  c->tune_pair_by = pby;
+  c->tune_pair_pf = ppf;
   if ((rc = tcsem_reset_fields(c, 0, nbatch))) return rc;
   c->tune_nbatch = nbatch;
   c->tune_by = by;
@@ -3332,8 +3463,10 @@ int tcsem_pair_timing(const tcsem_ctx *cc, double *ms, long *n) {
 int tcsem_set_pairs(tcsem_ctx *c, int rows, int pf) {
   if (check_ctx(c)) return TCSEM_EINVAL;
   if (rows != 0 && (!pair_rows_supported(c->prec, rows) || pf < 1 || pf > 2))
-    return fail(TCSEM_EINVAL, "two-step pass: rows 0 (off), 12 or 16; prefetch 1 or 2");
+    return fail(TCSEM_EINVAL, "two-step pass: rows 0 (off), 8, 12 or 16; prefetch 1 or 2");
   if (rows && c->g.info.N < 6) return fail(TCSEM_EINVAL, "two-step pass needs at least 6 levels");
+  c->pair_set = rows;
+  c->pair_set_pf = pf;
   pair_tiling(c, rows, pf);
   return TCSEM_OK;
 }

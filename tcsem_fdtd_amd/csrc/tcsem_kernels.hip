@@ -602,7 +602,8 @@ struct Lv2 {
   std::conditional_t<S8, unsigned, T> s;
 };
 template <typename T, int BY, int PF, bool S8>
-__global__ void __launch_bounds__(kMarchLanes *BY) k_march2(const PairArgs<T> pa) {
+__global__ void __launch_bounds__(kMarchLanes *BY)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && BY == 10 ? 5 : 1))) k_march2(const PairArgs<T> pa) {
   const MarchArgs<T> &a = pa.a;
   const int b = blockIdx.x;
   const int tile = (b & 7) * pa.nper + (b >> 3);  // XCD-contiguous bands of tiles
@@ -737,15 +738,18 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march2(const PairArgs<T> pa
       ayA = (T)0.25 * (sS[sp][ly][lxm] + sS[sp][ly][lx] + sS[sq][ly][lxm] + sS[sq][ly][lx]);
       {
         const T curl = (by_k - sBy[p][ly][lxm]) * rdx2 - (bx_k - sBx[p][lym][lx]) * rdy2;
-        ezA = dEz ? df_new(ezo, azA, cfA, curl) : zero;
+        const T v = df_new(ezo, azA, cfA, curl);
+        ezA = dEz ? v : zero;
       }
       {
         const T curl = (bzAk - sBzA[lym][lx]) * rdy2 - (by_k - by_m) * rdz2A;
-        exA = dEx ? df_new(exo, axA, cfA, curl) : zero;
+        const T v = df_new(exo, axA, cfA, curl);
+        exA = dEx ? v : zero;
       }
       {
         const T curl = (bx_k - bx_m) * rdz2A - (bzAk - sBzA[ly][lxm]) * rdx2;
-        eyA = dEy ? df_new(eyo, ayA, cfA, curl) : zero;
+        const T v = df_new(eyo, ayA, cfA, curl);
+        eyA = dEy ? v : zero;
       }
     }
     sEzA[ly][lx] = ezA;
@@ -753,15 +757,18 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march2(const PairArgs<T> pa
     {
       {
         const T curl = (b1yc - sB1y[p][ly][lxm]) * rdx2 - (b1xc - sB1x[p][lym][lx]) * rdy2;
-        ezB = bon && dEz ? df_new(e1zc, azc, cfB, curl) : zero;
+        const T v = df_new(e1zc, azc, cfB, curl);
+        ezB = bon && dEz ? v : zero;
       }
       {
         const T curl = (bzBk - sBzB[lym][lx]) * rdy2 - (b1yc - b1yn) * rdz2B;
-        exB = bon && dEx ? df_new(e1xc, axc, cfB, curl) : zero;
+        const T v = df_new(e1xc, axc, cfB, curl);
+        exB = bon && dEx ? v : zero;
       }
       {
         const T curl = (b1xc - b1xn) * rdz2B - (bzBk - sBzB[ly][lxm]) * rdx2;
-        eyB = bon && dEy ? df_new(e1yc, ayc, cfB, curl) : zero;
+        const T v = df_new(e1yc, ayc, cfB, curl);
+        eyB = bon && dEy ? v : zero;
       }
     }
     sEzB[ly][lx] = ezB;
@@ -1187,7 +1194,7 @@ hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEven
 }
 int pair_rows_supported(int prec, int rows) {
   (void)prec;
-  return rows == 12 || rows == 16 ? rows : 0;
+  return rows == 8 || rows == 10 || rows == 12 || rows == 16 ? rows : 0;
 }
 template <typename T, int BY, int PF, bool S8>
 hipError_t launch_march2_v(const PairArgs<T> &p, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
@@ -1220,6 +1227,8 @@ template <typename T>
 hipError_t launch_march2(const PairArgs<T> &p, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
   if (p.a.N < 6 || p.ntx < 1 || p.nty < 1) return hipErrorInvalidValue;
   switch (p.rows) {
+    case 8: return launch_march2_by<T, 8>(p, nslot, s, e0, e1);
+    case 10: return launch_march2_by<T, 10>(p, nslot, s, e0, e1);
     case 12: return launch_march2_by<T, 12>(p, nslot, s, e0, e1);
     case 16: return launch_march2_by<T, 16>(p, nslot, s, e0, e1);
   }

@@ -3,8 +3,11 @@
     python -m tcsem_fdtd_amd.measure.profile_run --workload marine_3d --precision 32 --batch 1 --steps 20
 
 Builds the deck, runs the library's march autotuner for the batch size (as bench.py does; its
-trial steps precede the field resets of tcsem_begin, the last k_fill_zero dispatches), advances `warmup` untimed steps, then
-`steps` steps, and prints one JSON line with the chosen tiling and the wall time per step.
+trial steps precede the field resets of tcsem_begin, the last k_fill_zero dispatches), advances
+through the source wavelet (the first 2 (n_src/2) steps, which also run the source pass) and
+`warmup` more steps untimed, then `steps` steps, and prints one JSON line with the chosen tiling
+and the wall time per step.  --pairs rows,pf (or 0) fixes the two-step pass over the library's
+choice.
 --workload uneven runs the uneven-model workload as one model batch (its autotuner, the field
 reset it leaves, then the same warm-up and timed steps).  --every-step-rx replaces the deck's
 sample times by one sample per step from the first step on,
@@ -34,6 +37,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--every-step-rx", action="store_true")
+    ap.add_argument("--pairs", default=None, help="two steps per pass: rows,pf, or 0 for one step at a time")
     ap.add_argument("--tiling", default=None,
                     help="fixed march tiling rows,pf[,zchunks[,pass|fused[,1|0]]] instead of the autotuner "
                          "(the last field: the 8-bit conductivity index on or off)")
@@ -52,6 +56,9 @@ def main():
         return models(T, W, a)
     deck = W.towline(T, ntx=a.batch) if a.workload == "towline" else getattr(W, a.workload)(T)
     with T.Context(T.make_setup(**deck.setup), deck.cond, a.precision) as c:
+        if a.pairs is not None:
+            f = [int(x) for x in a.pairs.split(",")]
+            c.set_pairs(f[0], f[1] if len(f) > 1 else 1)
         tiling = None if a.no_tune else c.autotune(a.batch, 20)
         r = dict(deck.rxtime)
         if a.every_step_rx:
@@ -66,15 +73,19 @@ def main():
         c.set_receivers(T.make_rxtime(r["T0"], r["Tn"], r["Nt"], r["is_log"], r["comps"]), deck.rx)
         c.set_transmitters([T.make_tx(**dict(t, txid=i + 1)) for i, t in enumerate(deck.txs)])
         c.begin(0, a.batch)
-        c.step(a.warmup)
+        lead = 0 if a.every_step_rx else 2 * (deck.setup["n_src"] // 2)
+        c.step(lead + a.warmup)
         c.sync()
+        p0 = c.pair_info()[2]
         t0 = time.perf_counter()
         c.step(a.steps)
         c.sync()
         dt = time.perf_counter() - t0
+        rows, pf, p1 = c.pair_info()
         print(json.dumps({"workload": deck.name, "precision": a.precision, "batch": a.batch,
                           "steps": a.steps, "ms_per_step": 1000 * dt / a.steps,
-                          "cells": c.L * c.M * c.N, "march_tiling": tiling}), flush=True)
+                          "cells": c.L * c.M * c.N, "march_tiling": tiling,
+                          "pairs": [rows, pf, p1 - p0] if rows else None}), flush=True)
 
 
 def models(T, W, a):

@@ -109,7 +109,7 @@ def test_pairs_fp64_equal_the_oracle():
 
 def test_pair_settings_are_checked():
     with T.Context(decks.small_setup(T), _cond(dict()), T.FP32) as c:
-        for rows, pf in ((10, 1), (12, 3), (8, 1)):
+        for rows, pf in ((9, 1), (12, 3), (14, 1), (4, 1)):
             with pytest.raises(T.TcsemError):
                 c.set_pairs(rows, pf)
         c.set_pairs(16, 2)
