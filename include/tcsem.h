@@ -159,7 +159,7 @@ int tcsem_reset_fields(tcsem_ctx *ctx, int first, int n);
  * z-chunks of the lower launch 1/2/3/4/6/8 as the depth allows, for chunked tilings the source of
  * the chunks' Bz partial sums (a pass over Bx, By, or the chunk sums the previous step's march
  * accumulated), and the 8-bit conductivity index on and off -- then, with that tiling for the top
- * levels, pairs of steps in the two-step pass (rows 8/12/16, prefetch 1-2; tcsem_set_pairs), kept
+ * levels, pairs of steps in the two-step pass (rows 8/12/16, FP64 8/12, prefetch 1-2; tcsem_set_pairs), kept
  * when 2% faster per step -- and keeps the choice for later batches of that size
  * (TCSEM_MARCH_ROWS / _PF / _ZCHUNKS / _ZSUM / _PAIRS and tcsem_set_pairs still override it).  It runs on the fields of slots [0, nbatch): a begun run
  * is discarded and the fields are left zeroed.  Outputs (nullable): the chosen rows, prefetch
@@ -239,7 +239,7 @@ long tcsem_graph_replays(const tcsem_ctx *ctx);
 int tcsem_sigma_table(const tcsem_ctx *ctx);
 
 /* Two steps per lower pass (no counterpart in the reference, whose steps each read and write
- * the whole state): with rows 8, 12 or 16 (workgroup rows of the pass) and pf 1 or 2 (prefetch
+ * the whole state): with rows 8, 12 or 16 (FP64: 8 or 12; workgroup rows of the pass) and pf 1 or 2 (prefetch
  * depth), tcsem_step and tcsem_run take each two consecutive steps after the source wavelet
  * whose first holds no receiver sample as one pass over levels N-1 .. 3 of both steps (the
  * intermediate state stays on chip there), then the top levels of each step as usual.  The

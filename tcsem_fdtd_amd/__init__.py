@@ -396,7 +396,7 @@ class Context:
         return self.L_.tcsem_sigma_table(self.h)
 
     def set_pairs(self, rows, pf=1):
-        """Take two steady-state steps per lower pass (rows 8, 12 or 16, prefetch pf 1 or 2; rows 0:
+        """Take two steady-state steps per lower pass (rows 8, 12 or 16 (FP64 8 or 12), prefetch pf 1 or 2; rows 0:
         off), over the library's own choice.  Results equal one step at a time (unchunked), bit for
         bit."""
         _check(self.L_.tcsem_set_pairs(self.h, rows, pf))

@@ -632,7 +632,7 @@ void choose_tiling(tcsem_ctx *c, int nbatch) {
       pby = r;
       ppf = q;
     } else {
-      std::fprintf(stderr, "tcsem: TCSEM_MARCH_PAIRS=%s is not 0 or rows,pf (rows 8, 12, 16; pf 1, 2)\n", e);
+      std::fprintf(stderr, "tcsem: TCSEM_MARCH_PAIRS=%s is not 0 or rows,pf (rows 8, 12, 16 (FP64: 8, 12); pf 1, 2)\n", e);
     }
   }
   if (c->pair_set >= 0) {
@@ -3463,7 +3463,7 @@ int tcsem_pair_timing(const tcsem_ctx *cc, double *ms, long *n) {
 int tcsem_set_pairs(tcsem_ctx *c, int rows, int pf) {
   if (check_ctx(c)) return TCSEM_EINVAL;
   if (rows != 0 && (!pair_rows_supported(c->prec, rows) || pf < 1 || pf > 2))
-    return fail(TCSEM_EINVAL, "two-step pass: rows 0 (off), 8, 12 or 16; prefetch 1 or 2");
+    return fail(TCSEM_EINVAL, "two-step pass: rows 0 (off), 8, 12 or 16 (FP64: 8 or 12); prefetch 1 or 2");
   if (rows && c->g.info.N < 6) return fail(TCSEM_EINVAL, "two-step pass needs at least 6 levels");
   c->pair_set = rows;
   c->pair_set_pf = pf;

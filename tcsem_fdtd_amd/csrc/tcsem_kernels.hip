@@ -602,8 +602,7 @@ struct Lv2 {
   std::conditional_t<S8, unsigned, T> s;
 };
 template <typename T, int BY, int PF, bool S8>
-__global__ void __launch_bounds__(kMarchLanes *BY)
-__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && BY == 10 ? 5 : 1))) k_march2(const PairArgs<T> pa) {
+__global__ void __launch_bounds__(kMarchLanes *BY) k_march2(const PairArgs<T> pa) {
   const MarchArgs<T> &a = pa.a;
   const int b = blockIdx.x;
   const int tile = (b & 7) * pa.nper + (b >> 3);  // XCD-contiguous bands of tiles
@@ -648,6 +647,12 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && BY == 10 ? 5 : 1))) k_march
                                reyi = rsrc_of((void *)(a.eyi + fo), fbytes),
                                rezi = rsrc_of((void *)(a.ezi + fo), fbytes);
   const bool erow = ly >= 1 && ly <= BY - 2, ezrow = ly >= 1;
+  // the work each row's wave does (uniform branches): step A's Bz sum on rows 0 .. BY-2, its Ez on
+  // rows 1 .. BY-1, its Ex, Ey and B on rows 1 .. BY-2; step B's Bz sum on rows 1 .. BY-3, its Ez
+  // on rows 2 .. BY-2, its Ex, Ey and B (the stored cells) on rows 2 .. BY-3 -- what the owned
+  // cells read, one ring further out for step A
+  const bool rBzA = ly <= BY - 2, rA = ly >= 1 && ly <= BY - 2;
+  const bool rBzB = ly >= 1 && ly <= BY - 3, rEzB = ly >= 2 && ly <= BY - 2, rB = ly >= 2 && ly <= BY - 3;
   const auto cfA = make_coef<T>(a.lam, a.dt);
   const auto cfB = make_coef<T>(pa.lamB, pa.dtB);
   const T dtbA = (T)a.dtb, dtbB = (T)pa.dtbB, zero = 0;
@@ -713,11 +718,16 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && BY == 10 ? 5 : 1))) k_march
     const unsigned offA = (unsigned)k * PSb + colb, offB = (unsigned)kBc * PSb + colb;
     // ---- P1
     const T bx_k = cur.bx, by_k = cur.by, bx_m = nxt.bx, by_m = nxt.by;
-    bzA.add(dzA * ((sBx[p][ly][lxp] - bx_k) * rdx + (sBy[p][lyp][lx] - by_k) * rdy));
-    const T bzAk = bzA.val();
+    T bzAk = zero, bzBk = zero;
+    if (rBzA) {
+      bzA.add(dzA * ((sBx[p][ly][lxp] - bx_k) * rdx + (sBy[p][lyp][lx] - by_k) * rdy));
+      bzAk = bzA.val();
+    }
     sBzA[ly][lx] = bzAk;
-    bzB.add(dzB * ((sB1x[p][ly][lxp] - b1xc) * rdx + (sB1y[p][lyp][lx] - b1yc) * rdy));
-    const T bzBk = bzB.val();
+    if (rBzB) {
+      bzB.add(dzB * ((sB1x[p][ly][lxp] - b1xc) * rdx + (sB1y[p][lyp][lx] - b1yc) * rdy));
+      bzBk = bzB.val();
+    }
     sBzB[ly][lx] = bzBk;
     stage(k - 1, nxt);
     const T ezo = cur.ez, exo = cur.ex, eyo = cur.ey;
@@ -726,8 +736,8 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && BY == 10 ? 5 : 1))) k_march
     // ---- P2: intermediate E of level k (step A), new E of level k+2 (step B).  The conductivity
     // averages of a level are the same for both steps: step A forms them, step B takes them two
     // iterations later from registers.
-    T ezA, exA, eyA, azA, axA, ayA;
-    {
+    T ezA = zero, exA = zero, eyA = zero, azA = zero, axA = zero, ayA = zero;
+    if (ezrow) {
       const int sp = k & 1, sq = sp ^ 1;
       azA = (T)0.25 * (sS[sp][lym][lxm] + sS[sp][lym][lx] + sS[sp][ly][lxm] + sS[sp][ly][lx]);
       {
@@ -741,34 +751,38 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && BY == 10 ? 5 : 1))) k_march
         const T v = df_new(ezo, azA, cfA, curl);
         ezA = dEz ? v : zero;
       }
-      {
-        const T curl = (bzAk - sBzA[lym][lx]) * rdy2 - (by_k - by_m) * rdz2A;
-        const T v = df_new(exo, axA, cfA, curl);
-        exA = dEx ? v : zero;
-      }
-      {
-        const T curl = (bx_k - bx_m) * rdz2A - (bzAk - sBzA[ly][lxm]) * rdx2;
-        const T v = df_new(eyo, ayA, cfA, curl);
-        eyA = dEy ? v : zero;
+      if (rA) {
+        {
+          const T curl = (bzAk - sBzA[lym][lx]) * rdy2 - (by_k - by_m) * rdz2A;
+          const T v = df_new(exo, axA, cfA, curl);
+          exA = dEx ? v : zero;
+        }
+        {
+          const T curl = (bx_k - bx_m) * rdz2A - (bzAk - sBzA[ly][lxm]) * rdx2;
+          const T v = df_new(eyo, ayA, cfA, curl);
+          eyA = dEy ? v : zero;
+        }
       }
     }
     sEzA[ly][lx] = ezA;
-    T ezB, exB, eyB;
-    {
+    T ezB = zero, exB = zero, eyB = zero;
+    if (rEzB) {
       {
         const T curl = (b1yc - sB1y[p][ly][lxm]) * rdx2 - (b1xc - sB1x[p][lym][lx]) * rdy2;
         const T v = df_new(e1zc, azc, cfB, curl);
         ezB = bon && dEz ? v : zero;
       }
-      {
-        const T curl = (bzBk - sBzB[lym][lx]) * rdy2 - (b1yc - b1yn) * rdz2B;
-        const T v = df_new(e1xc, axc, cfB, curl);
-        exB = bon && dEx ? v : zero;
-      }
-      {
-        const T curl = (b1xc - b1xn) * rdz2B - (bzBk - sBzB[ly][lxm]) * rdx2;
-        const T v = df_new(e1yc, ayc, cfB, curl);
-        eyB = bon && dEy ? v : zero;
+      if (rB) {
+        {
+          const T curl = (bzBk - sBzB[lym][lx]) * rdy2 - (b1yc - b1yn) * rdz2B;
+          const T v = df_new(e1xc, axc, cfB, curl);
+          exB = bon && dEx ? v : zero;
+        }
+        {
+          const T curl = (b1xc - b1xn) * rdz2B - (bzBk - sBzB[ly][lxm]) * rdx2;
+          const T v = df_new(e1yc, ayc, cfB, curl);
+          eyB = bon && dEy ? v : zero;
+        }
       }
     }
     sEzB[ly][lx] = ezB;
@@ -778,9 +792,13 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && BY == 10 ? 5 : 1))) k_march
     bsto(rey, offB, stB && dEy, eyB);
     __syncthreads();
     // ---- P3: intermediate B of level k (step A, into LDS), new B of level k+2 (step B)
-    const T bxA = bx_k + dtbA * ((e1yn - eyA) * rdzA - (sEzA[lyp][lx] - ezA) * rdy);
-    const T byA = by_k + dtbA * ((sEzA[ly][lxp] - ezA) * rdx - (e1xn - exA) * rdzA);
-    const T b1xA = dBx ? bxA : zero, b1yA = dBy ? byA : zero;
+    T b1xA = zero, b1yA = zero;
+    if (rA) {
+      const T bxA = bx_k + dtbA * ((e1yn - eyA) * rdzA - (sEzA[lyp][lx] - ezA) * rdy);
+      const T byA = by_k + dtbA * ((sEzA[ly][lxp] - ezA) * rdx - (e1xn - exA) * rdzA);
+      b1xA = dBx ? bxA : zero;
+      b1yA = dBy ? byA : zero;
+    }
     sB1x[p][ly][lx] = b1xA;
     sB1y[p][ly][lx] = b1yA;
     const bool stA = own && k == klo;  // level 2 of the intermediate state, for the top launches
@@ -789,10 +807,12 @@ __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && BY == 10 ? 5 : 1))) k_march
     bsto(mez, offA, stA && dEz, ezA);
     bsto(mbx, offA, stA && dBx, b1xA);
     bsto(mby, offA, stA && dBy, b1yA);
-    const T bxB = b1xc + dtbB * ((e2y - eyB) * rdzB - (sEzB[lyp][lx] - ezB) * rdy);
-    const T byB = b1yc + dtbB * ((sEzB[ly][lxp] - ezB) * rdx - (e2x - exB) * rdzB);
-    bsto(rbx, offB, stB && dBx, bxB);
-    bsto(rby, offB, stB && dBy, byB);
+    if (rB) {
+      const T bxB = b1xc + dtbB * ((e2y - eyB) * rdzB - (sEzB[lyp][lx] - ezB) * rdy);
+      const T byB = b1yc + dtbB * ((sEzB[ly][lxp] - ezB) * rdx - (e2x - exB) * rdzB);
+      bsto(rbx, offB, stB && dBx, bxB);
+      bsto(rby, offB, stB && dBy, byB);
+    }
     e2x = exB;
     e2y = eyB;
     b1xc = b1xn;
@@ -1192,9 +1212,9 @@ hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEven
   }
   return hipErrorInvalidValue;
 }
+// 8, 12 and 16 rows; FP64 not 16 (its registers would spill)
 int pair_rows_supported(int prec, int rows) {
-  (void)prec;
-  return rows == 8 || rows == 10 || rows == 12 || rows == 16 ? rows : 0;
+  return rows == 8 || rows == 12 || (rows == 16 && prec == 32) ? rows : 0;
 }
 template <typename T, int BY, int PF, bool S8>
 hipError_t launch_march2_v(const PairArgs<T> &p, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
@@ -1228,9 +1248,10 @@ hipError_t launch_march2(const PairArgs<T> &p, int nslot, hipStream_t s, hipEven
   if (p.a.N < 6 || p.ntx < 1 || p.nty < 1) return hipErrorInvalidValue;
   switch (p.rows) {
     case 8: return launch_march2_by<T, 8>(p, nslot, s, e0, e1);
-    case 10: return launch_march2_by<T, 10>(p, nslot, s, e0, e1);
     case 12: return launch_march2_by<T, 12>(p, nslot, s, e0, e1);
-    case 16: return launch_march2_by<T, 16>(p, nslot, s, e0, e1);
+    case 16:
+      if constexpr (sizeof(T) == 4) return launch_march2_by<T, 16>(p, nslot, s, e0, e1);
+      break;
   }
   return hipErrorInvalidValue;
 }

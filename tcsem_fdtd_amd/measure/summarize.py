@@ -19,7 +19,11 @@ DIR is a rocprofv3 output folder (`-d DIR -o run --output-format csv`).  Dispatc
 last k_fill_zero (the field reset of tcsem_begin, after the autotuner's trial steps) are left out.
 Steps are counted as k_surface dispatches (one per step: the warm-up steps after the reset and
 the timed ones).  The march per step is the sum of its lower and top launches; its algorithmic bytes
-are the canonical L*M*(N-1) cell-updates x 13 words x element size.  FETCH_SIZE and WRITE_SIZE (KiB per
+are the canonical L*M*(N-1) cell-updates x 13 words x element size.  When the run takes two steps
+per pass (k_march2), the pass is also reported per launch: its algorithmic bytes are L*M*(N-1)
+cells x --pass-words (10 + 1/B: sigma and the old state read, the new state written, once for two
+cell-updates) x element size, and with --key its calibrated HBM bytes per launch go to the
+traffic file (the counters' k_march total per step is still printed).  FETCH_SIZE and WRITE_SIZE (KiB per
 dispatch) are scaled by what they read for a copy kernel of known size (stream_probe copy).
 """
 import argparse
@@ -108,6 +112,14 @@ def march(a):
         out.append("- algorithmic bytes per step: %d cell-updates x %g words x %d B = %.1f MB -> %.0f GB/s "
                    "(%.3f of 8000 GB/s)" % (a.cells, a.words, a.esz, algo / 1e6, algo / span / 1e3,
                                              algo / span / 1e3 / 8000.0))
+        pn = sum(n for k, n, avg, tot in tab if "k_march2" in k)
+        if pn:
+            pavg = sum(tot for k, n, avg, tot in tab if "k_march2" in k) / pn
+            palgo = a.cells * a.pass_words * a.esz
+            out.append("- k_march2 (two steps per pass): %d launches, average %.2f us; algorithmic bytes per "
+                       "launch: %d cells x %g words x %d B = %.1f MB -> %.0f GB/s (%.3f of 8000 GB/s)"
+                       % (pn, pavg, a.cells, a.pass_words, a.esz, palgo / 1e6, palgo / pavg / 1e3,
+                          palgo / pavg / 1e3 / 8000.0))
     else:
         out.append("- algorithmic bytes per step: %d cell-updates x %g words x %d B = %.1f MB"
                    % (a.cells, a.words, a.esz, algo / 1e6))
@@ -129,6 +141,15 @@ def march(a):
         out.append("- HBM traffic per step of k_march (`%s`, %d steps, tiling %s): FETCH %.0f KiB, WRITE %.0f KiB "
                    "raw; calibrated %.1f MB = %.3f x algorithmic" % (a.pmc_cmd, pmc_steps, a.tiling, f, w,
                                                                    traffic / 1e6, traffic / algo))
+        pf_, npf = counter_sum(a.fetch, "FETCH_SIZE", "k_march2")
+        pw_, npw = counter_sum(a.write, "WRITE_SIZE", "k_march2")
+        if npf and npw:
+            palgo = a.cells * a.pass_words * a.esz
+            traffic = (pf_ / npf * ff + pw_ / npw * fw) * 1024.0
+            out.append("- HBM traffic per launch of k_march2 (%d launches): FETCH %.0f KiB, WRITE %.0f KiB raw; "
+                       "calibrated %.1f MB = %.3f x its algorithmic %.1f MB" % (npf, pf_ / npf, pw_ / npw,
+                                                                           traffic / 1e6, traffic / palgo,
+                                                                           palgo / 1e6))
         if a.traffic and a.key:
             try:
                 with open(a.traffic) as fh:
@@ -246,6 +267,8 @@ def main():
     m.add_argument("--words", type=float, default=WORDS,
                    help="algorithmic words per cell-update (13; 12 + 1/B for a B-transmitter batch)")
     m.add_argument("--cells", type=int, required=True, help="cell-updates per step: L*M*(N-1) x batch")
+    m.add_argument("--pass-words", type=float, default=11.0,
+                   help="algorithmic words per cell of a two-step pass launch (11; 10 + 1/B for a batch)")
     m.add_argument("--esz", type=int, required=True)
     m.add_argument("--fetch")
     m.add_argument("--write")

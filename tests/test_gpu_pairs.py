@@ -63,7 +63,7 @@ def _same(a, b):
 
 @pytest.mark.parametrize("precision,rows,pf,sigma8", [
     (T.FP32, 12, 1, True), (T.FP32, 16, 2, True), (T.FP32, 12, 2, False), (T.FP32, 16, 1, False),
-    (T.FP64, 12, 1, True), (T.FP64, 16, 2, False)])
+    (T.FP64, 12, 1, True), (T.FP64, 8, 2, False)])
 def test_pairs_equal_single_steps(monkeypatch, precision, rows, pf, sigma8):
     monkeypatch.setenv("TCSEM_SIGMA8", "1" if sigma8 else "0")
     ref = _run(precision, None)
@@ -84,7 +84,7 @@ def test_pairs_with_several_transmitters():
 def test_pairs_on_other_grids(kw):
     """One tile, ragged tiles, and a grid of few levels (N small) and rows."""
     ref = _run(T.FP64, None, kw=kw, steps=300)
-    got = _run(T.FP64, (16, 1), kw=kw, steps=300)
+    got = _run(T.FP64, (12, 2), kw=kw, steps=300)
     assert got[3] > 50
     _same(got, ref)
 
@@ -116,3 +116,8 @@ def test_pair_settings_are_checked():
         assert c.pair_info() == (16, 2, 0)
         c.set_pairs(0, 1)
         assert c.pair_info()[0] == 0
+    with T.Context(decks.small_setup(T), _cond(dict()), T.FP64) as c:
+        with pytest.raises(T.TcsemError):
+            c.set_pairs(16, 1)  # FP64 passes of 16 rows would spill their registers
+        c.set_pairs(8, 2)
+        assert c.pair_info()[:2] == (8, 2)
