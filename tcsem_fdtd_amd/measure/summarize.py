@@ -210,15 +210,26 @@ def bench(a):
         for k, (n, t) in sorted(acc.items(), key=lambda x: -x[1][1]):
             out.append("| `%s` | %d | %.2f | %.1f |" % (_short(k), n, t / n / 1e3, t / 1e3))
         march = sum(t for k, (n, t) in acc.items() if "k_march" in k or "k_zsum" in k) / max(steps, 1) / 1e3
-        out += ["", "- steps in the range: %d; k_march per step (lower + top launch, and any k_zsum pass): %.2f us"
-                % (steps, march)]
+        out += ["", "- steps in the range: %d; k_march per step (lower + top launch, and any k_zsum pass; "
+                "with two steps per pass, half the pass and both top launches): %.2f us" % (steps, march)]
+        pn = sum(n for k, (n, t) in acc.items() if "k_march2" in k)
+        pavg = sum(t for k, (n, t) in acc.items() if "k_march2" in k) / max(pn, 1) / 1e3
+        if pn:
+            out.append("- k_march2 (two steps per pass): %d launches, average %.2f us" % (pn, pavg))
         path = comp.get(lab[len("timed "):])
         if path and line:
             v = line
             for key in path.split("."):
                 v = v[key]
+            ref, what = (pavg, "the k_march2 average") if pn else (march, "the rocprof figure")
             out.append("- the bench line's own figure (`%s`, HIP events recorded by the launches' dispatch "
-                       "packets): %.2f us, %.3f x the rocprof figure" % (path, 1e3 * v, 1e3 * v / march))
+                       "packets): %.2f us, %.3f x %s" % (path, 1e3 * v, 1e3 * v / ref, what))
+            if pn:
+                v2 = line
+                for key in path.rsplit(".", 1)[0].split(".") + ["march_ms_per_step"]:
+                    v2 = v2[key]
+                out.append("- the bench line's march per step (`march_ms_per_step` beside it): %.2f us, %.3f x "
+                           "the rocprof k_march per step" % (1e3 * v2, 1e3 * v2 / march))
         out.append("")
     print("\n".join(out))
 
