@@ -596,6 +596,29 @@ k_march(const MarchArgs<T> la, const MarchArgs<T> *geo) {
 // Step B idles on the first two iterations (levels N+1, N: dz 0, nothing stored, inputs zero)
 // and step A on the last (k = 1: nothing kept), so every iteration runs the same code.  The
 // lower pass never reads level 0 of the old state (the previous pair's surface kernel writes it).
+// The same register of the lane to the left (l-1) or right (l+1) in the wavefront, by a DPP
+// whole-wave shift (wave_shr:1, wave_shl:1); the lanes past either end read 0.  The pass's rows
+// are single wavefronts, so its x-neighbours need no LDS round trip (ds reads 21 -> 11 per level).
+constexpr int kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138;
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, true);
+  const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, true);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <typename T>
+__device__ __forceinline__ T from_left(T v) {
+  return dpp_move<kDppWaveShr1>(v);
+}
+template <typename T>
+__device__ __forceinline__ T from_right(T v) {
+  return dpp_move<kDppWaveShl1>(v);
+}
 template <typename T, bool S8>
 struct Lv2 {
   T bx, by, ex, ey, ez;
@@ -628,6 +651,10 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march2(const PairArgs<T> pa
   const int lxm = max(lx - 1, 0), lxp = min(lx + 1, W - 1), lym = max(ly - 1, 0), lyp = min(ly + 1, BY - 1);
   const T rdx = a.rdx[ic + 1], rdx2 = a.rdx2[ic + 1], rdy = a.rdy[jc + 1], rdy2 = a.rdy2[jc + 1];
   const int slot = (int)blockIdx.y;
+  // x-neighbours by lane shifts (FP32), or from LDS (FP64: an x-neighbour from a lane shift lets
+  // the compiler fuse the other product of a curl into its FMA, which would round differently
+  // from the one-step march; the conductivity sums, which have no products, shift in both)
+  constexpr bool kX = sizeof(T) == 4;
   const long long fo = (long long)slot * a.fstride - (a.PX + a.xm);  // allocation base of the slot
   const long long po = (long long)slot * a.pstride - (a.PX + a.xm);
   const T *__restrict__ bxi = a.bxi + fo;
@@ -683,19 +710,22 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march2(const PairArgs<T> pa
   LvT R[S];
 #pragma unroll
   for (int d = 0; d < S - 1; ++d) R[d] = load(khi - d);
-  auto stage = [&](const int k, const LvT &v) {  // B_old and sigma of level k into LDS
+  auto stage = [&](const int k, const LvT &v) -> T {  // B_old and sigma of level k into LDS
     sBx[k & 1][ly][lx] = v.bx;
     sBy[k & 1][ly][lx] = v.by;
+    T sv;
     if constexpr (S8)
-      sS[k & 1][ly][lx] = sLut[v.s];
+      sv = sLut[v.s];
     else
-      sS[k & 1][ly][lx] = v.s;
+      sv = v.s;
+    sS[k & 1][ly][lx] = sv;
+    return sv;  // the lane's own conductivity of level k
   };
   // step B's inputs of its idle iterations read as zero (each lane clears its own cells)
   sB1x[0][ly][lx] = sB1x[1][ly][lx] = zero;
   sB1y[0][ly][lx] = sB1y[1][ly][lx] = zero;
   __syncthreads();  // the table sLut
-  stage(khi, R[0]);
+  T sgk = stage(khi, R[0]);  // the lane's conductivity of level k
   Acc<T> bzA, bzB;  // running Bz sums of the old and of the intermediate B
   bzA.init();
   bzB.init();
@@ -720,16 +750,16 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march2(const PairArgs<T> pa
     const T bx_k = cur.bx, by_k = cur.by, bx_m = nxt.bx, by_m = nxt.by;
     T bzAk = zero, bzBk = zero;
     if (rBzA) {
-      bzA.add(dzA * ((sBx[p][ly][lxp] - bx_k) * rdx + (sBy[p][lyp][lx] - by_k) * rdy));
+      bzA.add(dzA * (((kX ? from_right(bx_k) : sBx[p][ly][lxp]) - bx_k) * rdx + (sBy[p][lyp][lx] - by_k) * rdy));
       bzAk = bzA.val();
     }
     sBzA[ly][lx] = bzAk;
     if (rBzB) {
-      bzB.add(dzB * ((sB1x[p][ly][lxp] - b1xc) * rdx + (sB1y[p][lyp][lx] - b1yc) * rdy));
+      bzB.add(dzB * (((kX ? from_right(b1xc) : sB1x[p][ly][lxp]) - b1xc) * rdx + (sB1y[p][lyp][lx] - b1yc) * rdy));
       bzBk = bzB.val();
     }
     sBzB[ly][lx] = bzBk;
-    stage(k - 1, nxt);
+    const T sgm = stage(k - 1, nxt);  // the lane's conductivity of level k-1
     const T ezo = cur.ez, exo = cur.ex, eyo = cur.ey;
     dst = load(k - 1 - PF);
     __syncthreads();
@@ -739,15 +769,16 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march2(const PairArgs<T> pa
     T ezA = zero, exA = zero, eyA = zero, azA = zero, axA = zero, ayA = zero;
     if (ezrow) {
       const int sp = k & 1, sq = sp ^ 1;
-      azA = (T)0.25 * (sS[sp][lym][lxm] + sS[sp][lym][lx] + sS[sp][ly][lxm] + sS[sp][ly][lx]);
+      const T sgkL = from_left(sgk), sgmL = from_left(sgm);
+      azA = (T)0.25 * (sS[sp][lym][lxm] + sS[sp][lym][lx] + sgkL + sgk);
       {
-        const T sxk = (T)0.5 * (sS[sp][lym][lx] + sS[sp][ly][lx]);
-        const T sxm = (T)0.5 * (sS[sq][lym][lx] + sS[sq][ly][lx]);
+        const T sxk = (T)0.5 * (sS[sp][lym][lx] + sgk);
+        const T sxm = (T)0.5 * (sS[sq][lym][lx] + sgm);
         axA = (T)0.5 * (sxk + sxm);
       }
-      ayA = (T)0.25 * (sS[sp][ly][lxm] + sS[sp][ly][lx] + sS[sq][ly][lxm] + sS[sq][ly][lx]);
+      ayA = (T)0.25 * (sgkL + sgk + sgmL + sgm);
       {
-        const T curl = (by_k - sBy[p][ly][lxm]) * rdx2 - (bx_k - sBx[p][lym][lx]) * rdy2;
+        const T curl = (by_k - (kX ? from_left(by_k) : sBy[p][ly][lxm])) * rdx2 - (bx_k - sBx[p][lym][lx]) * rdy2;
         const T v = df_new(ezo, azA, cfA, curl);
         ezA = dEz ? v : zero;
       }
@@ -758,7 +789,7 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march2(const PairArgs<T> pa
           exA = dEx ? v : zero;
         }
         {
-          const T curl = (bx_k - bx_m) * rdz2A - (bzAk - sBzA[ly][lxm]) * rdx2;
+          const T curl = (bx_k - bx_m) * rdz2A - (bzAk - (kX ? from_left(bzAk) : sBzA[ly][lxm])) * rdx2;
           const T v = df_new(eyo, ayA, cfA, curl);
           eyA = dEy ? v : zero;
         }
@@ -768,7 +799,7 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march2(const PairArgs<T> pa
     T ezB = zero, exB = zero, eyB = zero;
     if (rEzB) {
       {
-        const T curl = (b1yc - sB1y[p][ly][lxm]) * rdx2 - (b1xc - sB1x[p][lym][lx]) * rdy2;
+        const T curl = (b1yc - (kX ? from_left(b1yc) : sB1y[p][ly][lxm])) * rdx2 - (b1xc - sB1x[p][lym][lx]) * rdy2;
         const T v = df_new(e1zc, azc, cfB, curl);
         ezB = bon && dEz ? v : zero;
       }
@@ -779,7 +810,7 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march2(const PairArgs<T> pa
           exB = bon && dEx ? v : zero;
         }
         {
-          const T curl = (b1xc - b1xn) * rdz2B - (bzBk - sBzB[ly][lxm]) * rdx2;
+          const T curl = (b1xc - b1xn) * rdz2B - (bzBk - (kX ? from_left(bzBk) : sBzB[ly][lxm])) * rdx2;
           const T v = df_new(e1yc, ayc, cfB, curl);
           eyB = bon && dEy ? v : zero;
         }
@@ -795,7 +826,7 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march2(const PairArgs<T> pa
     T b1xA = zero, b1yA = zero;
     if (rA) {
       const T bxA = bx_k + dtbA * ((e1yn - eyA) * rdzA - (sEzA[lyp][lx] - ezA) * rdy);
-      const T byA = by_k + dtbA * ((sEzA[ly][lxp] - ezA) * rdx - (e1xn - exA) * rdzA);
+      const T byA = by_k + dtbA * (((kX ? from_right(ezA) : sEzA[ly][lxp]) - ezA) * rdx - (e1xn - exA) * rdzA);
       b1xA = dBx ? bxA : zero;
       b1yA = dBy ? byA : zero;
     }
@@ -809,7 +840,7 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march2(const PairArgs<T> pa
     bsto(mby, offA, stA && dBy, b1yA);
     if (rB) {
       const T bxB = b1xc + dtbB * ((e2y - eyB) * rdzB - (sEzB[lyp][lx] - ezB) * rdy);
-      const T byB = b1yc + dtbB * ((sEzB[ly][lxp] - ezB) * rdx - (e2x - exB) * rdzB);
+      const T byB = b1yc + dtbB * (((kX ? from_right(ezB) : sEzB[ly][lxp]) - ezB) * rdx - (e2x - exB) * rdzB);
       bsto(rbx, offB, stB && dBx, bxB);
       bsto(rby, offB, stB && dBy, byB);
     }
@@ -831,6 +862,7 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march2(const PairArgs<T> pa
     azn = azA;
     axn = axA;
     ayn = ayA;
+    sgk = sgm;
     // no barrier here: the next iteration's first phase writes only the Bz sums and the staging
     // buffers of level k-2, which no wave reads after this iteration's second barrier, and reads
     // the intermediate B of level k+1, which this phase does not write
