@@ -177,6 +177,9 @@ def models_run(T, decks, precision, steps, warmup, tune=True, single_steps=0):
                 b.sync()
                 sec = time.perf_counter() - t0
             out["timing"] = b.timing()
+            pass_ms, npass = b.pair_timing()
+            rows, _, pairs = b.pair_info()
+            out["pairs"] = {"rows": rows, "pass_ms": pass_ms, "passes_timed": npass} if rows and npass else None
             b.enable_timing(False)
             out["ex"] = np.concatenate([c.get(T.EX, 0).ravel() for c in ctxs])
         out["gcell_s"] = cells * steps / sec / 1e9
@@ -190,13 +193,28 @@ def models_run(T, decks, precision, steps, warmup, tune=True, single_steps=0):
 
 
 def models_roofline(res, esz, traffic_entry=(None, None)):
+    step_ms = res["timing"]["march_ms"]
+    pr = res.get("pairs")
+    if pr:
+        # the model batch's two-step pass: 11 words per cell of every member per launch (each
+        # member reads its own sigma), for two cell-updates
+        algo = res["march_cells"] * 11 * esz
+        ms = pr["pass_ms"]
+        gbs = algo / (ms * 1e-3) / 1e9
+        equiv = res["march_cells"] * WORDS_PER_CELL * esz / (step_ms * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": traffic_entry[0],
+                "traffic_tiling": traffic_entry[1],
+                "kernel": "k_march2 (model batch, two steps per pass, rows %d)" % pr["rows"],
+                "algorithmic_bytes_per_launch": int(algo), "cell_updates_per_launch": 2 * res["march_cells"],
+                "avg_launch_ms": round(ms, 5), "march_ms_per_step": round(step_ms, 5),
+                "one_step_equivalent": {"GBs": round(equiv, 1), "frac": round(equiv / PEAK_HBM_GBS, 4)}}
     algo = res["march_cells"] * WORDS_PER_CELL * esz
-    ms = res["timing"]["march_ms"]
-    gbs = algo / (ms * 1e-3) / 1e9
+    gbs = algo / (step_ms * 1e-3) / 1e9
     return {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": round(gbs / PEAK_HBM_GBS, 4), "traffic": traffic_entry[0],
             "traffic_tiling": traffic_entry[1], "kernel": "k_march (model batch)",
-            "algorithmic_bytes_per_launch": int(algo), "avg_launch_ms": round(ms, 5)}
+            "algorithmic_bytes_per_launch": int(algo), "avg_launch_ms": round(step_ms, 5)}
 
 
 def roofline(res, esz, nbatch, traffic_entry):
@@ -556,7 +574,7 @@ def main():
     # 64 conductivity models of uneven grids (96^3 .. 224x224x160), one x-dipole each, in one
     # model batch: FP32 and FP64, FP32 also stepped model by model for comparison
     models = W.uneven_models(T)
-    nm, wm = max(args.steps // 5, 1), args.warmup
+    nm, wm = max(args.steps // 2, 8), args.warmup  # at least one timed pass or step
     m32 = models_run(T, models, T.FP32, nm, wm, tune, single_steps=max(nm // 2, 1))
     m64 = models_run(T, models, T.FP64, nm, wm, tune)
     models_err = float(np.linalg.norm(m32["ex"] - m64["ex"]) / max(np.linalg.norm(m64["ex"]), 1e-300))
@@ -610,7 +628,7 @@ def main():
         "uneven_models": {"workload": "64 models, grids 96^3 .. 224x224x160 (%d cells), 1 x-dipole and 41 receivers each"
                                       % m32["cells"], "steps": nm, "warmup": wm,
                           "fp32": {"value": round(m32["gcell_s"], 3), "ms_per_step": round(m32["ms_per_step"], 5),
-                                   "roofline": models_roofline(m32, 4, load_traffic("uneven64_fp32")),
+                                   "roofline": models_roofline(m32, 4, load_traffic(traffic_key("uneven64_fp32", m32))),
                                    "march_tiling": m32["tiling"], "kernel_ms": kern(m32),
                                    "unbatched": {k: round(v, 5) if isinstance(v, float) else v
                                                  for k, v in m32["unbatched"].items()}},

@@ -264,7 +264,8 @@ int tcsem_pair_timing(const tcsem_ctx *ctx, double *ms, long *n);
  * and must outlive the batch; their traces are read with tcsem_get_traces(ctx, 0, ...) as
  * usual.  A member whose receivers have all been sampled is left out of later calls.
  *   tcsem_batch_autotune: times nsteps batch steps for every rows x prefetch tiling (before the
- *     first step; the members' fields are zeroed again) and keeps the fastest.
+ *     first step; the members' fields are zeroed again) and keeps the fastest, then pairs of
+ *     steps in the two-step pass (rows 8/12/16, FP64 8/12), kept when 2% faster per step.
  *   tcsem_batch_step: nsteps steps of every member still sampling (per member as tcsem_step).
  *   tcsem_batch_run: steps until every member has taken all samples (max_steps > 0 caps the
  *     member step count; members finish at the end of a 256-step chunk). */
@@ -275,9 +276,21 @@ int tcsem_batch_step(tcsem_batch *batch, int nsteps);
 int tcsem_batch_run(tcsem_batch *batch, int max_steps, int *steps_done);
 int tcsem_batch_sync(tcsem_batch *batch);
 /* As tcsem_timing for the batch's launches: t[0] volume march per step (lower and top launch
- * over all members), t[1] surface level, t[2] air-plane chain; n[i] counts. */
+ * over all members; for a pair of steps half of the two-step pass and both top launches), t[1]
+ * surface level, t[2] air-plane chain; n[i] counts. */
 int tcsem_batch_enable_timing(tcsem_batch *batch, int on);
 int tcsem_batch_timing(tcsem_batch *batch, double t[3], long n[3]);
+/* Two steps per pass for the batch (no counterpart in the reference), as tcsem_set_pairs for a
+ * context: rows 8, 12 or 16 (FP64: 8 or 12) at prefetch depth 1, or 0 (one step at a time), over
+ * the library's choice (TCSEM_MARCH_PAIRS, the batch autotuner's pick, else 12-row passes when
+ * the members' pass tiles give at least two workgroups per CU).  Pairs run when every member
+ * still sampling is past its source wavelet and takes no sample after the pair's first step.
+ * The results equal one step at a time, bit for bit (tested).  tcsem_batch_pair_info: the rows
+ * of the last step call (0: one step at a time), the autotuner's pick (-1: not tuned) and the
+ * pairs taken; tcsem_batch_pair_timing: the pass's average device time while timing is enabled. */
+int tcsem_batch_set_pairs(tcsem_batch *batch, int rows);
+int tcsem_batch_pair_info(const tcsem_batch *batch, int *rows, int *tuned, long *pairs);
+int tcsem_batch_pair_timing(tcsem_batch *batch, double *ms, long *n);
 void tcsem_batch_destroy(tcsem_batch *batch);
 
 #ifdef __cplusplus

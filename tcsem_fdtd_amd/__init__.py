@@ -127,6 +127,9 @@ def lib():
         "tcsem_batch_destroy": (None, [P]),
         "tcsem_batch_enable_timing": (C.c_int, [P, C.c_int]),
         "tcsem_batch_timing": (C.c_int, [P, dp, C.POINTER(C.c_long)]),
+        "tcsem_batch_set_pairs": (C.c_int, [P, C.c_int]),
+        "tcsem_batch_pair_info": (C.c_int, [P, ip, ip, C.POINTER(C.c_long)]),
+        "tcsem_batch_pair_timing": (C.c_int, [P, dp, C.POINTER(C.c_long)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -447,7 +450,24 @@ class Batch:
     def autotune(self, nsteps=10):
         r, p, ms = C.c_int(), C.c_int(), C.c_double()
         _check(self.L_.tcsem_batch_autotune(self.h, nsteps, C.byref(r), C.byref(p), C.byref(ms)))
-        return {"rows": r.value, "pf": p.value, "step_ms": ms.value}
+        return {"rows": r.value, "pf": p.value, "pairs": self.pair_info()[1] or None, "step_ms": ms.value}
+
+    def set_pairs(self, rows):
+        """Two steps per pass for the batch (rows 8, 12 or 16, FP64 8 or 12; 0: one at a time)."""
+        _check(self.L_.tcsem_batch_set_pairs(self.h, rows))
+
+    def pair_info(self):
+        """(rows of the last step call's passes, 0 for none; the autotuner's pick, -1 untuned;
+        pairs taken so far)."""
+        r, t, n = C.c_int(), C.c_int(), C.c_long()
+        _check(self.L_.tcsem_batch_pair_info(self.h, C.byref(r), C.byref(t), C.byref(n)))
+        return r.value, t.value, n.value
+
+    def pair_timing(self):
+        """(average device time [ms] of the two-step pass alone, passes timed)."""
+        ms, n = C.c_double(), C.c_long()
+        _check(self.L_.tcsem_batch_pair_timing(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
 
     def step(self, n=1):
         _check(self.L_.tcsem_batch_step(self.h, n))

@@ -1920,9 +1920,19 @@ struct tcsem_batch {
   bool timing = false;
   std::vector<hipEvent_t> evpool;
   size_t evused = 0;
-  std::vector<tcsem_ctx::Span> spans;  // kinds 0 march, 1 surface, 2 air-plane chain
-  double tacc[3] = {0, 0, 0};
-  long tn[3] = {0, 0, 0};
+  std::vector<tcsem_ctx::Span> spans;  // kinds 0 march, 1 surface, 2 air-plane chain, 3 two-step pass
+  double tacc[4] = {0, 0, 0, 0};
+  long tn[4] = {0, 0, 0, 0};
+  // Two steps per pass (prefetch 1): tcsem_batch_set_pairs' rows (-1: none), the autotuner's (-1:
+  // not tuned), the rows of the current records (0: off); per-member pass records (2 parities)
+  // and the top launches' records (step A's then step B's, 2 parities each); the pass work list.
+  int pair_set = -1, pair_tuned = -1, pair_by = 0;
+  char *d_pgeo = nullptr;  // 2 x nmodel PairArgs
+  char *d_tgeo = nullptr;  // 4 x nmodel MarchArgs
+  int2 *d_pwork = nullptr;
+  size_t pwork_cap = 0;
+  int pwork_len = 0;
+  long pairs_done = 0;
 };
 
 namespace {
@@ -1955,8 +1965,63 @@ void batch_tiling(const tcsem_batch *b, int &by, int &pf) {
 // Builds and uploads the member records for tiling (by, pf) and returns the launch records: the
 // largest sizes over the members (grid and LDS extents), the common tiling and level range.
 // The batch's streams must be idle (the records are overwritten).
+// A work list of (member, tile) per workgroup for tiles[q] tiles of each active member: workgroup
+// w runs on XCD w mod 8, so list x of the 8 per-XCD lists fills workgroups x, x + 8, ...  Each
+// active member's tiles go to one XCD where possible, which keeps the halo rows shared by its
+// neighbouring tiles in that XCD's L2; a member with more tiles than half an XCD's share is cut
+// into row-major bands of that size.  Pieces go, largest first, to the XCD with the least work
+// (tiles x levels).  Returns the list's length (a multiple of 8), uploaded to *d (grown as needed).
+int batch_work_list(tcsem_batch *b, const std::vector<int> &tiles, int2 **d, size_t *cap, int *len) {
+  const int nm = (int)b->m.size();
+  long long total = 0;
+  for (int q = 0; q < nm; ++q)
+    if (b->active[q]) total += tiles[q];
+  const long long chunk = std::max(1LL, (total + 15) / 16);
+  struct Piece {
+    int member, t0, n;
+    long long work;
+  };
+  std::vector<Piece> pieces;
+  for (int q = 0; q < nm; ++q) {
+    if (!b->active[q]) continue;
+    const int nt = tiles[q];
+    for (int t0 = 0; t0 < nt; t0 += (int)chunk) {
+      const int n = (int)std::min<long long>(chunk, nt - t0);
+      pieces.push_back({q, t0, n, (long long)n * b->m[q]->g.info.N});
+    }
+  }
+  std::stable_sort(pieces.begin(), pieces.end(), [](const Piece &x, const Piece &y) { return x.work > y.work; });
+  std::vector<std::vector<int2>> lists(8);
+  long long load[8] = {0};
+  for (const Piece &pc : pieces) {
+    const int x = (int)(std::min_element(load, load + 8) - load);
+    for (int t = pc.t0; t < pc.t0 + pc.n; ++t) lists[x].push_back(make_int2(pc.member, t));
+    load[x] += pc.work;
+  }
+  size_t n = 1;
+  for (const auto &l : lists) n = std::max(n, l.size());
+  std::vector<int2> tbl(8 * n, make_int2(-1, -1));
+  for (int x = 0; x < 8; ++x)
+    for (size_t i = 0; i < lists[x].size(); ++i) tbl[8 * i + x] = lists[x][i];
+  if (tbl.size() > *cap) {
+    (void)hipFree(*d);
+    *d = nullptr;
+    *cap = 0;
+    if (hipMalloc((void **)d, tbl.size() * sizeof(int2)) != hipSuccess) return fail(TCSEM_ENOMEM, "work list allocation");
+    *cap = tbl.size();
+  }
+  HIPCHK(hipMemcpy(*d, tbl.data(), tbl.size() * sizeof(int2), hipMemcpyHostToDevice));
+  *len = (int)tbl.size();
+  return TCSEM_OK;
+}
+
+int pass_tiles_x(int L) { return (L + kMarchLanes - 5) / (kMarchLanes - 4); }
+int pass_tiles_y(int M, int by) { return (M + by - 5) / (by - 4); }
+
+// pby > 0: also the records of two steps per pass with pby-row tiles (each member's intermediate
+// state allocated on first use).
 template <typename T>
-int batch_records(tcsem_batch *b, int by, int pf, MarchArgs<T> &la, UpcontArgs<T> &ul) {
+int batch_records(tcsem_batch *b, int by, int pf, MarchArgs<T> &la, UpcontArgs<T> &ul, int pby = 0) {
   const int nm = (int)b->m.size();
   std::vector<MarchArgs<T>> rec(2 * (size_t)nm);
   std::vector<UpcontArgs<T>> urec(nm);
@@ -2005,58 +2070,71 @@ int batch_records(tcsem_batch *b, int by, int pf, MarchArgs<T> &la, UpcontArgs<T
   la.sig8 = nullptr;
   if (b->s8 && std::all_of(b->m.begin(), b->m.end(), [](const tcsem_ctx *c) { return c->d_sig8 != nullptr; }))
     la.sig8 = b->m[0]->d_sig8;
+  for (int q = 0; q < nm; ++q) {  // every member record as the launch reads it (a member's own choice aside)
+    const tcsem_ctx *c = b->m[q];
+    for (int p = 0; p < 2; ++p) rec[(size_t)p * nm + q].sig8 = la.sig8 ? c->d_sig8 + c->PX + c->xm : nullptr;
+  }
   la.k_hi = la.N - 1;
   la.k_lo = 1;
   ul.tiled = 1;
-  // March work list.  Workgroup w runs on XCD w mod 8, so list x of the 8 per-XCD lists fills
-  // workgroups x, x + 8, ...  Each active member's tiles go to one XCD where possible, which keeps
-  // the halo rows shared by its neighbouring tiles in that XCD's L2; a member with more tiles than
-  // half an XCD's share is cut into row-major bands of that size.  Pieces go, largest first, to
-  // the XCD with the least work (tiles x levels).
+  // the march's work list
   {
-    long long total = 0;
-    for (int q = 0; q < nm; ++q)
-      if (b->active[q]) total += (long long)b->m[q]->ntx * b->m[q]->nty;
-    const long long cap = std::max(1LL, (total + 15) / 16);
-    struct Piece {
-      int member, t0, n;
-      long long work;
-    };
-    std::vector<Piece> pieces;
+    std::vector<int> tiles(nm);
+    for (int q = 0; q < nm; ++q) tiles[q] = b->m[q]->ntx * b->m[q]->nty;
+    if (int rc = batch_work_list(b, tiles, &b->d_work, &b->work_cap, &b->work_len)) return rc;
+    la.work = b->d_work;
+    la.nper = b->work_len / 8;  // the march grid: 8 x len workgroups, one row
+  }
+  b->pair_by = 0;
+  if (pby > 0) {  // two steps per pass: the pass and top-launch records, the pass work list
+    std::vector<PairArgs<T>> prec(2 * (size_t)nm);
+    std::vector<MarchArgs<T>> trec(4 * (size_t)nm);
+    std::vector<int> tiles(nm);
     for (int q = 0; q < nm; ++q) {
-      if (!b->active[q]) continue;
-      const tcsem_ctx *c = b->m[q];
-      const int nt = c->ntx * c->nty;
-      for (int t0 = 0; t0 < nt; t0 += (int)cap) {
-        const int n = (int)std::min<long long>(cap, nt - t0);
-        pieces.push_back({q, t0, n, (long long)n * c->g.info.N});
+      tcsem_ctx *c = b->m[q];
+      HIPCHK(ensure_mid(c));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      const long long org = c->PX + c->xm, fs = c->fstride * c->nslots, ps = c->pstride * c->nslots;
+      T *fm = (T *)c->d_mid, *pl = (T *)c->d_planes;
+      const tcsem_grid_info &I = c->g.info;
+      tiles[q] = pass_tiles_x(I.L) * pass_tiles_y(I.M, pby);
+      for (int p = 0; p < 2; ++p) {
+        const MarchArgs<T> &r = rec[(size_t)p * nm + q];
+        PairArgs<T> pr{};
+        pr.a = r;
+        pr.mex = fm + org;
+        pr.mey = fm + fs + org;
+        pr.mez = fm + 2 * fs + org;
+        pr.mbx = fm + 3 * fs + org;
+        pr.mby = fm + 4 * fs + org;
+        pr.cs2 = pl + 5 * ps + org;
+        pr.cc2 = pl + 6 * ps + org;
+        pr.rows = pby;
+        pr.ntx = pass_tiles_x(I.L);
+        pr.nty = pass_tiles_y(I.M, pby);
+        prec[(size_t)p * nm + q] = pr;
+        MarchArgs<T> ta = r;  // step A's top launch and surface: old state -> intermediate state
+        ta.ex = pr.mex;
+        ta.ey = pr.mey;
+        ta.ez = pr.mez;
+        ta.bx = pr.mbx;
+        ta.by = pr.mby;
+        trec[(size_t)p * nm + q] = ta;
+        MarchArgs<T> tb = r;  // step B's top launch and surface: intermediate state -> new state
+        tb.exi = pr.mex;
+        tb.eyi = pr.mey;
+        tb.ezi = pr.mez;
+        tb.bxi = pr.mbx;
+        tb.byi = pr.mby;
+        tb.cs = pr.cs2;
+        tb.cc = pr.cc2;
+        trec[(size_t)(2 + p) * nm + q] = tb;
       }
     }
-    std::stable_sort(pieces.begin(), pieces.end(), [](const Piece &x, const Piece &y) { return x.work > y.work; });
-    std::vector<std::vector<int2>> lists(8);
-    long long load[8] = {0};
-    for (const Piece &pc : pieces) {
-      const int x = (int)(std::min_element(load, load + 8) - load);
-      for (int t = pc.t0; t < pc.t0 + pc.n; ++t) lists[x].push_back(make_int2(pc.member, t));
-      load[x] += pc.work;
-    }
-    size_t len = 1;
-    for (const auto &l : lists) len = std::max(len, l.size());
-    std::vector<int2> tbl(8 * len, make_int2(-1, -1));
-    for (int x = 0; x < 8; ++x)
-      for (size_t i = 0; i < lists[x].size(); ++i) tbl[8 * i + x] = lists[x][i];
-    if (tbl.size() > b->work_cap) {
-      (void)hipFree(b->d_work);
-      b->d_work = nullptr;
-      b->work_cap = 0;
-      if (hipMalloc((void **)&b->d_work, tbl.size() * sizeof(int2)) != hipSuccess)
-        return fail(TCSEM_ENOMEM, "work list allocation");
-      b->work_cap = tbl.size();
-    }
-    HIPCHK(hipMemcpy(b->d_work, tbl.data(), tbl.size() * sizeof(int2), hipMemcpyHostToDevice));
-    b->work_len = (int)tbl.size();
-    la.work = b->d_work;
-    la.nper = (int)len;  // the march grid: 8 x len workgroups, one row
+    if (int rc = batch_work_list(b, tiles, &b->d_pwork, &b->pwork_cap, &b->pwork_len)) return rc;
+    HIPCHK(hipMemcpy(b->d_pgeo, prec.data(), prec.size() * sizeof(PairArgs<T>), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(b->d_tgeo, trec.data(), trec.size() * sizeof(MarchArgs<T>), hipMemcpyHostToDevice));
+    b->pair_by = pby;
   }
   HIPCHK(hipMemcpy(b->d_geo, rec.data(), rec.size() * sizeof(MarchArgs<T>), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(b->d_ugeo, urec.data(), urec.size() * sizeof(UpcontArgs<T>), hipMemcpyHostToDevice));
@@ -2152,6 +2230,113 @@ int batch_enqueue(tcsem_batch *b, const MarchArgs<T> &la, const UpcontArgs<T> &u
   return TCSEM_OK;
 }
 
+// Two batch steps as one two-step pass (rows k and k+1 of the scalar table, record parity par),
+// then each step's top launch, chain and surface over every member, as do_pair issues them for a
+// context.  live: advance the members' schedules and sample (no sample fires after the first step).
+template <typename T>
+int batch_enqueue_pair(tcsem_batch *b, const MarchArgs<T> &la, const UpcontArgs<T> &ul, int par, int row,
+                       bool live) {
+  const int nm = (int)b->m.size();
+  const MarchArgs<T> *tga = (const MarchArgs<T> *)b->d_tgeo + (size_t)par * nm;
+  const MarchArgs<T> *tgb = (const MarchArgs<T> *)b->d_tgeo + (size_t)(2 + par) * nm;
+  const PairArgs<T> *pg = (const PairArgs<T> *)b->d_pgeo + (size_t)par * nm;
+  const UpcontArgs<T> *ugeo = (const UpcontArgs<T> *)b->d_ugeo;
+  hipStream_t st = b->stream, st2 = b->stream2;
+  PairArgs<T> pl{};
+  pl.a = la;
+  pl.a.work = b->d_pwork;
+  pl.a.pf = 1;
+  pl.a.gscal = b->d_scal + (size_t)row * nm * 4;
+  pl.gscalB = b->d_scal + (size_t)(row + 1) * nm * 4;
+  pl.rows = b->pair_by;
+  pl.nper = b->pwork_len / 8;
+  MarchArgs<T> ta = la, tb = la;
+  ta.gscal = pl.a.gscal;
+  tb.gscal = pl.gscalB;
+  ta.k_hi = 1;
+  ta.k_lo = 1;
+  ta.carry = 1;
+  tb.k_hi = 2;
+  tb.k_lo = 1;
+  tb.carry = 1;
+  const bool timed = b->timing && (b->nenq % kTimingStride == 0 || (b->nenq + 1) % kTimingStride == 0);
+  hipEvent_t te[14] = {};
+  if (timed)
+    for (auto &e : te) {
+      if (b->evused == b->evpool.size()) {
+        hipEvent_t x = nullptr;
+        if (hipEventCreate(&x) != hipSuccess) return fail(TCSEM_EDEVICE, "timing event creation failed");
+        b->evpool.push_back(x);
+      }
+      e = b->evpool[b->evused++];
+    }
+  b->nenq += 2;
+  HIPCHK(launch_march2<T>(pl, 1, st, te[0], te[1], pg));
+  HIPCHK(hipStreamWaitEvent(st, b->xev[1], 0));
+  HIPCHK(launch_march<T>(ta, 1, st, te[2], te[3], tga));
+  HIPCHK(launch_upcont<T>(ul, nm, st, te[4], te[5], ugeo));
+  HIPCHK(launch_surface<T>(ta, nm, st, te[6], te[7], tga));
+  HIPCHK(launch_march<T>(tb, 1, st, te[8], te[9], tgb));
+  HIPCHK(hipEventRecord(b->xev[0], st));
+  HIPCHK(hipStreamWaitEvent(st2, b->xev[0], 0));
+  HIPCHK(launch_upcont<T>(ul, nm, st2, te[10], te[11], ugeo));
+  HIPCHK(launch_surface<T>(tb, nm, st2, te[12], te[13], tgb));
+  if (timed) {
+    b->spans.push_back({0, 0, te[0], te[1]});
+    b->spans.push_back({0, 1, te[2], te[3]});
+    b->spans.push_back({0, 1, te[8], te[9]});
+    b->spans.push_back({2, 1, te[4], te[5]});
+    b->spans.push_back({2, 1, te[10], te[11]});
+    b->spans.push_back({1, 1, te[6], te[7]});
+    b->spans.push_back({1, 1, te[12], te[13]});
+    b->spans.push_back({3, 1, te[0], te[1]});
+  }
+  if (live) {
+    for (int q = 0; q < nm; ++q) {
+      tcsem_ctx *c = b->m[q];
+      c->cur ^= 1;
+      for (int h = 0; h < 2; ++h) {
+        advance_time(c);
+        if (b->active[q])
+          if (int rc = sample_rx<T>(c, st2, false)) return rc;
+      }
+    }
+    ++b->pairs_done;
+  }
+  HIPCHK(hipEventRecord(b->xev[1], st2));
+  return TCSEM_OK;
+}
+
+// Whether the batch's next two steps may run as a pair: every member still sampling is past its
+// source wavelet and takes no receiver sample after the first of the two steps.
+bool batch_pair_ok(tcsem_batch *b) {
+  for (size_t q = 0; q < b->m.size(); ++q) {
+    if (!b->active[q]) continue;
+    tcsem_ctx *c = b->m[q];
+    if (c->n < 2 * (c->setup.n_src / 2) || events_ahead(c, 1)) return false;
+  }
+  return true;
+}
+
+// Rows of the batch's two-step passes: tcsem_batch_set_pairs' choice, else TCSEM_MARCH_PAIRS
+// ("0" or "rows[,1]"), else the autotuner's, else 12-row (FP64: 8-row) passes when the members'
+// pass tiles give at least two workgroups per CU (0: one step at a time); measured on the 64
+// uneven models, FP64 batches ran 6% faster with 8 rows than with 12.
+int batch_pair_rows(tcsem_batch *b) {
+  if (b->pair_set >= 0) return b->pair_set;
+  if (const char *e = std::getenv("TCSEM_MARCH_PAIRS")) {
+    int r = 0, q = 1;
+    if (std::sscanf(e, "%d,%d", &r, &q) >= 1 && (r == 0 || (pair_rows_supported(b->prec, r) && q == 1))) return r;
+  }
+  if (b->pair_tuned >= 0) return b->pair_tuned;
+  const int rows = b->prec == 64 ? 8 : 12;
+  long long tiles = 0;
+  for (size_t q = 0; q < b->m.size(); ++q)
+    if (b->active[q])
+      tiles += (long long)pass_tiles_x(b->m[q]->g.info.L) * pass_tiles_y(b->m[q]->g.info.M, rows);
+  return tiles >= 2LL * device_cus() ? rows : 0;
+}
+
 // Orders the batch's work before what is issued next on the batch's and the members' streams.
 int batch_join(tcsem_batch *b) {
   HIPCHK(hipEventRecord(b->xev[1], b->stream2));
@@ -2219,9 +2404,19 @@ int batch_step_t(tcsem_batch *b, int nsteps) {
   batch_tiling(b, by, pf);
   MarchArgs<T> la;
   UpcontArgs<T> ul;
-  if (int rc = batch_records<T>(b, by, pf, la, ul)) return rc;
-  for (int k = 0; k < nsteps; ++k)
-    if (int rc = batch_enqueue<T>(b, la, ul, k & 1, k, true)) return rc;
+  const int pby = nsteps >= 2 ? batch_pair_rows(b) : 0;
+  if (int rc = batch_records<T>(b, by, pf, la, ul, pby)) return rc;
+  // the record parity flips once per step or pair (a pair leaves its result where one step would)
+  int par = 0;
+  for (int k = 0; k < nsteps; par ^= 1) {
+    if (pby && k + 1 < nsteps && batch_pair_ok(b)) {
+      if (int rc = batch_enqueue_pair<T>(b, la, ul, par, k, true)) return rc;
+      k += 2;
+    } else {
+      if (int rc = batch_enqueue<T>(b, la, ul, par, k, true)) return rc;
+      k += 1;
+    }
+  }
   b->steps += nsteps;
   return batch_join(b);
 }
@@ -2229,7 +2424,8 @@ int batch_step_t(tcsem_batch *b, int nsteps) {
 // Times batch steps (zeroed fields, no source, no sampling) for each rows x prefetch tiling and
 // keeps the fastest; the members' fields are zeroed again afterwards.
 template <typename T>
-int batch_autotune_t(tcsem_batch *b, int nsteps, int *best_by, int *best_pf, int *best_s8, double *best_ms) {
+int batch_autotune_t(tcsem_batch *b, int nsteps, int *best_by, int *best_pf, int *best_s8, int *best_pairs,
+                     double *best_ms) {
   const int nm = (int)b->m.size();
   if (int rc = batch_quiesce(b)) return rc;
   for (int q = 0; q < nm; ++q) b->active[q] = 1;
@@ -2290,6 +2486,56 @@ int batch_autotune_t(tcsem_batch *b, int nsteps, int *best_by, int *best_pf, int
     if (rc) break;
   }
   b->s8 = s80;
+  // two steps per pass with the one-step pick for the top launches: kept when 2% faster per step
+  int pbest = 0;
+  if (rc == TCSEM_OK && *best_by) {
+    b->s8 = *best_s8 != 0;
+    double pms = 1e300;
+    if ((rc = batch_scal_reserve(b, 2)) == TCSEM_OK) {
+      std::vector<double> scal2(scal);
+      scal2.insert(scal2.end(), scal.begin(), scal.end());
+      const hipError_t e = hipMemcpy(b->d_scal, scal2.data(), scal2.size() * sizeof(double), hipMemcpyHostToDevice);
+      if (e != hipSuccess) rc = fail(TCSEM_EDEVICE, "batch autotune: %s", hipGetErrorString(e));
+    }
+    const std::vector<int> prow = b->prec == 64 ? std::vector<int>{12, 8} : std::vector<int>{12, 16, 8};
+    const int npair = std::max(1, nsteps / 2);
+    for (int r : prow) {
+      if (rc) break;
+      MarchArgs<T> la;
+      UpcontArgs<T> ul;
+      if ((rc = batch_records<T>(b, *best_by, *best_pf, la, ul, r))) break;
+      int par = 0;
+      if ((rc = batch_enqueue_pair<T>(b, la, ul, par, 0, false))) break;  // first launches of this shape
+      par ^= 1;
+      hipError_t e = hipStreamWaitEvent(b->stream, b->xev[1], 0);
+      if (e == hipSuccess) e = hipEventRecord(e0, b->stream);
+      for (int k = 0; k < npair && e == hipSuccess && rc == TCSEM_OK; ++k, par ^= 1)
+        rc = batch_enqueue_pair<T>(b, la, ul, par, 0, false);
+      if (rc) break;
+      if (e == hipSuccess) e = hipStreamWaitEvent(b->stream, b->xev[1], 0);
+      if (e == hipSuccess) e = hipEventRecord(e1, b->stream);
+      if (e == hipSuccess) e = hipEventSynchronize(e1);
+      float ms = 0.f;
+      if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+      if (e != hipSuccess) {
+        rc = fail(TCSEM_EDEVICE, "batch autotune: %s", hipGetErrorString(e));
+        break;
+      }
+      const double per = (double)ms / (2.0 * npair);
+      if (per < pms) {
+        pms = per;
+        pbest = r;
+      }
+    }
+    b->s8 = s80;
+    if (rc == TCSEM_OK && pbest && pms < 0.98 * best) {
+      best = pms;
+    } else {
+      pbest = 0;
+    }
+  }
+  *best_pairs = pbest;
+  b->pair_by = 0;  // the records are rebuilt by the next step call
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   b->timing = timing0;
@@ -3507,12 +3753,15 @@ int tcsem_batch_create(tcsem_ctx *const *models, int nmodel, tcsem_batch **out) 
   b->active.assign(nmodel, 1);
   const size_t rsz = b->prec == 64 ? sizeof(MarchArgs<double>) : sizeof(MarchArgs<float>);
   const size_t usz = b->prec == 64 ? sizeof(UpcontArgs<double>) : sizeof(UpcontArgs<float>);
+  const size_t psz = b->prec == 64 ? sizeof(PairArgs<double>) : sizeof(PairArgs<float>);
   bool ok = hipStreamCreate(&b->stream) == hipSuccess && hipStreamCreate(&b->stream2) == hipSuccess &&
             hipEventCreateWithFlags(&b->xev[0], hipEventDisableTiming) == hipSuccess &&
             hipEventCreateWithFlags(&b->xev[1], hipEventDisableTiming) == hipSuccess &&
             hipEventCreateWithFlags(&b->done, hipEventDisableTiming) == hipSuccess &&
             hipMalloc((void **)&b->d_geo, 2 * nmodel * rsz) == hipSuccess &&
-            hipMalloc((void **)&b->d_ugeo, nmodel * usz) == hipSuccess;
+            hipMalloc((void **)&b->d_ugeo, nmodel * usz) == hipSuccess &&
+            hipMalloc((void **)&b->d_pgeo, 2 * nmodel * psz) == hipSuccess &&
+            hipMalloc((void **)&b->d_tgeo, 4 * nmodel * rsz) == hipSuccess;
   if (!ok) {
     tcsem_batch_destroy(b);
     return fail(TCSEM_EDEVICE, "batch stream, event or record allocation failed");
@@ -3529,6 +3778,9 @@ void tcsem_batch_destroy(tcsem_batch *b) {
   (void)hipFree(b->d_ugeo);
   (void)hipFree(b->d_scal);
   (void)hipFree(b->d_work);
+  (void)hipFree(b->d_pgeo);
+  (void)hipFree(b->d_tgeo);
+  (void)hipFree(b->d_pwork);
   for (auto &e : b->xev)
     if (e) (void)hipEventDestroy(e);
   if (b->done) (void)hipEventDestroy(b->done);
@@ -3544,14 +3796,15 @@ int tcsem_batch_autotune(tcsem_batch *b, int nsteps, int *rows, int *pf, double 
   if (nsteps < 1 || nsteps > 1000) return fail(TCSEM_EINVAL, "1 <= steps <= 1000");
   for (const tcsem_ctx *c : b->m)
     if (int rc = batch_fail_if(c->steps != 0, "autotune a batch before its first step")) return rc;
-  int by = 0, p = 0, s8 = 0;
+  int by = 0, p = 0, s8 = 0, pr = 0;
   double t = 0.0;
-  const int rc = b->prec == 64 ? batch_autotune_t<double>(b, nsteps, &by, &p, &s8, &t)
-                               : batch_autotune_t<float>(b, nsteps, &by, &p, &s8, &t);
+  const int rc = b->prec == 64 ? batch_autotune_t<double>(b, nsteps, &by, &p, &s8, &pr, &t)
+                               : batch_autotune_t<float>(b, nsteps, &by, &p, &s8, &pr, &t);
   if (rc) return rc;
   b->by = by;
   b->pf = p;
   b->s8 = s8 != 0;
+  b->pair_tuned = pr;
   if (rows) *rows = by;
   if (pf) *pf = p;
   if (ms) *ms = t;
@@ -3609,10 +3862,34 @@ int tcsem_batch_enable_timing(tcsem_batch *b, int on) {
   if (!b) return fail(TCSEM_EINVAL, "null batch");
   if (int rc = batch_collect(b)) return rc;
   b->timing = on != 0;
-  for (int k = 0; k < 3; ++k) {
+  for (int k = 0; k < 4; ++k) {
     b->tacc[k] = 0;
     b->tn[k] = 0;
   }
+  return TCSEM_OK;
+}
+
+int tcsem_batch_set_pairs(tcsem_batch *b, int rows) {
+  if (!b) return fail(TCSEM_EINVAL, "null batch");
+  if (rows != 0 && !pair_rows_supported(b->prec, rows))
+    return fail(TCSEM_EINVAL, "two-step pass: rows 0 (off), 8, 12 or 16 (FP64: 8 or 12)");
+  b->pair_set = rows;
+  return TCSEM_OK;
+}
+
+int tcsem_batch_pair_info(const tcsem_batch *b, int *rows, int *tuned, long *pairs) {
+  if (!b || !rows || !tuned || !pairs) return fail(TCSEM_EINVAL, "null argument");
+  *rows = b->pair_by;
+  *tuned = b->pair_tuned;
+  *pairs = b->pairs_done;
+  return TCSEM_OK;
+}
+
+int tcsem_batch_pair_timing(tcsem_batch *b, double *ms, long *n) {
+  if (!b || !ms || !n) return fail(TCSEM_EINVAL, "null argument");
+  if (int rc = batch_collect(b)) return rc;
+  *n = b->tn[3];
+  *ms = b->tn[3] ? b->tacc[3] / b->tn[3] : 0.0;
   return TCSEM_OK;
 }
 

@@ -120,6 +120,7 @@ struct PairArgs {
   T *cs2, *cc2;                    // carry planes of step B (plane origin, slot 0)
   double lamB, dtB, dtbB;          // step B's scalars
   int rows, ntx, nty, nper;
+  const double *gscalB;            // model batch, launch record only: step B's row of the scalar table
 };
 
 // Receiver gather descriptor (EFInterp2Rx.cu:60-470) for one field group (E or B) of one event.
@@ -218,7 +219,7 @@ hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEven
 int pair_rows_supported(int prec, int rows);
 template <typename T>
 hipError_t launch_march2(const PairArgs<T> &p, int nslot, hipStream_t s, hipEvent_t e0 = nullptr,
-                         hipEvent_t e1 = nullptr);
+                         hipEvent_t e1 = nullptr, const PairArgs<T> *geo = nullptr);
 template <typename T>
 hipError_t launch_surface(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEvent_t e0 = nullptr,
                           hipEvent_t e1 = nullptr, const MarchArgs<T> *geo = nullptr);

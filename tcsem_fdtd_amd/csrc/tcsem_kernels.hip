@@ -624,12 +624,32 @@ struct Lv2 {
   T bx, by, ex, ey, ez;
   std::conditional_t<S8, unsigned, T> s;
 };
-template <typename T, int BY, int PF, bool S8>
-__global__ void __launch_bounds__(kMarchLanes *BY) k_march2(const PairArgs<T> pa) {
+// GEO: a model batch, one record per member (its fields, tables, sizes, intermediate state, carry
+// planes and pass tiles) and a 1-D grid over the launch's work list of (member, tile) pairs; the
+// launch record la carries the work list and both steps' scalar tables.
+template <typename T, int BY, int PF, bool S8, bool GEO>
+__global__ void __launch_bounds__(kMarchLanes *BY) k_march2(const PairArgs<T> la, const PairArgs<T> *geo) {
+  PairArgs<T> pa;
+  int tile;
+  if constexpr (GEO) {
+    const int2 w = la.a.work[blockIdx.x];
+    if (w.x < 0) return;
+    pa = geo[w.x];
+    const double *qa = la.a.gscal + 4 * w.x, *qb = la.gscalB + 4 * w.x;
+    pa.a.lam = qa[0];
+    pa.a.dt = qa[1];
+    pa.a.dtb = qa[2];
+    pa.lamB = qb[0];
+    pa.dtB = qb[1];
+    pa.dtbB = qb[2];
+    tile = w.y;
+  } else {
+    pa = la;
+    const int b = blockIdx.x;
+    tile = (b & 7) * pa.nper + (b >> 3);  // XCD-contiguous bands of tiles
+    if (tile >= pa.ntx * pa.nty) return;
+  }
   const MarchArgs<T> &a = pa.a;
-  const int b = blockIdx.x;
-  const int tile = (b & 7) * pa.nper + (b >> 3);  // XCD-contiguous bands of tiles
-  if (tile >= pa.ntx * pa.nty) return;
   constexpr int W = kMarchLanes, TX = W - 4, TY = BY - 4;
   extern __shared__ __align__(16) unsigned char zdyn[];  // 3 N entries of T
   __shared__ T sBx[2][BY][W], sBy[2][BY][W], sS[2][BY][W];
@@ -650,7 +670,7 @@ __global__ void __launch_bounds__(kMarchLanes *BY) k_march2(const PairArgs<T> pa
   const unsigned colb = (unsigned)((jc + 1) * a.PX + a.xm + ic) * (unsigned)sizeof(T);
   const int lxm = max(lx - 1, 0), lxp = min(lx + 1, W - 1), lym = max(ly - 1, 0), lyp = min(ly + 1, BY - 1);
   const T rdx = a.rdx[ic + 1], rdx2 = a.rdx2[ic + 1], rdy = a.rdy[jc + 1], rdy2 = a.rdy2[jc + 1];
-  const int slot = (int)blockIdx.y;
+  const int slot = GEO ? 0 : (int)blockIdx.y;  // a model batch's records point at each member's slot 0
   // x-neighbours by lane shifts (FP32), or from LDS (FP64: an x-neighbour from a lane shift lets
   // the compiler fuse the other product of a curl into its FMA, which would round differently
   // from the one-step march; the conductivity sums, which have no products, shift in both)
@@ -1248,41 +1268,52 @@ hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEven
 int pair_rows_supported(int prec, int rows) {
   return rows == 8 || rows == 12 || (rows == 16 && prec == 32) ? rows : 0;
 }
-template <typename T, int BY, int PF, bool S8>
-hipError_t launch_march2_v(const PairArgs<T> &p, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+template <typename T, int BY, int PF, bool S8, bool GEO>
+hipError_t launch_march2_v(const PairArgs<T> &p, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                           const PairArgs<T> *geo) {
   const size_t zl = (size_t)3 * p.a.N * sizeof(T);
   static size_t allowed = 0;  // per instantiation; only grows
   if (zl > allowed) {
-    const hipError_t e = hipFuncSetAttribute((const void *)k_march2<T, BY, PF, S8>,
+    const hipError_t e = hipFuncSetAttribute((const void *)k_march2<T, BY, PF, S8, GEO>,
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)zl);
     if (e != hipSuccess) return e;
     allowed = zl;
   }
-  dim3 blk(kMarchLanes, BY), grd(8 * p.nper, nslot);
-  launch_k(k_march2<T, BY, PF, S8>, grd, blk, (std::uint32_t)zl, s, e0, e1, p);
+  dim3 blk(kMarchLanes, BY), grd(8 * p.nper, GEO ? 1 : nslot);
+  launch_k(k_march2<T, BY, PF, S8, GEO>, grd, blk, (std::uint32_t)zl, s, e0, e1, p, geo);
   return hipGetLastError();
 }
 template <typename T, int BY>
-hipError_t launch_march2_by(const PairArgs<T> &p, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+hipError_t launch_march2_by(const PairArgs<T> &p, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                            const PairArgs<T> *geo) {
   const bool s8 = p.a.sig8 != nullptr;
+  if (geo) {  // model batches run prefetch depth 1
+    if (p.a.pf != 1) return hipErrorInvalidValue;
+    return s8 ? launch_march2_v<T, BY, 1, true, true>(p, nslot, s, e0, e1, geo)
+              : launch_march2_v<T, BY, 1, false, true>(p, nslot, s, e0, e1, geo);
+  }
   switch (p.a.pf) {
     case 1:
-      return s8 ? launch_march2_v<T, BY, 1, true>(p, nslot, s, e0, e1)
-                : launch_march2_v<T, BY, 1, false>(p, nslot, s, e0, e1);
+      return s8 ? launch_march2_v<T, BY, 1, true, false>(p, nslot, s, e0, e1, geo)
+                : launch_march2_v<T, BY, 1, false, false>(p, nslot, s, e0, e1, geo);
     case 2:
-      return s8 ? launch_march2_v<T, BY, 2, true>(p, nslot, s, e0, e1)
-                : launch_march2_v<T, BY, 2, false>(p, nslot, s, e0, e1);
+      return s8 ? launch_march2_v<T, BY, 2, true, false>(p, nslot, s, e0, e1, geo)
+                : launch_march2_v<T, BY, 2, false, false>(p, nslot, s, e0, e1, geo);
   }
   return hipErrorInvalidValue;
 }
+// geo == nullptr: one model (slots in grid y); else a model batch (p: the work list, the scalar
+// tables of both steps, the largest N; geo: one record per member).
 template <typename T>
-hipError_t launch_march2(const PairArgs<T> &p, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-  if (p.a.N < 6 || p.ntx < 1 || p.nty < 1) return hipErrorInvalidValue;
+hipError_t launch_march2(const PairArgs<T> &p, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
+                         const PairArgs<T> *geo) {
+  if (p.a.N < 6 || (!geo && (p.ntx < 1 || p.nty < 1)) || (geo && (!p.a.work || !p.gscalB)))
+    return hipErrorInvalidValue;
   switch (p.rows) {
-    case 8: return launch_march2_by<T, 8>(p, nslot, s, e0, e1);
-    case 12: return launch_march2_by<T, 12>(p, nslot, s, e0, e1);
+    case 8: return launch_march2_by<T, 8>(p, nslot, s, e0, e1, geo);
+    case 12: return launch_march2_by<T, 12>(p, nslot, s, e0, e1, geo);
     case 16:
-      if constexpr (sizeof(T) == 4) return launch_march2_by<T, 16>(p, nslot, s, e0, e1);
+      if constexpr (sizeof(T) == 4) return launch_march2_by<T, 16>(p, nslot, s, e0, e1, geo);
       break;
   }
   return hipErrorInvalidValue;
@@ -1346,8 +1377,10 @@ template hipError_t launch_march<float>(const MarchArgs<float> &, int, hipStream
                                         const MarchArgs<float> *);
 template hipError_t launch_march<double>(const MarchArgs<double> &, int, hipStream_t, hipEvent_t, hipEvent_t,
                                          const MarchArgs<double> *);
-template hipError_t launch_march2<float>(const PairArgs<float> &, int, hipStream_t, hipEvent_t, hipEvent_t);
-template hipError_t launch_march2<double>(const PairArgs<double> &, int, hipStream_t, hipEvent_t, hipEvent_t);
+template hipError_t launch_march2<float>(const PairArgs<float> &, int, hipStream_t, hipEvent_t, hipEvent_t,
+                                         const PairArgs<float> *);
+template hipError_t launch_march2<double>(const PairArgs<double> &, int, hipStream_t, hipEvent_t, hipEvent_t,
+                                          const PairArgs<double> *);
 template hipError_t launch_surface<float>(const MarchArgs<float> &, int, hipStream_t, hipEvent_t, hipEvent_t,
                                           const MarchArgs<float> *);
 template hipError_t launch_surface<double>(const MarchArgs<double> &, int, hipStream_t, hipEvent_t, hipEvent_t,
