@@ -9,7 +9,8 @@ through the source wavelet (the first 2 (n_src/2) steps, which also run the sour
 and the wall time per step.  --pairs rows,pf (or 0) fixes the two-step pass over the library's
 choice.
 --workload uneven runs the uneven-model workload as one model batch (its autotuner, the field
-reset it leaves, then the same warm-up and timed steps).  --every-step-rx replaces the deck's
+reset it leaves, then the same source wavelet, warm-up and timed steps; --pairs rows, or 0, fixes
+the batch's two-step pass).  --every-step-rx replaces the deck's
 sample times by one sample per step from the first step on,
 so that every step runs the receiver gathers of all receivers and components.  Used by the
 rocprof summaries under profiles/.
@@ -99,22 +100,27 @@ def models(T, W, a):
         c.set_transmitters([T.make_tx(**dict(t, txid=i + 1)) for i, t in enumerate(d.txs)])
         ctxs.append(c)
     with T.Batch(ctxs) as b:
+        if a.pairs is not None:
+            b.set_pairs(int(a.pairs.split(",")[0]))
         tiling = None if a.no_tune else b.autotune(10)
         # the autotuner zeroes the members' fields; restart their runs
         for c in ctxs:
             c.reset(0, 1)
-        b.step(a.warmup)
+        lead = max(2 * (d.setup["n_src"] // 2) for d in decks)
+        b.step(lead + a.warmup)
         b.sync()
+        p0 = b.pair_info()[2]
         t0 = time.perf_counter()
         b.step(a.steps)
         b.sync()
         dt = time.perf_counter() - t0
+        rows, _, p1 = b.pair_info()
     cells = sum(c.L * c.M * c.N for c in ctxs)
     for c in ctxs:
         c.close()
     print(json.dumps({"workload": "uneven_models(%d)" % a.models, "precision": a.precision,
                       "steps": a.steps, "ms_per_step": 1000 * dt / a.steps, "cells": cells,
-                      "march_tiling": tiling}), flush=True)
+                      "march_tiling": tiling, "pairs": [rows, p1 - p0] if rows else None}), flush=True)
 
 
 if __name__ == "__main__":
