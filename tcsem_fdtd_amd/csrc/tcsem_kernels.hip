@@ -17,6 +17,10 @@
 // edge lanes fetch the columns either side of the block.
 // Level 0 needs the air fields from the upward continuation of Bz(0), so the march stops at
 // level 1 and writes Bz_old(0); k_surface finishes level 0 after the air planes are ready.
+//
+// Two-step pass (k_march2): on steady-state steps two consecutive steps share one bottom-up pass
+// over levels N-1 .. 3, the second step two levels behind the first, so the state between them
+// stays in registers and LDS; the top levels of both steps run through k_march and k_surface.
 #include <type_traits>
 
 #include "tcsem_internal.h"
@@ -580,8 +584,8 @@ k_march(const MarchArgs<T> la, const MarchArgs<T> *geo) {
 // the intermediate B at m and m-1 and E at m, so it runs two levels behind step A: iteration k
 // takes step A at level k and step B at level k+2, and the intermediate state below level 2 lives
 // in registers and LDS only.  Per cell-update that halves the HBM traffic of the lower levels
-// (read sigma and the old state once, write the new state once per two steps) for 1.6x the
-// lanes of the one-step march's owned cells.
+// (read sigma and the old state once, write the new state once per two steps), for a halo ring
+// two cells wide instead of one.
 // Every quantity is the one-step march's expression in the same operation order, so the two
 // steps give the bits of two one-step passes.  The tile is 64 lanes (columns i0-2 .. i0+61) by
 // BY rows (j0-2 .. j0+BY-3): step A is exact on lanes and rows 1 .. 62 / BY-2 (the neighbours its
@@ -589,8 +593,8 @@ k_march(const MarchArgs<T> la, const MarchArgs<T> *geo) {
 // workgroup owns.  Cells outside the grid read and compute as zero, as memory holds them.
 // Per iteration, three phases with a barrier after the first two (the third runs into the next
 // iteration's first, which touches none of its LDS):
-//   P1  A: Bz sum of level k (B_old of level k from LDS); stage B_old and sigma of level k-1;
-//          loads of level k-1-PF.   B: Bz sum of level k+2 (intermediate B from LDS).
+//   P1  A: Bz sum of level k (B_old of level k: y-neighbours from LDS); stage B_old and sigma of
+//          level k-1; loads of level k-1-PF.   B: Bz sum of level k+2 (intermediate B).
 //   P2  A: intermediate E of level k.   B: new E of level k+2 (stored).
 //   P3  A: intermediate B of level k (into LDS for step B).   B: new B of level k+2 (stored).
 // Step B idles on the first two iterations (levels N+1, N: dz 0, nothing stored, inputs zero)
@@ -598,7 +602,7 @@ k_march(const MarchArgs<T> la, const MarchArgs<T> *geo) {
 // lower pass never reads level 0 of the old state (the previous pair's surface kernel writes it).
 // The same register of the lane to the left (l-1) or right (l+1) in the wavefront, by a DPP
 // whole-wave shift (wave_shr:1, wave_shl:1); the lanes past either end read 0.  The pass's rows
-// are single wavefronts, so its x-neighbours need no LDS round trip (ds reads 21 -> 11 per level).
+// are single wavefronts, so its x-neighbours need no LDS round trip (in FP32, half its LDS reads).
 constexpr int kDppWaveShl1 = 0x130, kDppWaveShr1 = 0x138;
 template <int CTRL>
 __device__ __forceinline__ float dpp_move(float v) {
