@@ -1268,9 +1268,11 @@ hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEven
   }
   return hipErrorInvalidValue;
 }
-// 8, 12 and 16 rows; FP64 not 16 (its registers would spill)
+// 8, 12, 14 and 16 rows (FP32), 8, 11 and 12 (FP64: more rows would spill its registers)
 int pair_rows_supported(int prec, int rows) {
-  return rows == 8 || rows == 12 || (rows == 16 && prec == 32) ? rows : 0;
+  if (rows == 8 || rows == 12) return rows;
+  if (prec == 32) return rows == 14 || rows == 16 ? rows : 0;
+  return rows == 11 ? rows : 0;
 }
 template <typename T, int BY, int PF, bool S8, bool GEO>
 hipError_t launch_march2_v(const PairArgs<T> &p, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
@@ -1315,7 +1317,13 @@ hipError_t launch_march2(const PairArgs<T> &p, int nslot, hipStream_t s, hipEven
     return hipErrorInvalidValue;
   switch (p.rows) {
     case 8: return launch_march2_by<T, 8>(p, nslot, s, e0, e1, geo);
+    case 11:
+      if constexpr (sizeof(T) == 8) return launch_march2_by<T, 11>(p, nslot, s, e0, e1, geo);
+      break;
     case 12: return launch_march2_by<T, 12>(p, nslot, s, e0, e1, geo);
+    case 14:
+      if constexpr (sizeof(T) == 4) return launch_march2_by<T, 14>(p, nslot, s, e0, e1, geo);
+      break;
     case 16:
       if constexpr (sizeof(T) == 4) return launch_march2_by<T, 16>(p, nslot, s, e0, e1, geo);
       break;
