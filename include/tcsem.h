@@ -244,9 +244,10 @@ int tcsem_sigma_table(const tcsem_ctx *ctx);
  * whose first holds no receiver sample as one pass over levels N-1 .. 3 of both steps (the
  * intermediate state stays on chip there), then the top levels of each step as usual.  The
  * results equal, bit for bit, those of one step at a time with an unchunked march (tested).
- * rows 0 turns it off.  Without a call the library decides at tcsem_begin: the autotuner's choice
- * for the batch size, else TCSEM_MARCH_PAIRS ("0" or "rows,pf"), else 12-row passes where they
- * give at least two workgroups per CU; a call holds for the context's later runs.  Graph replays,
+ * rows 0 turns it off.  Without a call the library decides at tcsem_begin: TCSEM_MARCH_PAIRS ("0"
+ * or "rows,pf"), else the autotuner's choice for the batch size, else 12-row passes where they
+ * give at least two workgroups per CU (14-row FP32 or 11-row FP64 passes where those leave fewer
+ * idle CUs in the last round of workgroups); a call holds for the context's later runs.  Graph replays,
  * where on, take precedence.  Needs N >= 6.
  * tcsem_pair_info reports the setting and the pairs taken so far; tcsem_pair_timing the average
  * device time [ms] of the two-step pass alone (timing enabled; its top launches count in

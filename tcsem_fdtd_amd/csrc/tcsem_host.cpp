@@ -619,9 +619,19 @@ void choose_tiling(tcsem_ctx *c, int nbatch) {
   // Two steps per pass: tcsem_set_pairs' choice, else TCSEM_MARCH_PAIRS ("0", or "rows,pf"), else
   // the autotuner's for this batch size, else 12-row passes at prefetch 1 when they give at least
   // two workgroups per CU (faster than one step at a time on every such workload measured; fewer
-  // leave CUs idle, and the one-step march's z-chunks serve those grids better).
+  // leave CUs idle, and the one-step march's z-chunks serve those grids better). The pass holds
+  // one workgroup per CU and a round of them takes about as long as the tile has rows, so 14-row
+  // (FP32) or 11-row (FP64) passes replace the 12-row ones where their rounds x rows is 5% or
+  // more smaller: a plane whose 12-row tiles leave the last round mostly empty (512 x 512).
   int pby = 0, ppf = 1;
-  if ((long long)((L + kMarchLanes - 5) / (kMarchLanes - 4)) * ((M + 7) / 8) * nbatch >= 2LL * cu) pby = 12;
+  auto ptiles = [&](int b) {
+    return (long long)((L + kMarchLanes - 5) / (kMarchLanes - 4)) * ((M + b - 5) / (b - 4)) * nbatch;
+  };
+  if (ptiles(12) >= 2LL * cu) {
+    const int alt = c->prec == 32 ? 14 : 11;
+    auto cost = [&](int b) { return (double)((ptiles(b) + cu - 1) / cu) * b; };
+    pby = cost(alt) <= 0.95 * cost(12) ? alt : 12;
+  }
   if (c->tune_nbatch == nbatch) {
     pby = c->tune_pair_by;
     ppf = c->tune_pair_pf;
