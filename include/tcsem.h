@@ -159,7 +159,7 @@ int tcsem_reset_fields(tcsem_ctx *ctx, int first, int n);
  * z-chunks of the lower launch 1/2/3/4/6/8 as the depth allows, for chunked tilings the source of
  * the chunks' Bz partial sums (a pass over Bx, By, or the chunk sums the previous step's march
  * accumulated), and the 8-bit conductivity index on and off -- then, with that tiling for the top
- * levels, pairs of steps in the two-step pass (rows 8/12/14/16, FP64 8/11/12, prefetch 1-2; tcsem_set_pairs), kept
+ * levels, pairs of steps in the two-step pass (rows 8/10/12/14/16, FP64 8/10/11/12, prefetch 1-2; tcsem_set_pairs), kept
  * when 2% faster per step -- and keeps the choice for later batches of that size
  * (TCSEM_MARCH_ROWS / _PF / _ZCHUNKS / _ZSUM / _PAIRS and tcsem_set_pairs still override it).  It runs on the fields of slots [0, nbatch): a begun run
  * is discarded and the fields are left zeroed.  Outputs (nullable): the chosen rows, prefetch
@@ -239,7 +239,7 @@ long tcsem_graph_replays(const tcsem_ctx *ctx);
 int tcsem_sigma_table(const tcsem_ctx *ctx);
 
 /* Two steps per lower pass (no counterpart in the reference, whose steps each read and write
- * the whole state): with rows 8, 12, 14 or 16 (FP64: 8, 11 or 12; workgroup rows of the pass) and pf 1 or 2 (prefetch
+ * the whole state): with rows 8, 10, 12, 14 or 16 (FP64: 8, 10, 11 or 12; workgroup rows of the pass) and pf 1 or 2 (prefetch
  * depth), tcsem_step and tcsem_run take each two consecutive steps after the source wavelet
  * whose first holds no receiver sample as one pass over levels N-1 .. 3 of both steps (the
  * intermediate state stays on chip there), then the top levels of each step as usual.  The
@@ -266,7 +266,7 @@ int tcsem_pair_timing(const tcsem_ctx *ctx, double *ms, long *n);
  * usual.  A member whose receivers have all been sampled is left out of later calls.
  *   tcsem_batch_autotune: times nsteps batch steps for every rows x prefetch tiling (before the
  *     first step; the members' fields are zeroed again) and keeps the fastest, then pairs of
- *     steps in the two-step pass (rows 8/12/14/16, FP64 8/11/12), kept when 2% faster per step.
+ *     steps in the two-step pass (rows 8/10/12/14/16, FP64 8/10/11/12), kept when 2% faster per step.
  *   tcsem_batch_step: nsteps steps of every member still sampling (per member as tcsem_step).
  *   tcsem_batch_run: steps until every member has taken all samples (max_steps > 0 caps the
  *     member step count; members finish at the end of a 256-step chunk). */
@@ -282,7 +282,7 @@ int tcsem_batch_sync(tcsem_batch *batch);
 int tcsem_batch_enable_timing(tcsem_batch *batch, int on);
 int tcsem_batch_timing(tcsem_batch *batch, double t[3], long n[3]);
 /* Two steps per pass for the batch (no counterpart in the reference), as tcsem_set_pairs for a
- * context: rows 8, 12, 14 or 16 (FP64: 8, 11 or 12) at prefetch depth 1, or 0 (one step at a time), over
+ * context: rows 8, 10, 12, 14 or 16 (FP64: 8, 10, 11 or 12) at prefetch depth 1, or 0 (one step at a time), over
  * the library's choice (TCSEM_MARCH_PAIRS, the batch autotuner's pick, else 12-row passes when
  * the members' pass tiles give at least two workgroups per CU).  Pairs run when every member
  * still sampling is past its source wavelet and takes no sample after the pair's first step.

@@ -399,7 +399,7 @@ class Context:
         return self.L_.tcsem_sigma_table(self.h)
 
     def set_pairs(self, rows, pf=1):
-        """Take two steady-state steps per lower pass (rows 8, 12, 14 or 16 (FP64 8, 11 or 12), prefetch pf 1 or 2; rows 0:
+        """Take two steady-state steps per lower pass (rows 8, 10, 12, 14 or 16 (FP64 8, 10, 11 or 12), prefetch pf 1 or 2; rows 0:
         off), over the library's own choice.  Results equal one step at a time (unchunked), bit for
         bit."""
         _check(self.L_.tcsem_set_pairs(self.h, rows, pf))
@@ -453,7 +453,7 @@ class Batch:
         return {"rows": r.value, "pf": p.value, "pairs": self.pair_info()[1] or None, "step_ms": ms.value}
 
     def set_pairs(self, rows):
-        """Two steps per pass for the batch (rows 8, 12, 14 or 16, FP64 8, 11 or 12; 0: one at a time)."""
+        """Two steps per pass for the batch (rows 8, 10, 12, 14 or 16, FP64 8, 10, 11 or 12; 0: one at a time)."""
         _check(self.L_.tcsem_batch_set_pairs(self.h, rows))
 
     def pair_info(self):

@@ -642,7 +642,7 @@ void choose_tiling(tcsem_ctx *c, int nbatch) {
       pby = r;
       ppf = q;
     } else {
-      std::fprintf(stderr, "tcsem: TCSEM_MARCH_PAIRS=%s is not 0 or rows,pf (rows 8, 12, 14, 16 (FP64: 8, 11, 12); pf 1, 2)\n", e);
+      std::fprintf(stderr, "tcsem: TCSEM_MARCH_PAIRS=%s is not 0 or rows,pf (rows 8, 10, 12, 14, 16 (FP64: 8, 10, 11, 12); pf 1, 2)\n", e);
     }
   }
   if (c->pair_set >= 0) {
@@ -1871,7 +1871,7 @@ int autotune_pairs(tcsem_ctx *c, int nbatch, int nsteps, int *best_by, int *best
   };
   const int npair = std::max(1, nsteps / 2);
   int rc = TCSEM_OK;
-  for (int by : {12, 14, 16, 11, 8})
+  for (int by : {12, 14, 16, 11, 10, 8})
     for (int pf = 1; pf <= 2 && rc == TCSEM_OK; ++pf) {
       if (!pair_rows_supported(c->prec, by)) continue;
       pair_tiling(c, by, pf);
@@ -2507,7 +2507,7 @@ int batch_autotune_t(tcsem_batch *b, int nsteps, int *best_by, int *best_pf, int
       const hipError_t e = hipMemcpy(b->d_scal, scal2.data(), scal2.size() * sizeof(double), hipMemcpyHostToDevice);
       if (e != hipSuccess) rc = fail(TCSEM_EDEVICE, "batch autotune: %s", hipGetErrorString(e));
     }
-    const std::vector<int> prow = b->prec == 64 ? std::vector<int>{12, 11, 8} : std::vector<int>{12, 14, 16, 8};
+    const std::vector<int> prow = b->prec == 64 ? std::vector<int>{12, 11, 10, 8} : std::vector<int>{12, 14, 16, 10, 8};
     const int npair = std::max(1, nsteps / 2);
     for (int r : prow) {
       if (rc) break;
@@ -3719,7 +3719,7 @@ int tcsem_pair_timing(const tcsem_ctx *cc, double *ms, long *n) {
 int tcsem_set_pairs(tcsem_ctx *c, int rows, int pf) {
   if (check_ctx(c)) return TCSEM_EINVAL;
   if (rows != 0 && (!pair_rows_supported(c->prec, rows) || pf < 1 || pf > 2))
-    return fail(TCSEM_EINVAL, "two-step pass: rows 0 (off), 8, 12, 14 or 16 (FP64: 8, 11 or 12); prefetch 1 or 2");
+    return fail(TCSEM_EINVAL, "two-step pass: rows 0 (off), 8, 10, 12, 14 or 16 (FP64: 8, 10, 11 or 12); prefetch 1 or 2");
   if (rows && c->g.info.N < 6) return fail(TCSEM_EINVAL, "two-step pass needs at least 6 levels");
   c->pair_set = rows;
   c->pair_set_pf = pf;
@@ -3882,7 +3882,7 @@ int tcsem_batch_enable_timing(tcsem_batch *b, int on) {
 int tcsem_batch_set_pairs(tcsem_batch *b, int rows) {
   if (!b) return fail(TCSEM_EINVAL, "null batch");
   if (rows != 0 && !pair_rows_supported(b->prec, rows))
-    return fail(TCSEM_EINVAL, "two-step pass: rows 0 (off), 8, 12, 14 or 16 (FP64: 8, 11 or 12)");
+    return fail(TCSEM_EINVAL, "two-step pass: rows 0 (off), 8, 10, 12, 14 or 16 (FP64: 8, 10, 11 or 12)");
   b->pair_set = rows;
   return TCSEM_OK;
 }

@@ -1268,9 +1268,9 @@ hipError_t launch_march(const MarchArgs<T> &a, int nslot, hipStream_t s, hipEven
   }
   return hipErrorInvalidValue;
 }
-// 8, 12, 14 and 16 rows (FP32), 8, 11 and 12 (FP64: more rows would spill its registers)
+// 8, 10, 12, 14 and 16 rows (FP32), 8, 10, 11 and 12 (FP64: more rows would spill its registers)
 int pair_rows_supported(int prec, int rows) {
-  if (rows == 8 || rows == 12) return rows;
+  if (rows == 8 || rows == 10 || rows == 12) return rows;
   if (prec == 32) return rows == 14 || rows == 16 ? rows : 0;
   return rows == 11 ? rows : 0;
 }
@@ -1317,6 +1317,7 @@ hipError_t launch_march2(const PairArgs<T> &p, int nslot, hipStream_t s, hipEven
     return hipErrorInvalidValue;
   switch (p.rows) {
     case 8: return launch_march2_by<T, 8>(p, nslot, s, e0, e1, geo);
+    case 10: return launch_march2_by<T, 10>(p, nslot, s, e0, e1, geo);
     case 11:
       if constexpr (sizeof(T) == 8) return launch_march2_by<T, 11>(p, nslot, s, e0, e1, geo);
       break;

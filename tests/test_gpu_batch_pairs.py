@@ -63,8 +63,9 @@ def _same(got, ref):
         assert np.array_equal(got[1][key], ref[1][key]), key
 
 
-@pytest.mark.parametrize("precision,rows", [(T.FP64, 12), (T.FP64, 8), (T.FP64, 11), (T.FP32, 12),
-                                            (T.FP32, 16), (T.FP32, 14), (T.FP32, 8)])
+@pytest.mark.parametrize("precision,rows", [(T.FP64, 12), (T.FP64, 8), (T.FP64, 11), (T.FP64, 10),
+                                            (T.FP32, 12), (T.FP32, 16), (T.FP32, 14), (T.FP32, 10),
+                                            (T.FP32, 8)])
 def test_batch_pairs_equal_single_steps(precision, rows):
     tns = [0.25] * len(MODELS)
     ref = _batch(precision, 0, tns, max_steps=400)
@@ -101,7 +102,7 @@ def test_batch_pair_settings_are_checked():
         ctxs.append(c)
     try:
         with T.Batch(ctxs) as b:
-            for rows in (16, 14, 10, 4):
+            for rows in (16, 14, 9, 4):
                 with pytest.raises(T.TcsemError):
                     b.set_pairs(rows)
             b.set_pairs(8)

@@ -64,7 +64,8 @@ def _same(a, b):
 @pytest.mark.parametrize("precision,rows,pf,sigma8", [
     (T.FP32, 12, 1, True), (T.FP32, 16, 2, True), (T.FP32, 12, 2, False), (T.FP32, 16, 1, False),
     (T.FP32, 14, 2, True), (T.FP32, 14, 1, False),
-    (T.FP64, 12, 1, True), (T.FP64, 8, 2, False), (T.FP64, 11, 1, True), (T.FP64, 11, 2, False)])
+    (T.FP64, 12, 1, True), (T.FP64, 8, 2, False), (T.FP64, 11, 1, True), (T.FP64, 11, 2, False),
+    (T.FP32, 10, 2, False), (T.FP64, 10, 1, True)])
 def test_pairs_equal_single_steps(monkeypatch, precision, rows, pf, sigma8):
     monkeypatch.setenv("TCSEM_SIGMA8", "1" if sigma8 else "0")
     ref = _run(precision, None)
@@ -110,7 +111,7 @@ def test_pairs_fp64_equal_the_oracle():
 
 def test_pair_settings_are_checked():
     with T.Context(decks.small_setup(T), _cond(dict()), T.FP32) as c:
-        for rows, pf in ((9, 1), (12, 3), (10, 1), (4, 1), (11, 1)):
+        for rows, pf in ((9, 1), (12, 3), (6, 1), (4, 1), (11, 1)):
             with pytest.raises(T.TcsemError):
                 c.set_pairs(rows, pf)
         c.set_pairs(16, 2)
