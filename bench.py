@@ -633,7 +633,8 @@ def main():
                                    "unbatched": {k: round(v, 5) if isinstance(v, float) else v
                                                  for k, v in m32["unbatched"].items()}},
                           "fp64": {"value": round(m64["gcell_s"], 3), "ms_per_step": round(m64["ms_per_step"], 5),
-                                   "roofline": models_roofline(m64, 8), "march_tiling": m64["tiling"],
+                                   "roofline": models_roofline(m64, 8, load_traffic(traffic_key("uneven64_fp64", m64))),
+                                   "march_tiling": m64["tiling"],
                                    "kernel_ms": kern(m64)},
                           "fp32_vs_fp64_ex_rel_l2_after_window": models_err},
         "marine_1d": {"workload": "marine_1d 64x64x48, 1 x-dipole, 21 receivers, complete run (tcsem_run, "
