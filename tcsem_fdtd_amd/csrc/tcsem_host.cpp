@@ -1627,7 +1627,7 @@ hipError_t enqueue_pair(tcsem_ctx *c, const MarchArgs<T> &a, const MarchArgs<T> 
   if ((e = launch_march2<T>(p, nb, st, te[0], te[1])) != hipSuccess) return e;
   if ((e = hipStreamWaitEvent(st, c->xev[1], 0)) != hipSuccess) return e;  // the previous step's stream2 work
   if ((e = launch_march<T>(ta, nb, st, te[2], te[3])) != hipSuccess) return e;
-  if ((e = launch_upcont<T>(u, nb, st, te[4], te[5])) != hipSuccess) return e;
+  if ((e = launch_upcont<T>(u, nb, st, te[4], te[5], nullptr, true)) != hipSuccess) return e;
   if ((e = launch_surface<T>(ta, nb, st, te[6], te[7])) != hipSuccess) return e;
   if ((e = launch_march<T>(tb, nb, st, te[8], te[9])) != hipSuccess) return e;
   if ((e = hipEventRecord(c->xev[0], st)) != hipSuccess) return e;
@@ -2284,7 +2284,7 @@ int batch_enqueue_pair(tcsem_batch *b, const MarchArgs<T> &la, const UpcontArgs<
   HIPCHK(launch_march2<T>(pl, 1, st, te[0], te[1], pg));
   HIPCHK(hipStreamWaitEvent(st, b->xev[1], 0));
   HIPCHK(launch_march<T>(ta, 1, st, te[2], te[3], tga));
-  HIPCHK(launch_upcont<T>(ul, nm, st, te[4], te[5], ugeo));
+  HIPCHK(launch_upcont<T>(ul, nm, st, te[4], te[5], ugeo, true));
   HIPCHK(launch_surface<T>(ta, nm, st, te[6], te[7], tga));
   HIPCHK(launch_march<T>(tb, 1, st, te[8], te[9], tgb));
   HIPCHK(hipEventRecord(b->xev[0], st));

@@ -237,9 +237,12 @@ hipError_t launch_gather(const GatherArgs &g, int is_b, int nslot, const MarchAr
                          hipStream_t s, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 // geo (device array of nslot UpcontArgs records, or null): the air planes of a model batch; u then
 // carries the largest L, M, L1, M1, H and FFT working lengths over the records (tiled chain only).
+// alone: the chain runs with nothing beside it on the GPU (step A's chain in a pair of steps), so
+// the tiled chain takes 256 threads per line instead of the shape sized to share the CUs with a
+// concurrent march.  The results do not depend on it.
 template <typename T>
 hipError_t launch_upcont(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEvent_t e0 = nullptr,
-                         hipEvent_t e1 = nullptr, const UpcontArgs<T> *geo = nullptr);
+                         hipEvent_t e1 = nullptr, const UpcontArgs<T> *geo = nullptr, bool alone = false);
 hipError_t launch_fill_zero(void *p, size_t bytes, hipStream_t s);
 // Work planes of the allocation-free upward continuation: a file-scope device array of
 // kUpcontScratchBytes per device (tcsem_upcont.hip).  upcont_scratch_init looks its address up on

@@ -697,9 +697,15 @@ struct Tile {
   int w, q;
   size_t lds;
 };
+// Threads per line and lines per workgroup.  A chain that shares the CUs with a march takes one
+// wavefront per line once there are many lines, so that it holds few CU slots; a chain with
+// nothing beside it (step A's chain in a pair of steps) takes 256 threads per line, whose
+// shorter per-line latency wins when it has the GPU to itself.  Each line's arithmetic does not
+// depend on the shape.
 template <typename T>
-Tile tile_shape(int m, long long nlines) {
+Tile tile_shape(int m, long long nlines, bool alone) {
   constexpr size_t kHalfCu = 80 * 1024;
+  if (alone) return {256, 1, tile_lds<T>(m, 1)};
   if (nlines >= 1024) {
     if (tile_lds<T>(m, 4) <= kHalfCu) return {64, 4, tile_lds<T>(m, 4)};
     if (tile_lds<T>(m, 2) <= kHalfCu) return {64, 2, tile_lds<T>(m, 2)};
@@ -720,8 +726,8 @@ hipError_t launch_cols_e(const UpcontArgs<T> &u, const Tile &t, int nslot, hipSt
 }
 
 template <typename T, bool GEO>
-hipError_t launch_cols(const UpcontArgs<T> &u, int nslot, hipStream_t s, const UpcontArgs<T> *geo) {
-  const Tile t = tile_shape<T>(u.py.m, (long long)u.H * nslot);
+hipError_t launch_cols(const UpcontArgs<T> &u, int nslot, hipStream_t s, const UpcontArgs<T> *geo, bool alone) {
+  const Tile t = tile_shape<T>(u.py.m, (long long)u.H * nslot, alone);
   const int need = (u.M1 + t.w - 1) / t.w;  // column entries per thread
   if (need <= 1) return launch_cols_e<T, 1, GEO>(u, t, nslot, s, geo);
   if (need <= 2) return launch_cols_e<T, 2, GEO>(u, t, nslot, s, geo);
@@ -732,16 +738,17 @@ hipError_t launch_cols(const UpcontArgs<T> &u, int nslot, hipStream_t s, const U
 }
 
 template <typename T, bool GEO>
-hipError_t launch_tiled(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEvent_t e0, const UpcontArgs<T> *geo) {
+hipError_t launch_tiled(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEvent_t e0, const UpcontArgs<T> *geo,
+                        bool alone) {
   const int npair = (u.M + 1) / 2;
-  Tile t = tile_shape<T>(u.px.m, (long long)npair * nslot);
+  Tile t = tile_shape<T>(u.px.m, (long long)npair * nslot, alone);
   hipError_t e = allow_lds(k_up_rows_fwd<T, GEO>, t.lds);
   if (e != hipSuccess) return e;
   launch_k(k_up_rows_fwd<T, GEO>, dim3((npair + t.q - 1) / t.q, nslot), dim3(t.w, t.q), (std::uint32_t)t.lds, s, e0,
            nullptr, u, geo);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  if ((e = launch_cols<T, GEO>(u, nslot, s, geo)) != hipSuccess) return e;
-  t = tile_shape<T>(u.px.m, (long long)u.nyr * nslot);
+  if ((e = launch_cols<T, GEO>(u, nslot, s, geo, alone)) != hipSuccess) return e;
+  t = tile_shape<T>(u.px.m, (long long)u.nyr * nslot, alone);
   if ((e = allow_lds(k_up_rows_inv<T, GEO>, t.lds)) != hipSuccess) return e;
   hipLaunchKernelGGL((k_up_rows_inv<T, GEO>), dim3((u.nyr + t.q - 1) / t.q, nslot), dim3(t.w, t.q), t.lds, s, u, geo);
   return hipGetLastError();
@@ -774,10 +781,11 @@ hipError_t launch_lined(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEve
 
 template <typename T>
 hipError_t launch_upcont(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEvent_t e0, hipEvent_t e1,
-                         const UpcontArgs<T> *geo) {
+                         const UpcontArgs<T> *geo, bool alone) {
   if (geo && !u.tiled) return hipErrorInvalidValue;  // model batches run the tiled chain only
-  hipError_t e = geo ? launch_tiled<T, true>(u, nslot, s, e0, geo)
-                     : (u.tiled ? launch_tiled<T, false>(u, nslot, s, e0, nullptr) : launch_lined<T>(u, nslot, s, e0));
+  hipError_t e = geo ? launch_tiled<T, true>(u, nslot, s, e0, geo, alone)
+                     : (u.tiled ? launch_tiled<T, false>(u, nslot, s, e0, nullptr, alone)
+                                : launch_lined<T>(u, nslot, s, e0));
   if (e != hipSuccess) return e;
   const int n = u.M * (u.L + 1) + (u.M + 1) * u.L;
   const dim3 grid((n + 255) / 256, nslot), block(256);
@@ -793,9 +801,9 @@ hipError_t launch_upcont(const UpcontArgs<T> &u, int nslot, hipStream_t s, hipEv
 }
 
 template hipError_t launch_upcont<float>(const UpcontArgs<float> &, int, hipStream_t, hipEvent_t, hipEvent_t,
-                                         const UpcontArgs<float> *);
+                                         const UpcontArgs<float> *, bool);
 template hipError_t launch_upcont<double>(const UpcontArgs<double> &, int, hipStream_t, hipEvent_t, hipEvent_t,
-                                          const UpcontArgs<double> *);
+                                          const UpcontArgs<double> *, bool);
 
 // ---- work planes of the allocation-free upward continuation ------------------------------------
 // The tiled chain's work planes (A, G and the regular-plane Bx, By of one slot) for callers that
