@@ -159,8 +159,8 @@ int tcsem_reset_fields(tcsem_ctx *ctx, int first, int n);
  * z-chunks of the lower launch 1/2/3/4/6/8 as the depth allows, for chunked tilings the source of
  * the chunks' Bz partial sums (a pass over Bx, By, or the chunk sums the previous step's march
  * accumulated), and the 8-bit conductivity index on and off -- then, with that tiling for the top
- * levels, pairs of steps in the two-step pass (rows 8/10/12/14/16, FP64 8/10/11/12, prefetch 1-2; tcsem_set_pairs), kept
- * when 2% faster per step -- and keeps the choice for later batches of that size
+ * levels, pairs of steps in the two-step pass (rows 8/10/12/14/16, FP64 8/10/11/12, prefetch 1-2; tcsem_set_pairs;
+ * every shape timed in two interleaved rounds, the faster kept), kept when 2% faster per step -- and keeps the choice for later batches of that size
  * (TCSEM_MARCH_ROWS / _PF / _ZCHUNKS / _ZSUM / _PAIRS and tcsem_set_pairs still override it).  It runs on the fields of slots [0, nbatch): a begun run
  * is discarded and the fields are left zeroed.  Outputs (nullable): the chosen rows, prefetch
  * depth, z-chunks, partial-sum source (1 chunk sums, 0 pass) and the time per step [ms]. */
@@ -266,7 +266,8 @@ int tcsem_pair_timing(const tcsem_ctx *ctx, double *ms, long *n);
  * usual.  A member whose receivers have all been sampled is left out of later calls.
  *   tcsem_batch_autotune: times nsteps batch steps for every rows x prefetch tiling (before the
  *     first step; the members' fields are zeroed again) and keeps the fastest, then pairs of
- *     steps in the two-step pass (rows 8/10/12/14/16, FP64 8/10/11/12), kept when 2% faster per step.
+ *     steps in the two-step pass (rows 8/10/12/14/16, FP64 8/10/11/12; two interleaved rounds), kept
+ *     when 2% faster per step.
  *   tcsem_batch_step: nsteps steps of every member still sampling (per member as tcsem_step).
  *   tcsem_batch_run: steps until every member has taken all samples (max_steps > 0 caps the
  *     member step count; members finish at the end of a 256-step chunk). */

@@ -1844,7 +1844,8 @@ int autotune_march(tcsem_ctx *c, int nbatch, int nsteps, int *best_by, int *best
 // Times pairs of steps as they run in production (the two-step pass, then the top levels of both
 // steps, step B's chain and surface beside the next pass) for each pass shape, with the one-step
 // tiling already applied for the top launches; returns the fastest shape and its time per step
-// (rows 0: none timed).
+// (rows 0: none timed).  The shapes are timed in two interleaved rounds and each keeps its faster
+// time, so that one noisy sample does not decide between shapes a few percent apart.
 template <typename T>
 int autotune_pairs(tcsem_ctx *c, int nbatch, int nsteps, int *best_by, int *best_pf, double *best_ms) {
   *best_by = 0;
@@ -1871,9 +1872,17 @@ int autotune_pairs(tcsem_ctx *c, int nbatch, int nsteps, int *best_by, int *best
   };
   const int npair = std::max(1, nsteps / 2);
   int rc = TCSEM_OK;
+  struct Shape {
+    int by, pf;
+    double ms;
+  };
+  std::vector<Shape> shapes;
   for (int by : {12, 14, 16, 11, 10, 8})
-    for (int pf = 1; pf <= 2 && rc == TCSEM_OK; ++pf) {
-      if (!pair_rows_supported(c->prec, by)) continue;
+    for (int pf = 1; pf <= 2; ++pf)
+      if (pair_rows_supported(c->prec, by)) shapes.push_back({by, pf, 1e300});
+  for (int round = 0; round < 2 && rc == TCSEM_OK; ++round)
+    for (Shape &sh : shapes) {
+      const int by = sh.by, pf = sh.pf;
       pair_tiling(c, by, pf);
       hipError_t e = one_pair();  // the first launches of the shape, untimed
       if (e == hipSuccess) e = hipStreamWaitEvent(st, c->xev[1], 0);
@@ -1888,12 +1897,13 @@ int autotune_pairs(tcsem_ctx *c, int nbatch, int nsteps, int *best_by, int *best
         rc = fail(TCSEM_EDEVICE, "autotune: %s", hipGetErrorString(e));
         break;
       }
-      const double per = (double)ms / (2.0 * npair);
-      if (per < *best_ms) {
-        *best_ms = per;
-        *best_by = by;
-        *best_pf = pf;
-      }
+      sh.ms = std::min(sh.ms, (double)ms / (2.0 * npair));
+    }
+  for (const Shape &sh : shapes)
+    if (rc == TCSEM_OK && sh.ms < *best_ms) {
+      *best_ms = sh.ms;
+      *best_by = sh.by;
+      *best_pf = sh.pf;
     }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
@@ -2509,8 +2519,12 @@ int batch_autotune_t(tcsem_batch *b, int nsteps, int *best_by, int *best_pf, int
     }
     const std::vector<int> prow = b->prec == 64 ? std::vector<int>{12, 11, 10, 8} : std::vector<int>{12, 14, 16, 10, 8};
     const int npair = std::max(1, nsteps / 2);
-    for (int r : prow) {
+    // two interleaved rounds over the pass heights; each keeps its faster time
+    std::vector<double> pm(prow.size(), 1e300);
+    for (int round = 0; round < 2 * (int)prow.size(); ++round) {
       if (rc) break;
+      const size_t ir = round % prow.size();
+      const int r = prow[ir];
       MarchArgs<T> la;
       UpcontArgs<T> ul;
       if ((rc = batch_records<T>(b, *best_by, *best_pf, la, ul, r))) break;
@@ -2531,12 +2545,13 @@ int batch_autotune_t(tcsem_batch *b, int nsteps, int *best_by, int *best_pf, int
         rc = fail(TCSEM_EDEVICE, "batch autotune: %s", hipGetErrorString(e));
         break;
       }
-      const double per = (double)ms / (2.0 * npair);
-      if (per < pms) {
-        pms = per;
-        pbest = r;
-      }
+      pm[ir] = std::min(pm[ir], (double)ms / (2.0 * npair));
     }
+    for (size_t ir = 0; ir < prow.size() && rc == TCSEM_OK; ++ir)
+      if (pm[ir] < pms) {
+        pms = pm[ir];
+        pbest = prow[ir];
+      }
     b->s8 = s80;
     if (rc == TCSEM_OK && pbest && pms < 0.98 * best) {
       best = pms;
