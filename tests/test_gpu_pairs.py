@@ -124,3 +124,21 @@ def test_pair_settings_are_checked():
                 c.set_pairs(rows, 1)
         c.set_pairs(8, 2)
         assert c.pair_info()[:2] == (8, 2)
+
+
+@pytest.mark.parametrize("precision,rows", [(T.FP32, 14), (T.FP64, 11)])
+def test_default_pass_height_follows_wave_quantisation(monkeypatch, precision, rows):
+    """Without a setting or the autotuner, a 512 x 512 plane takes 14-row FP32 (11-row FP64) passes:
+    on MI355X's 256 CUs, at one workgroup per CU, its 468 (666) tiles run in 2 (3) rounds, where
+    the 576 tiles of 12 rows need 3; a plane too small for two workgroups per CU takes none."""
+    monkeypatch.delenv("TCSEM_MARCH_PAIRS", raising=False)
+    for kw, want in ((dict(nx=500, ny=500, nz=8), rows), (dict(), 0)):
+        with T.Context(decks.small_setup(T, **kw), _cond(kw), precision) as c:
+            c.set_receivers(T.make_rxtime(0.02, 0.25, 16, 1, (1, 1, 1, 1, 1, 1)), RX)
+            c.set_transmitters([T.make_tx(**TXS[0])])
+            c.begin(0, 1)
+            assert c.pair_info()[0] == want, (c.L, c.M)
+            if want:
+                assert (c.L, c.M) == (512, 512)
+                c.step(260)  # past the source wavelet: pairs are taken
+                assert c.pair_info()[2] > 0
