@@ -64,7 +64,7 @@ def timed_range(label):
             lib.roctxRangePop()
 
 
-def gpu_run(T, deck, precision, nbatch, steps, warmup, tune=True):
+def gpu_run(T, deck, precision, nbatch, steps, warmup, tune=True, one_step_too=False):
     import numpy as np
     st = T.make_setup(**deck.setup)
     with T.Context(st, deck.cond, precision) as c:
@@ -95,9 +95,23 @@ def gpu_run(T, deck, precision, nbatch, steps, warmup, tune=True):
         rows, pf, pairs1 = c.pair_info()
         c.enable_timing(False)
         ex = c.get(T.EX, 0)
+        one = None
+        if one_step_too and rows:
+            # the same model one step at a time (same one-step tiling, two steps per pass off),
+            # continuing from the timed window
+            c.set_pairs(0, 1)
+            c.step(warmup)
+            c.sync()
+            u0 = time.perf_counter()
+            c.step(steps)
+            c.sync()
+            u1 = time.perf_counter()
+            one = {"value": round(c.L * c.M * c.N * nbatch * steps / (u1 - u0) / 1e9, 3),
+                   "ms_per_step": round(1000.0 * (u1 - u0) / steps, 5)}
     cells = c.L * c.M * c.N
     sec = t1 - t0
     return {
+        "one_step": one,
         "gcell_s": cells * nbatch * steps / sec / 1e9,
         "ms_per_step": 1000.0 * sec / steps,
         "timing": tm, "L": c.L, "M": c.M, "N": c.N, "ex_norm": float(np.linalg.norm(ex)),
@@ -563,7 +577,7 @@ def main():
     # the 256 x 256 x 160 marine model, FP32 and FP64, one transmitter
     deck = W.marine_3d(T)
     r32 = gpu_run(T, deck, T.FP32, 1, args.steps, args.warmup, tune)
-    r64 = gpu_run(T, deck, T.FP64, 1, args.steps, args.warmup, tune)
+    r64 = gpu_run(T, deck, T.FP64, 1, args.steps, args.warmup, tune, one_step_too=True)
     field_err = float(np.linalg.norm(r32["ex"] - r64["ex"]) / max(np.linalg.norm(r64["ex"]), 1e-300))
     r32.pop("ex"), r64.pop("ex")
     # 32 x-dipoles along a towline on the same grid, FP32 batched
@@ -616,7 +630,8 @@ def main():
                                "side_kernels": side_kernels(r32, deck, 4, 1)},
                       "fp64": {"value": round(r64["gcell_s"], 3), "ms_per_step": round(r64["ms_per_step"], 5),
                                "roofline": roofline(r64, 8, 1, load_traffic(traffic_key("marine_3d_fp64", r64))),
-                               "march_tiling": r64["tiling"], "kernel_ms": kern(r64)},
+                               "march_tiling": r64["tiling"], "kernel_ms": kern(r64),
+                               "one_step": r64["one_step"]},
                       "fp32_vs_fp64_ex_rel_l2_after_window": field_err},
         "fp32_batched": {"value": round(rb["gcell_s"], 3), "batch": args.batch,
                          "workload": "towline %d x-dipoles, marine_3d grid" % args.batch,
@@ -644,6 +659,9 @@ def main():
         # 256 x 256 x 160 grid they share, and the large model's FP64 window)
         "ratio_fp32_batched_vs_fp64": round(rb["gcell_s"] / r64["gcell_s"], 3),
         "ratio_fp32_batched_vs_fp64_large_5": round(rb["gcell_s"] / rl64["gcell_s"], 3),
+        # and against the same FP64 model one step at a time, when it takes two steps per pass
+        "ratio_fp32_batched_vs_fp64_one_step": (round(rb["gcell_s"] / r64["one_step"]["value"], 3)
+                                                if r64["one_step"] else None),
     }
     line["cpu_baseline"] = None if args.no_cpu else cpu_baseline(W, T)
     print(json.dumps(line), flush=True)
