@@ -273,8 +273,16 @@ def roofline(res, esz, nbatch, traffic_entry):
 
 
 def traffic_key(key, res):
-    """The counter summary's key for a run: the two-step pass has entries of its own."""
-    return key + "_pairs" if res.get("pairs") else key
+    """The counter summary's key for a run: the two-step pass has entries of its own, one per pass
+    shape where that shape was counted (key_pairs_rows,pf), else one for the workload."""
+    pr = res.get("pairs")
+    if not pr:
+        return key
+    if pr.get("pf"):
+        exact = "%s_pairs_%d,%d" % (key, pr["rows"], pr["pf"])
+        if load_traffic(exact)[0] is not None:
+            return exact
+    return key + "_pairs"
 
 
 def _fft_len(n):
