@@ -80,6 +80,22 @@ def gpu_run(T, deck, precision, nbatch, steps, warmup, tune=True, one_step_too=F
         # untimed: the timed steps are steady-state steps, 99 % of a run's steps
         c.step(source_steps(deck))
         c.step(warmup)
+        one = None
+        rows0, pf0, _ = c.pair_info()
+        if one_step_too and rows0:
+            # the same model one step at a time (same one-step tiling, two steps per pass off),
+            # timed before the window, then the library's choice is restored
+            c.set_pairs(0, 1)
+            c.step(warmup)
+            c.sync()
+            u0 = time.perf_counter()
+            c.step(steps)
+            c.sync()
+            u1 = time.perf_counter()
+            one = {"value": round(c.L * c.M * c.N * nbatch * steps / (u1 - u0) / 1e9, 3),
+                   "ms_per_step": round(1000.0 * (u1 - u0) / steps, 5)}
+            c.set_pairs(rows0, pf0)
+            c.step(warmup)
         c.sync()
         # kernel times come from HIP events recorded on the library's streams inside the timed
         # region (read after it)
@@ -95,19 +111,6 @@ def gpu_run(T, deck, precision, nbatch, steps, warmup, tune=True, one_step_too=F
         rows, pf, pairs1 = c.pair_info()
         c.enable_timing(False)
         ex = c.get(T.EX, 0)
-        one = None
-        if one_step_too and rows:
-            # the same model one step at a time (same one-step tiling, two steps per pass off),
-            # continuing from the timed window
-            c.set_pairs(0, 1)
-            c.step(warmup)
-            c.sync()
-            u0 = time.perf_counter()
-            c.step(steps)
-            c.sync()
-            u1 = time.perf_counter()
-            one = {"value": round(c.L * c.M * c.N * nbatch * steps / (u1 - u0) / 1e9, 3),
-                   "ms_per_step": round(1000.0 * (u1 - u0) / steps, 5)}
     cells = c.L * c.M * c.N
     sec = t1 - t0
     return {
